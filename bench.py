@@ -1,0 +1,137 @@
+"""Benchmark of the solver step on one MI355X: MLUPS and HBM-roofline fraction on the pulsatile Womersley pipe.
+
+Workload: radius 40, length 480 pipe along z (about 2.41 M fluid cells), plug velocity inlet
+u(t) = 0.02 (1 + 0.5 sin(2 pi t / 4000)), pressure outlet at rho = 1, tau = 0.56, regularised BGK with the Galilean
+correction, Bouzidi walls, FP32. A "step" is one solver step over all fluid cells (clock + step kernels).
+
+Prints one JSON line. ``roofline.achieved`` = algorithmic bytes of the step kernel per launch (244 B per fluid cell:
+27 populations read + 27 written x 4 B, previous velocity read 12 B, velocity + density written 16 B) / the kernel's
+mean duration measured with HIP events on the solver's stream. ``cpu_baseline`` times the oracle (the C restatement
+of the reference step, FP32, all host threads) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_LUP_FP32 = 244
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS & % of HBM peak (sparse D3Q27); cases/hour + p99 solve latency"
+
+
+def build_workload():
+    from cerebrosim_amd.geometry import womersley_pipe
+    lat, sc, rc = womersley_pipe(40.0, 480)
+    return lat
+
+
+def make_solver(lat, layout):
+    from cerebrosim_amd import lbm
+    tau = 0.56
+    s = lbm.Solver(lat, (tau - 0.5) / 3, model="regularised", layout=layout)
+    s.set_drive([0.02, 0.0, 0.01], period=4000)
+    s.set_outlet_pressure([0.0])
+    return s
+
+
+def kernel_time_ms(s, launches=40):
+    """Mean duration of the step kernel: HIP events recorded on the solver's stream around each eager launch."""
+    import torch
+    st = s._tstream
+    times = []
+    for _ in range(launches):
+        s._clock()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        s.backend.step(s, s.cur)
+        e1.record(st)
+        s.cur = 1 - s.cur
+        s.steps += 1
+        times.append((e0, e1))
+    st.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in times]))
+
+
+def cpu_baseline(lat, steps):
+    import oracle
+    tau = 0.56
+    threads = oracle.default_threads()
+    o = oracle.OracleSolver(lat, (tau - 0.5) / 3, model="regularised", threads=threads)
+    o.set_drive([0.02, 0.0, 0.01], period=4000)
+    o.set_outlet_pressure([0.0])
+    o.run(2)
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt = time.perf_counter() - t0
+    return {"value": round(o.N * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the same {o.N}-cell pipe (oracle: C restatement of lbm.py k_step, FP32, "
+                      f"{threads} threads; the reference's Warp CPU path cannot run here)"}
+
+
+def load_traffic(layout):
+    path = os.path.join(ROOT, "profiles", f"traffic_step_{layout}_womersley.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--layout", default="indexed")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (the step of one lattice "
+              f"runs on one device; independent cases shard across GPUs without exchange)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    t0 = time.perf_counter()
+    lat = build_workload()
+    N = lat["nbr"].shape[1]
+    print(f"bench: lattice {N} cells built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    s = make_solver(lat, args.layout)
+    s.run(args.warmup)
+    s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(args.steps)
+    s.synchronize()
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    if not s.ok():
+        raise RuntimeError("solver diverged during the benchmark")
+    mlups = N * args.steps / secs / 1e6
+    k_ms = kernel_time_ms(s)
+    achieved = BYTES_PER_LUP_FP32 * N / (k_ms * 1e-3) / 1e9
+    traffic = load_traffic(s.layout)
+    out = {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(secs / args.steps * 1e3, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "womersley_pipe_R40_L480", "cells": N, "tau": 0.56, "layout": s.layout,
+                   "storage": "fp32", "model": "regularised+galilean, bouzidi"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_step", "kernel_ms": round(k_ms, 5), "bytes_per_cell": BYTES_PER_LUP_FP32},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(lat, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,131 @@
+"""ctypes binding of libcerebro_lbm.so (declared in include/cerebro_lbm.h).
+
+The library is built in place by ``__graft_entry__.build()`` (cerebrosim_amd/csrc/Makefile). There is no fallback:
+if the library is missing or fails to load, every solver entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcerebro_lbm.so")
+
+c_int, c_i32, c_i64, c_float, c_void_p = ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_void_p
+CS_MODEL_TRT, CS_MODEL_REGULARISED = 0, 1
+CS_REDUCE_BLOCKS = 512
+CS_REDUCE_WORK = 2 * CS_REDUCE_BLOCKS
+
+
+class ClockArgs(ctypes.Structure):
+    _fields_ = [("tick", c_void_p), ("coef", c_void_p), ("nh", c_i32), ("period", c_i32), ("ramp", c_i32),
+                ("drive", c_void_p)]
+
+
+class StepArgs(ctypes.Structure):
+    _fields_ = [("f_in", c_void_p), ("f_out", c_void_p), ("nbr", c_void_p), ("qw", c_void_p), ("uin", c_void_p),
+                ("inl", c_void_p), ("p_out", c_void_p), ("drive", c_void_p), ("om_cell", c_void_p),
+                ("vel_old", c_void_p), ("vel", c_void_p), ("rho", c_void_p), ("n_cells", c_i64),
+                ("force", c_float * 3), ("magic", c_float), ("sigma", c_float), ("bouzidi", c_i32),
+                ("model", c_i32), ("galilean", c_i32)]
+
+
+class FluxArgs(ctypes.Structure):
+    _fields_ = [("idx", c_void_p), ("w", c_void_p), ("slot", c_void_p), ("m", c_i64), ("vel", c_void_p),
+                ("vel_elem", c_i64), ("vel_comp", c_i64), ("R", c_void_p), ("drive", c_void_p), ("p_out", c_void_p),
+                ("q_last", c_void_p), ("n_open", c_i32), ("pref", c_float), ("alpha", c_float)]
+
+
+class GatherArgs(ctypes.Structure):
+    _fields_ = [("idx", c_void_p), ("slot", c_void_p), ("m", c_i64), ("rho", c_void_p), ("vel", c_void_p),
+                ("vel_elem", c_i64), ("vel_comp", c_i64), ("out", c_void_p), ("umax", c_void_p),
+                ("n_probe", c_i32)]
+
+
+# exported symbol -> argument types (restype is int for all but cs_last_error)
+SIGNATURES = {
+    "cs_last_error": [],
+    "cs_version": [],
+    "cs_stream_create": [ctypes.POINTER(c_void_p)],
+    "cs_stream_destroy": [c_void_p],
+    "cs_stream_synchronize": [c_void_p],
+    "cs_device_synchronize": [],
+    "cs_graph_begin": [c_void_p],
+    "cs_graph_end": [c_void_p, ctypes.POINTER(c_void_p)],
+    "cs_graph_launch": [c_void_p, c_void_p],
+    "cs_graph_destroy": [c_void_p],
+    "cs_clock": [ctypes.POINTER(ClockArgs), c_void_p],
+    "cs_step_indexed": [ctypes.POINTER(StepArgs), c_void_p],
+    "cs_flux_bc": [ctypes.POINTER(FluxArgs), c_void_p],
+    "cs_gather": [ctypes.POINTER(GatherArgs), c_void_p],
+    "cs_reduce_finite": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the library (no device work). Raises ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_char_p if name == "cs_last_error" else c_int
+    _lib = L
+    return L
+
+
+def call(name, *args):
+    """Call a C-ABI function and raise NativeError with the library's message on failure."""
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        msg = load().cs_last_error().decode(errors="replace")
+        raise NativeError(f"{name} returned {rc}: {msg}")
+    return rc
+
+
+class Graph:
+    """Executable graph captured from launches on one stream (owns the handle)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self, stream):
+        call("cs_graph_launch", self.handle, stream)
+
+    def __del__(self):
+        if self.handle is not None and _lib is not None:
+            _lib.cs_graph_destroy(self.handle)
+            self.handle = None
+
+
+class capture:
+    """Context manager: capture every launch enqueued on `stream` into a Graph (``cap.graph``)."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.graph = None
+
+    def __enter__(self):
+        call("cs_graph_begin", self.stream)
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = c_void_p()
+        if exc_type is not None:
+            load().cs_graph_end(self.stream, ctypes.byref(h))   # end the capture, discard the graph
+            if h.value:
+                load().cs_graph_destroy(h)
+            return False
+        call("cs_graph_end", self.stream, ctypes.byref(h))
+        self.graph = Graph(h.value)
+        return False

@@ -1,0 +1,169 @@
+// Small per-step kernels: the drive clock (lbm.py:90-102), the slab-flow reduction with the opening-pressure
+// update (simulate.py:65-82), the probe reduction (simulate.py:85-90) and the finite/mass reduction behind ok()
+// (lbm.py:454-455). The reductions are deterministic: each thread sums a fixed strided subset in order, then a
+// fixed-shape tree in LDS combines the partial sums (the reference's float atomics have no defined order).
+#include <cmath>
+
+#include "common.hpp"
+
+namespace cs {
+namespace {
+
+// ---- k_clock ------------------------------------------------------------------------------------------------
+__global__ void k_clock(cs_clock_args a) {
+    if (threadIdx.x != 0) return;
+    const int t = a.tick[0];
+    const float pi = 3.14159265358979323846f;
+    const float ph = 2.0f * pi * (float)(t % a.period) / (float)a.period;
+    float d = a.coef[0];
+    for (int k = 1; k <= a.nh; ++k) d += a.coef[2 * k - 1] * cosf((float)k * ph) + a.coef[2 * k] * sinf((float)k * ph);
+    if (t < a.ramp) d *= 0.5f - 0.5f * cosf(pi * (float)t / (float)a.ramp);
+    a.drive[0] = d;
+    a.tick[0] = t + 1;
+}
+
+// ---- block-wide sum in a fixed order (1024 threads = 16 waves of 64) ---------------------------------------
+constexpr int kRedThreads = 1024;
+
+template <typename T>
+__device__ T block_sum(T v, T* lds) {
+    // wave-level butterfly (fixed pattern), then the 16 wave totals summed in order by wave 0
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    T total = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kRedThreads / 64; ++k) total += lds[k];
+    __syncthreads();
+    return total;   // valid in thread 0
+}
+
+__device__ float block_max(float v, float* lds) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    float m = 0.0f;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kRedThreads / 64; ++k) m = fmaxf(m, lds[k]);
+    __syncthreads();
+    return m;
+}
+
+// ---- k_flux + k_bc (one workgroup) ----------------------------------------------------------------------------
+constexpr int kMaxOpen = 64;
+
+__global__ __launch_bounds__(kRedThreads) void k_flux_bc(cs_flux_args a) {
+    __shared__ float lds[kRedThreads / 64];
+    __shared__ float qs[kMaxOpen];
+    for (int k = 0; k < a.n_open; ++k) {
+        float acc = 0.0f;
+        for (int64_t t = threadIdx.x; t < a.m; t += kRedThreads) {
+            if (a.slot[t] != k) continue;
+            const int64_t s = a.idx[t];
+            const float* v = a.vel + s * a.vel_elem;
+            acc += v[0] * a.w[3 * t] + v[a.vel_comp] * a.w[3 * t + 1] + v[2 * a.vel_comp] * a.w[3 * t + 2];
+        }
+        const float tot = block_sum(acc, lds);
+        if (threadIdx.x == 0) qs[k] = tot;
+    }
+    if (threadIdx.x == 0) {
+        a.p_out[0] = a.drive[0];
+        for (int k = 0; k < a.n_open; ++k) {
+            a.q_last[k] = qs[k];
+            if (k > 0) a.p_out[k] = a.p_out[k] + a.alpha * (a.R[k] * qs[k] - a.pref - a.p_out[k]);
+        }
+    }
+}
+
+// ---- k_gather (one workgroup) ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(kRedThreads) void k_gather(cs_gather_args a) {
+    __shared__ float lds[kRedThreads / 64];
+    float vmax = 0.0f;
+    for (int64_t t = threadIdx.x; t < a.m; t += kRedThreads) {
+        const int64_t s = a.idx[t];
+        const float* v = a.vel + s * a.vel_elem;
+        const float x = v[0], y = v[a.vel_comp], z = v[2 * a.vel_comp];
+        vmax = fmaxf(vmax, sqrtf(x * x + y * y + z * z));
+    }
+    const float m = block_max(vmax, lds);
+    if (threadIdx.x == 0) a.umax[0] = m;
+    for (int k = 0; k < a.n_probe; ++k) {
+        float acc = 0.0f;
+        for (int64_t t = threadIdx.x; t < a.m; t += kRedThreads)
+            if (a.slot[t] == k) acc += a.rho[a.idx[t]] / 3.0f;
+        const float tot = block_sum(acc, lds);
+        if (threadIdx.x == 0) a.out[k] = tot;
+    }
+}
+
+// ---- finite check + FP64 sum ----------------------------------------------------------------------------------
+__global__ __launch_bounds__(kRedThreads) void k_finite_partial(const float* x, int64_t n, double* work) {
+    __shared__ double lds[kRedThreads / 64];
+    double s = 0.0, bad = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * kRedThreads;
+    for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += stride) {
+        const float v = x[i];
+        if (isfinite(v)) s += (double)v;
+        else bad += 1.0;
+    }
+    const double ts = block_sum(s, lds);
+    const double tb = block_sum(bad, lds);
+    if (threadIdx.x == 0) {
+        work[blockIdx.x] = ts;
+        work[CS_REDUCE_BLOCKS + blockIdx.x] = tb;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_finite_final(const double* work, int nblocks, double* result) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0, b = 0.0;
+    for (int k = 0; k < nblocks; ++k) {
+        s += work[k];
+        b += work[CS_REDUCE_BLOCKS + k];
+    }
+    result[0] = s;
+    result[1] = b;
+}
+
+}  // namespace
+}  // namespace cs
+
+extern "C" int cs_clock(const cs_clock_args* a, void* stream) {
+    CS_REQUIRE(a && a->tick && a->coef && a->drive, "cs_clock: null argument");
+    CS_REQUIRE(a->period > 0 && a->nh >= 0 && a->ramp >= 0, "cs_clock: period %d / nh %d / ramp %d invalid",
+               a->period, a->nh, a->ramp);
+    cs::k_clock<<<1, 64, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_clock");
+    return CS_OK;
+}
+
+extern "C" int cs_flux_bc(const cs_flux_args* a, void* stream) {
+    CS_REQUIRE(a && a->R && a->drive && a->p_out && a->q_last, "cs_flux_bc: null argument");
+    CS_REQUIRE(a->n_open >= 1 && a->n_open <= cs::kMaxOpen, "cs_flux_bc: n_open %d not in [1, 64]", a->n_open);
+    CS_REQUIRE(a->m == 0 || (a->idx && a->w && a->slot && a->vel), "cs_flux_bc: null cell list");
+    cs::k_flux_bc<<<1, cs::kRedThreads, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_flux_bc");
+    return CS_OK;
+}
+
+extern "C" int cs_gather(const cs_gather_args* a, void* stream) {
+    CS_REQUIRE(a && a->out && a->umax, "cs_gather: null argument");
+    CS_REQUIRE(a->n_probe >= 0 && a->n_probe <= cs::kMaxOpen, "cs_gather: n_probe %d not in [0, 64]", a->n_probe);
+    CS_REQUIRE(a->m == 0 || (a->idx && a->slot && a->rho && a->vel), "cs_gather: null cell list");
+    cs::k_gather<<<1, cs::kRedThreads, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_gather");
+    return CS_OK;
+}
+
+extern "C" int cs_reduce_finite(const float* x, int64_t n, double* work, double* result, void* stream) {
+    CS_REQUIRE(x && work && result && n >= 0, "cs_reduce_finite: invalid argument");
+    int64_t want = (n + cs::kRedThreads - 1) / cs::kRedThreads;
+    const int blocks = (int)(want < 1 ? 1 : (want > CS_REDUCE_BLOCKS ? CS_REDUCE_BLOCKS : want));
+    cs::k_finite_partial<<<blocks, cs::kRedThreads, 0, cs::as_stream(stream)>>>(x, n, work);
+    CS_LAUNCHED("k_finite_partial");
+    cs::k_finite_final<<<1, 64, 0, cs::as_stream(stream)>>>(work, blocks, result);
+    CS_LAUNCHED("k_finite_final");
+    return CS_OK;
+}

@@ -1,0 +1,276 @@
+"""Sparse D3Q27 lattice-Boltzmann solver on MI355X -- drop-in for CerebroSim's ``lbm`` module (lbm.py).
+
+Same module surface as the reference: ``LATTICE, C, W, Q, OPP, WALL, INLET, OUTLET0, Units, build_lattice, scatter,
+Solver`` (+ ``synchronize``, the counterpart of ``wp.synchronize`` that the reference's callers use). The solver
+step runs as hand-written HIP kernels for gfx950 through the C ABI of libcerebro_lbm.so; PyTorch-ROCm tensors are
+only device buffers and the stream. There is no CPU execution path: constructing a Solver without a GPU or without
+the built library raises.
+
+Solver semantics follow lbm.py:346-455: per step the drive clock then the step kernel; ``run(steps, chunk)`` replays
+a captured graph of ``chunk`` (made even) steps and launches the remainder one by one; ``set_drive``/``set_force``
+invalidate the graphs; ``set_sponge``/``set_outlet_pressure`` update device buffers in place. ``_launch(a)`` is the
+per-step hook that subclasses extend (simulate.py:109-114 appends its flow/boundary kernels there); it is captured
+into the graphs like the reference's.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .lattice import C, INLET, LATTICE, OPP, OUTLET0, Q, WALL, W, Units, build_lattice, scatter  # noqa: F401
+
+__all__ = ["LATTICE", "C", "W", "Q", "OPP", "WALL", "INLET", "OUTLET0", "Units", "build_lattice", "scatter",
+           "Solver", "synchronize"]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def synchronize():
+    """Wait for all device work (counterpart of wp.synchronize)."""
+    _native.call("cs_device_synchronize")
+
+
+def _require_gpu(device):
+    torch = _torch()
+    _native.load()
+    if not torch.cuda.is_available():
+        raise RuntimeError("cerebrosim_amd.lbm.Solver needs a ROCm GPU (torch.cuda.is_available() is False); "
+                           "there is no CPU execution path")
+    dev = torch.device(device if device is not None else "cuda:0")
+    if dev.type != "cuda":
+        raise RuntimeError(f"device {device!r}: the solver runs on the GPU only")
+    return dev
+
+
+class _IndexedBackend:
+    """Reference layout on the device: f[2] (Q, N), nbr/q (Q, N), uin/vels (N, 3) -- any neighbour table."""
+
+    name = "indexed"
+
+    def __init__(self, s, lat):
+        torch = _torch()
+        dev, N = s.dev, s.N
+        self.f = [torch.zeros((Q, N), dtype=torch.float32, device=dev) for _ in range(2)]
+        self.nbr = torch.from_numpy(np.ascontiguousarray(lat["nbr"], np.int32)).to(dev)
+        self.q = torch.from_numpy(np.ascontiguousarray(lat["q"], np.float32)).to(dev)
+        self.uin = torch.from_numpy(np.ascontiguousarray(lat["uin"], np.float32)).to(dev)
+        self.inl = torch.from_numpy((lat["nbr"] == INLET).any(0).astype(np.int32)).to(dev)
+        self.vels = [torch.zeros((N, 3), dtype=torch.float32, device=dev) for _ in range(2)]
+        self.rho = torch.zeros(N, dtype=torch.float32, device=dev)
+
+    def step(self, s, a):
+        args = _native.StepArgs()
+        args.f_in, args.f_out = self.f[a].data_ptr(), self.f[1 - a].data_ptr()
+        args.nbr, args.qw, args.uin, args.inl = (self.nbr.data_ptr(), self.q.data_ptr(), self.uin.data_ptr(),
+                                                 self.inl.data_ptr())
+        args.p_out, args.drive, args.om_cell = s.p_out.data_ptr(), s.drv.data_ptr(), s.om.data_ptr()
+        args.vel_old, args.vel, args.rho = self.vels[a].data_ptr(), self.vels[1 - a].data_ptr(), self.rho.data_ptr()
+        args.n_cells = s.N
+        args.force = (ctypes.c_float * 3)(*s.force)
+        args.magic, args.sigma = s.magic, s.sigma
+        args.bouzidi, args.model, args.galilean = s.bouzidi, s.model, s.gal
+        _native.call("cs_step_indexed", ctypes.byref(args), s.stream_handle)
+
+    # state access in the reference layout
+    def get_f(self, cur):
+        return self.f[cur].cpu().numpy()
+
+    def set_f(self, cur, f):
+        self.f[cur].copy_(_torch().from_numpy(np.ascontiguousarray(f, np.float32)))
+
+    def velocity(self, cur):
+        return self.vels[cur]
+
+    def velocity_addressing(self):
+        return 3, 1            # (element stride, component stride) in floats
+
+    def slots(self, cells):
+        return np.asarray(cells, np.int32)
+
+    def set_velocity(self, cur, u):
+        self.vels[cur].copy_(_torch().from_numpy(np.ascontiguousarray(u, np.float32)))
+
+
+class Solver:
+    """Lattice Boltzmann on a sparse lattice (lbm.py:346-455). ``model`` "trt" (magic = Lambda) or "regularised";
+    ``sigma`` < 1 blends the regularised stress with the finite-difference one. ``layout`` selects the device
+    storage: "indexed" (reference (Q, N) layout, any neighbour table)."""
+
+    def __init__(self, lat, nu_lu, magic=3.0 / 16.0, bouzidi=True, model="trt", sigma=1.0, galilean=None,
+                 device=None, layout="indexed"):
+        torch = _torch()
+        self.dev = _require_gpu(device)
+        self.lat = lat
+        self.N = N = lat["nbr"].shape[1]
+        self.tau = 3.0 * nu_lu + 0.5
+        self.magic = float(magic)
+        self.bouzidi = int(bool(bouzidi))
+        self.model = {"trt": 0, "regularised": 1}[model]
+        self.sigma = float(sigma)
+        self.gal = int(LATTICE == "D3Q27" if galilean is None else bool(galilean))
+        if self.gal and LATTICE != "D3Q27":
+            raise ValueError("the Galilean correction needs D3Q27")
+        if LATTICE != "D3Q27":
+            raise NotImplementedError("the HIP kernels implement D3Q27 only")
+        dev = self.dev
+        self._tstream = torch.cuda.Stream(device=dev)          # default priority
+        self.stream_handle = ctypes.c_void_p(self._tstream.cuda_stream)
+        with torch.cuda.stream(self._tstream):
+            self.p_out = torch.zeros(max(1, lat["n_outlets"]), dtype=torch.float32, device=dev)
+            self.om = torch.full((N,), 1.0 / self.tau, dtype=torch.float32, device=dev)
+            self.tick = torch.zeros(1, dtype=torch.int32, device=dev)
+            self.drv = torch.zeros(1, dtype=torch.float32, device=dev)
+            self._work = torch.zeros(_native.CS_REDUCE_WORK, dtype=torch.float64, device=dev)
+            self._red = torch.zeros(2, dtype=torch.float64, device=dev)
+            if layout == "indexed":
+                self.backend = _IndexedBackend(self, lat)
+            else:
+                raise ValueError(f"unknown layout {layout!r}")
+        self._tstream.synchronize()
+        self.layout = self.backend.name
+        self.force = (0.0, 0.0, 0.0)
+        self.graphs = {}
+        self.set_drive([1.0])
+        self.cur = 0
+        self.steps = 0
+
+    # --- reference attributes backed by the device state
+    @property
+    def vels(self):
+        return [self.backend.velocity(0), self.backend.velocity(1)]
+
+    @property
+    def rho(self):
+        return self.backend.rho
+
+    @property
+    def f(self):
+        return self.backend.f
+
+    # --- inputs (lbm.py:384-419)
+    def _host_to_device(self, dst, array):
+        torch = _torch()
+        self._tstream.synchronize()
+        with torch.cuda.stream(self._tstream):
+            dst.copy_(torch.from_numpy(np.ascontiguousarray(array)).to(self.dev, non_blocking=False))
+        self._tstream.synchronize()
+
+    def set_drive(self, coef, period=1, ramp=0):
+        """drive(t) = coef[0] + sum_k coef[2k-1] cos(2 pi k t / period) + coef[2k] sin(...) scales the inlet profile
+        and the body force; ramp: smooth start over that many steps. Invalidates the captured graphs."""
+        torch = _torch()
+        coef = np.asarray(coef, np.float32)
+        if len(coef) % 2 == 0:
+            coef = np.append(coef, 0.0).astype(np.float32)
+        self._invalidate()
+        self.coef = torch.from_numpy(coef).to(self.dev)
+        self.nh, self.period, self.ramp = (len(coef) - 1) // 2, int(period), int(ramp)
+
+    def set_force(self, F):
+        self.force = tuple(float(np.float32(x)) for x in F)
+        self._invalidate()
+
+    def _invalidate(self):
+        """Drop the captured graphs (after the stream drained, so none is still executing)."""
+        self._tstream.synchronize()
+        self.graphs = {}
+
+    def set_sponge(self, length, tau_max, kinds=("inlet", "outlet")):
+        """Raise tau smoothly towards the openings of the given kinds: tau_max at the opening plane, back to the
+        fluid's tau `length` cells inside, within 1.5 radii of the opening axis (lbm.py:398-416). Written in place."""
+        lat = self.lat
+        P = lat["origin"] + lat["coords"] * lat["dx"]
+        tau = np.full(self.N, self.tau)
+        for o in lat["openings"]:
+            if o["kind"] not in kinds:
+                continue
+            centre = np.asarray(o["centre"], float)
+            normal = np.asarray(o["normal"], float)
+            rel = P - centre
+            along = rel @ normal
+            depth = -along / lat["dx"]
+            side = np.linalg.norm(rel - along[:, None] * normal, axis=1) < 1.5 * o["radius"]
+            w = np.clip(1.0 - depth / length, 0.0, 1.0)
+            w = w * w * (3.0 - 2.0 * w)
+            tau = np.maximum(tau, np.where(side, self.tau + (tau_max - self.tau) * w, self.tau))
+        self.tau_cell = tau
+        self._host_to_device(self.om, (1.0 / tau).astype(np.float32))
+
+    def set_outlet_pressure(self, p_lu):
+        self._host_to_device(self.p_out, np.asarray(p_lu, np.float32).reshape(-1))
+
+    # --- stepping (lbm.py:422-448)
+    def _clock(self):
+        a = _native.ClockArgs(self.tick.data_ptr(), self.coef.data_ptr(), self.nh, self.period, self.ramp,
+                              self.drv.data_ptr())
+        _native.call("cs_clock", ctypes.byref(a), self.stream_handle)
+
+    def _launch(self, a):
+        self._clock()
+        self.backend.step(self, a)
+
+    def run(self, steps, chunk=200):
+        """Advance `steps` steps: graphs of `chunk` steps (made even), remainder launched step by step."""
+        chunk += chunk % 2
+        full, rest = divmod(int(steps), chunk)
+        if full:
+            key = (self.cur, chunk)
+            if key not in self.graphs:
+                with _native.capture(self.stream_handle) as cap:
+                    a = self.cur
+                    for _ in range(chunk):
+                        self._launch(a)
+                        a = 1 - a
+                self.graphs[key] = cap.graph
+            g = self.graphs[key]
+            for _ in range(full):
+                g.launch(self.stream_handle)
+        for _ in range(rest):
+            self._launch(self.cur)
+            self.cur = 1 - self.cur
+        self.steps += int(steps)
+
+    def synchronize(self):
+        self._tstream.synchronize()
+
+    # --- outputs (lbm.py:450-455)
+    def fields(self):
+        """(pressure, velocity) per cell in lattice units, as numpy, in the lattice's cell order."""
+        self.synchronize()
+        p, u = self.backend.fields(self.cur) if hasattr(self.backend, "fields") else (
+            self.backend.rho.cpu().numpy(), self.backend.velocity(self.cur).cpu().numpy())
+        return p / 3.0, u
+
+    def ok(self):
+        """All densities finite (device reduction; reads back 16 bytes)."""
+        x = self.backend.rho
+        _native.call("cs_reduce_finite", ctypes.c_void_p(x.data_ptr()), x.numel(),
+                     ctypes.c_void_p(self._work.data_ptr()), ctypes.c_void_p(self._red.data_ptr()),
+                     self.stream_handle)
+        self.synchronize()
+        return bool(self._red[1].item() == 0)
+
+    # --- state access beyond the reference API (parity tests, fault injection, snapshots)
+    def get_f(self):
+        """Newest post-collision populations f~ (Q, N) in the reference layout, as numpy."""
+        self.synchronize()
+        return self.backend.get_f(self.cur)
+
+    def set_f(self, f):
+        """Replace the newest post-collision populations (Q, N), reference layout."""
+        self.synchronize()
+        with _torch().cuda.stream(self._tstream):
+            self.backend.set_f(self.cur, f)
+        self.synchronize()
+
+    def set_velocity(self, u):
+        """Replace the newest velocity field (N, 3) (the previous-step velocity of the next step)."""
+        self.synchronize()
+        with _torch().cuda.stream(self._tstream):
+            self.backend.set_velocity(self.cur, u)
+        self.synchronize()

@@ -1,0 +1,155 @@
+/* cerebro_lbm.h -- C ABI of the MI355X (gfx950) sparse D3Q27 lattice-Boltzmann solver step.
+ *
+ * The reference (CerebroSim, Python + NVIDIA Warp) has no native library: its solver step is a set of Warp
+ * kernels launched from Python with wp.launch and replayed from captured CUDA graphs. Each entry point below
+ * replaces one of those launches or Warp runtime calls; the citation names the reference interface it stands in
+ * for. Everything is plain C: device buffers are raw pointers (the Python layer allocates them as PyTorch-ROCm
+ * tensors and keeps them alive), streams and graphs are opaque handles, sizes are int64.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative CS_E* code; cs_last_error() then returns a message
+ *    describing the failure (thread-local, valid until the next failing call on the same thread).
+ *  - Launch functions only enqueue work on the given stream; nothing synchronises unless the name says so.
+ *  - Device pointers must stay valid for as long as any graph captured with them may be replayed.
+ *  - Populations are stored as deviations from the rest weights, f~_i = f_i - w_i (reference lbm.py:23).
+ *  - Link codes in neighbour tables: >= 0 source cell, -1 wall, -2 inlet, -3 - k outlet k (lbm.py:50).
+ */
+#ifndef CEREBRO_LBM_H
+#define CEREBRO_LBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS_OK 0
+#define CS_E_ARG (-1)       /* invalid argument (null pointer, bad size, unsupported option) */
+#define CS_E_HIP (-2)       /* a HIP runtime call failed */
+#define CS_E_LAYOUT (-3)    /* lattice not representable in the requested layout */
+#define CS_E_NOMEM (-4)     /* host allocation failed */
+
+#define CS_MODEL_TRT 0          /* lbm.py:359 "trt" */
+#define CS_MODEL_REGULARISED 1  /* lbm.py:359 "regularised" */
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* cs_last_error(void);
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int cs_version(void);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Streams and graphs -- replace wp.ScopedCapture / wp.capture_launch (lbm.py:437-444) and wp.synchronize
+ * (validate.py:262, 323, 326). Streams are created at the default priority.
+ * ------------------------------------------------------------------------------------------------------------- */
+int cs_stream_create(void** stream);
+int cs_stream_destroy(void* stream);
+int cs_stream_synchronize(void* stream);
+int cs_device_synchronize(void);
+/* Begin capturing every launch enqueued on `stream` (thread-local capture mode). */
+int cs_graph_begin(void* stream);
+/* End the capture and instantiate it; *graph receives an executable graph handle. */
+int cs_graph_end(void* stream, void** graph);
+int cs_graph_launch(void* graph, void* stream);
+int cs_graph_destroy(void* graph);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Drive clock -- replaces k_clock (lbm.py:90-102), launched once per step before the step kernel (lbm.py:423).
+ * drive[0] = coef[0] + sum_k coef[2k-1] cos(k phi) + coef[2k] sin(k phi), phi = 2 pi (t mod period) / period,
+ * times (1/2 - 1/2 cos(pi t / ramp)) while t < ramp; then tick[0] = t + 1.
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct cs_clock_args {
+    int32_t* tick;       /* device, 1 int */
+    const float* coef;   /* device, 2 nh + 1 floats: a0, a1, b1, a2, b2, ... */
+    int32_t nh;
+    int32_t period;
+    int32_t ramp;
+    float* drive;        /* device, 1 float */
+} cs_clock_args;
+int cs_clock(const cs_clock_args* a, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Solver step, indexed layout -- replaces k_step (lbm.py:140-257) launched at lbm.py:425 with dim = N.
+ * Same arrays as the reference: f_in/f_out (Q, N) row-major, nbr/qw (Q, N), uin/vel_old/vel (N, 3) float AoS,
+ * inl/om_cell/rho (N). Works for any neighbour table (any cell order, periodic wrap, hand-made lattices).
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct cs_step_args {
+    const float* f_in;
+    float* f_out;
+    const int32_t* nbr;
+    const float* qw;
+    const float* uin;
+    const int32_t* inl;
+    const float* p_out;
+    const float* drive;     /* device, 1 float, written by cs_clock */
+    const float* om_cell;   /* per-cell 1 / tau */
+    const float* vel_old;
+    float* vel;
+    float* rho;
+    int64_t n_cells;
+    float force[3];         /* body force per unit drive (lbm.py:394-396) */
+    float magic;            /* TRT magic parameter Lambda */
+    float sigma;            /* hybrid blend; 1 = pure regularised */
+    int32_t bouzidi;        /* 1 = interpolated walls, 0 = half-way bounce-back */
+    int32_t model;          /* CS_MODEL_* */
+    int32_t galilean;       /* 1 = third-order equilibrium and diagonal stress correction (D3Q27) */
+} cs_step_args;
+int cs_step_indexed(const cs_step_args* a, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Per-step flow measurement and boundary update -- replaces k_flux + k_bc (simulate.py:65-82) launched after each
+ * step at simulate.py:111-114. Q[slot] = sum over the M slab cells of vel[idx] . w (deterministic order), then
+ * p_out[0] = drive[0]; for k in 1..n_open-1: p_out[k] += alpha (R[k] Q[k] - pref - p_out[k]); q_last = Q.
+ * Cells are given as storage slots of the layout in use; component c of slot s is vel[s * vel_elem + c * vel_comp]
+ * ((N, 3) AoS: vel_elem = 3, vel_comp = 1; component planes of S slots: vel_elem = 1, vel_comp = S).
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct cs_flux_args {
+    const int32_t* idx;      /* device, M storage slots */
+    const float* w;          /* device, (M, 3) weights: +-normal / slab thickness */
+    const int32_t* slot;     /* device, M opening index per cell */
+    int64_t m;
+    const float* vel;        /* velocity of the step just taken */
+    int64_t vel_elem;
+    int64_t vel_comp;
+    const float* R;          /* device, n_open resistances (lattice units) */
+    const float* drive;      /* device, 1 float */
+    float* p_out;            /* device, n_open opening pressures */
+    float* q_last;           /* device, n_open flows of this step */
+    int32_t n_open;          /* at most 64 */
+    float pref;
+    float alpha;
+} cs_flux_args;
+int cs_flux_bc(const cs_flux_args* a, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Probe reduction -- replaces k_gather (simulate.py:85-90) launched at simulate.py:309: out[slot] = sum rho / 3
+ * over each probe's cells and umax = max |u|, both written (not accumulated), deterministic order.
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct cs_gather_args {
+    const int32_t* idx;      /* device, M storage slots */
+    const int32_t* slot;     /* device, M probe index per cell */
+    int64_t m;
+    const float* rho;
+    const float* vel;
+    int64_t vel_elem;
+    int64_t vel_comp;
+    float* out;              /* device, n_probe sums */
+    float* umax;             /* device, 1 float */
+    int32_t n_probe;         /* at most 64 */
+} cs_gather_args;
+int cs_gather(const cs_gather_args* a, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Health check -- replaces Solver.ok() (lbm.py:454-455), which copies rho to the host and tests isfinite. A
+ * deterministic two-pass device reduction over x[0..n) writes result[0] = sum of the finite values (FP64) and
+ * result[1] = number of non-finite values, so ok() reads back 16 bytes instead of 4 N. `work` is caller-provided
+ * device scratch of CS_REDUCE_WORK doubles; result is device memory of 2 doubles.
+ * ------------------------------------------------------------------------------------------------------------- */
+#define CS_REDUCE_BLOCKS 512
+#define CS_REDUCE_WORK (2 * CS_REDUCE_BLOCKS)
+int cs_reduce_finite(const float* x, int64_t n, double* work, double* result, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CEREBRO_LBM_H */

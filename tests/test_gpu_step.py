@@ -1,0 +1,136 @@
+"""GPU parity of the solver step against the CPU oracle and the frozen golden vectors, plus the reference's published
+physics results reproduced through the GPU path. Tolerance for FP32 parity: relative L2 of f, rho and u <= 1e-5."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from cerebrosim_amd import lbm
+from cerebrosim_amd.geometry import open_pipe, periodic_pipe, poiseuille_box
+from cerebrosim_amd.validation import l2, open_pipe_case, poiseuille_exact
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LAYOUTS = ["indexed"]
+TOL = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def golden():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_fixtures", os.path.join(HERE, "golden", "make_fixtures.py"))
+    mf = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mf)
+    return mf, np.load(os.path.join(HERE, "golden", "pipe_small.npz"))
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("case", ["reg", "trt", "hyb", "halfway"])
+def test_golden_vectors(layout, case):
+    mf, g = golden()
+    lat = mf.lattice()
+    assert np.array_equal(lat["nbr"], g["nbr"]) and np.array_equal(lat["q"], g["q"])
+    model, sigma, bz, steps = mf.CASES[case]
+    s = lbm.Solver(lat, mf.NU, model=model, sigma=sigma, bouzidi=bz, layout=layout)
+    mf.setup(s)
+    s.set_f(g["f0"])
+    s.set_velocity(g["u0"])
+    done = 0
+    for n in steps:
+        s.run(n - done)
+        done = n
+        p, u = s.fields()
+        errs = (rel(s.get_f(), g[f"{case}_f_{n}"]), rel(3 * p, g[f"{case}_rho_{n}"]), rel(u, g[f"{case}_u_{n}"]))
+        print(f"{layout} {case} step {n}: rel L2 f/rho/u {errs}", flush=True)
+        assert max(errs) <= TOL, errs
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_poiseuille_box_10000_steps_vs_oracle(layout):
+    """Steady Poiseuille workload: R = 10 cylinder in a 24 x 24 x 64 periodic box, tau = 0.8, body force 1e-6,
+    10,000 steps from rest; GPU vs oracle, and the profile vs the parabola."""
+    lat, r, R = poiseuille_box()
+    nu = (0.8 - 0.5) / 3
+    s = lbm.Solver(lat, nu, model="regularised", layout=layout)
+    o = oracle.OracleSolver(lat, nu, model="regularised")
+    for x in (s, o):
+        x.set_force([0, 0, 1e-6])
+    s.run(10000)
+    o.run(10000)
+    p, u = s.fields()
+    po, uo = o.fields()
+    errs = (rel(s.get_f(), o.get_f()), rel(p, po), rel(u, uo))
+    print(f"{layout} poiseuille box: rel L2 f/rho/u {errs}", flush=True)
+    assert max(errs) <= TOL, errs
+    ua = poiseuille_exact(r, R, 1e-6, nu)
+    assert l2(u[:, 2], ua) < 0.05
+    assert s.ok()
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_graph_replay_equals_eager(layout):
+    lat, _, _, _ = open_pipe((0, 0, 1), 12, lambda s: np.full_like(np.asarray(s, float), 4.0))
+    runs = []
+    for chunk in (10, 1000):               # 40 steps: 4 graph replays / 40 eager launches
+        s = lbm.Solver(lat, 0.05, model="regularised", layout=layout)
+        s.set_drive([0.02, 0.0, 0.01], period=40)
+        s.set_outlet_pressure([0.0])
+        s.run(40, chunk=chunk)
+        runs.append((s.get_f(), s.fields()[1], s.cur, s.steps))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2:] == runs[1][2:]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_ok_detects_non_finite(layout):
+    lat, _, _ = periodic_pipe(8)
+    s = lbm.Solver(lat, 0.1, model="regularised", layout=layout)
+    s.run(4)
+    assert s.ok()
+    f = s.get_f()
+    f[3, 5] = np.nan
+    s.set_f(f)
+    s.run(2)
+    assert not s.ok()
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_poiseuille_d20_published(layout, nu_healthy):
+    """Reference validation at its operating point (tau 0.50495): published L2 0.034291, flow error -0.038034
+    (geometry/solver.json:28-34). FP32 rounding-style band at this tau: 3e-3 relative."""
+    lat, r, R = periodic_pipe(20)
+    s = lbm.Solver(lat, nu_healthy, model="regularised", layout=layout)
+    F = 4 * nu_healthy * 0.02 / R ** 2
+    s.set_force([0, 0, F])
+    s.run(int(14 * R * R / (5.78 * nu_healthy)))
+    _, u = s.fields()
+    ua = poiseuille_exact(r, R, F, nu_healthy)
+    e, fl = l2(u[:, 2], ua), float(u[:, 2].sum() / ua.sum() - 1)
+    print(f"{layout} poiseuille D20: l2 {e} flow {fl}", flush=True)
+    assert abs(e / 0.03429092846606971 - 1) < 4e-3
+    assert abs(fl - -0.03803448455849523) < 2e-4
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("name,axis,ref", [
+    ("straight", (0, 0, 1), {"grad_err": 0.08768536633211488, "profile_l2": 0.01247263825422971,
+                             "flow_vs_inlet": -0.001167730051564786}),
+    ("tilted", (1, 2, 2), {"grad_err": 0.058565768905948445, "profile_l2": 0.02435777993182073,
+                           "flow_vs_inlet": -0.002742054870268462}),
+])
+def test_open_pipe_published(layout, name, axis, ref):
+    """Velocity inlet + pressure outlet + sponge, 60,000 steps (geometry/solver.json:70-112). Exercises inlet cells,
+    outlet anti-bounce-back, Bouzidi walls at oblique angles, the third-order equilibrium and the Galilean term."""
+    res = open_pipe_case(lambda lat, nu: lbm.Solver(lat, nu, model="regularised", layout=layout), axis)
+    print(f"{layout} open pipe {name}: {res}", flush=True)
+    assert res["ok"] and res["steps"] == 60000
+    assert abs(res["grad_err"] - ref["grad_err"]) < 2e-3
+    assert abs(res["profile_l2"] - ref["profile_l2"]) < 5e-4
+    assert abs(res["flow_vs_inlet"] - ref["flow_vs_inlet"]) < 5e-4

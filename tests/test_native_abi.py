@@ -1,0 +1,81 @@
+"""The C-ABI library loads without a GPU, exports every function declared in include/*.h, and the ctypes structures
+match the C header's layout (sizes and field offsets printed by a host C program compiled against the header)."""
+import ctypes
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+from cerebrosim_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(cs_[A-Za-z0-9_]+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_headers_found():
+    assert HEADERS, "no include/*.h"
+    assert "cs_step_indexed" in declared_functions()
+
+
+def test_library_exports_every_declared_function():
+    lib = _native.load()
+    missing = [n for n in sorted(declared_functions()) if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python binding knows every declared function
+    assert declared_functions() <= set(_native.SIGNATURES), declared_functions() - set(_native.SIGNATURES)
+
+
+def test_error_reporting_without_gpu():
+    lib = _native.load()
+    assert lib.cs_version() >= 10000
+    a = _native.StepArgs()                      # all-null pointers: rejected before any device work
+    assert lib.cs_step_indexed(ctypes.byref(a), None) == -1
+    assert b"cs_step_indexed" in lib.cs_last_error()
+    with pytest.raises(_native.NativeError):
+        _native.call("cs_step_indexed", ctypes.byref(a), None)
+
+
+STRUCTS = {"cs_clock_args": _native.ClockArgs, "cs_step_args": _native.StepArgs, "cs_flux_args": _native.FluxArgs,
+           "cs_gather_args": _native.GatherArgs}
+
+
+def test_ctypes_layout_matches_header(tmp_path):
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "cerebro_lbm.h"', "int main(void) {"]
+    for cname, py in STRUCTS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{cname}.{field} %zu\\n", offsetof({cname}, {field}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(line.rsplit(" ", 1) for line in out if line)
+    for cname, py in STRUCTS.items():
+        assert int(got[f"{cname} size"]) == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert int(got[f"{cname}.{field}"]) == getattr(py, field).offset, (cname, field)
+
+
+def test_solver_refuses_cpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from cerebrosim_amd import lbm
+    from cerebrosim_amd.geometry import periodic_pipe
+    lat, _, _ = periodic_pipe(6)
+    with pytest.raises(RuntimeError, match="GPU"):
+        lbm.Solver(lat, 0.1)

@@ -1,0 +1,282 @@
+"""Pins the CPU oracle: closed forms, conservation, boundary special cases, moment properties of the third-order and
+Galilean terms, and the reference's published physics results (geometry/solver.json)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from cerebrosim_amd import lattice as L
+from cerebrosim_amd.geometry import open_pipe, periodic_pipe
+from cerebrosim_amd.validation import l2, poiseuille_exact, womersley_exact
+
+C = L.C.astype(np.float64)
+W = L.W
+
+
+def periodic_box(n=6):
+    phi = -np.ones((n, n, n), np.float32)
+    return L.build_lattice(phi, [0, 0, 0], 1.0, periodic=(0, 1, 2))
+
+
+def small_pipe():
+    lat, sc, rc, ax = open_pipe((0, 0, 1), 8, lambda s: np.full_like(np.asarray(s, float), 3.0))
+    return lat
+
+
+def random_state(o, rng, amp=1e-3):
+    """Post-collision populations and previous velocities with smooth-ish random content."""
+    N = o.N
+    o.f[o.cur][:] = (amp * rng.standard_normal((27, N))).astype(o.dtype)
+    o.vels[o.cur][:] = (0.02 * rng.standard_normal((N, 3))).astype(o.dtype)
+
+
+# --- k_clock --------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("t,period,ramp", [(0, 1, 0), (7, 40, 0), (123, 400, 1000), (5000, 4000, 100)])
+def test_clock_closed_form(t, period, ramp):
+    coef = np.array([0.03, 0.01, -0.02, 0.004, 0.0015], np.float32)
+    nh = 2
+    d32, t1 = oracle.clock(t, coef, nh, period, ramp)
+    d64, _ = oracle.clock(t, coef, nh, period, ramp, dtype=np.float64)
+    ph = 2 * math.pi * (t % period) / period
+    coef = coef.astype(np.float64)
+    exact = coef[0] + sum(coef[2 * k - 1] * math.cos(k * ph) + coef[2 * k] * math.sin(k * ph) for k in (1, 2))
+    if t < ramp:
+        exact *= 0.5 - 0.5 * math.cos(math.pi * t / ramp)
+    assert t1 == t + 1
+    assert abs(d64 - exact) < 1e-12
+    assert abs(d32 - exact) < 2e-7 * max(1.0, abs(exact)) + 1e-8
+
+
+# --- conservation of one collision -----------------------------------------------------------------------------
+@pytest.mark.parametrize("model", ["regularised", "trt"])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 2e-6)])
+def test_mass_and_momentum_conserved(model, dtype, tol):
+    lat = periodic_box()
+    o = oracle.OracleSolver(lat, 0.02, model=model, dtype=dtype, threads=1)
+    rng = np.random.default_rng(3)
+    random_state(o, rng)
+    F = np.array([1e-4, -2e-4, 3e-4])
+    o.set_force(F)
+    f_in = o.f[o.cur].astype(np.float64).copy()
+    o.step_once(1.0)
+    f_out = o.f[o.cur].astype(np.float64)
+    # streamed (pulled) populations of a fully periodic box: f_s[i, n] = f_in[i, nbr[i, n]]
+    f_s = np.take_along_axis(f_in, lat["nbr"].astype(np.int64), axis=1)
+    dr = f_s.sum(0)
+    j = C.T @ f_s
+    Fr = np.asarray(F, np.float32).astype(np.float64)
+    scale = np.abs(f_in).max()
+    assert np.abs(f_out.sum(0) - dr).max() < tol * 30 * scale
+    assert np.abs(C.T @ f_out - (j + Fr[:, None])).max() < tol * 30 * scale
+    assert np.abs(o.rho.astype(np.float64) - dr).max() < tol * 30 * scale
+
+
+def test_rest_state_stays_zero():
+    lat = small_pipe()
+    for model in ("regularised", "trt"):
+        o = oracle.OracleSolver(lat, 0.05, model=model, threads=1)
+        o.set_outlet_pressure([0.0])
+        o.set_drive([0.0])                      # inlet velocity x 0
+        o.run(25)
+        assert not np.any(o.get_f()) and not np.any(o.vels[o.cur]) and not np.any(o.rho)
+
+
+# --- boundary links: special cases that reduce to simpler rules --------------------------------------------------
+def test_bouzidi_half_equals_halfway():
+    lat = small_pipe()
+    lat_half = dict(lat)
+    lat_half["q"] = np.full_like(lat["q"], 0.5)
+    rng = np.random.default_rng(5)
+    a = oracle.OracleSolver(lat_half, 0.05, bouzidi=True, model="regularised", threads=1)
+    b = oracle.OracleSolver(lat, 0.05, bouzidi=False, model="regularised", threads=1)
+    b.q[:] = 0.5                                 # the finite-difference wall distance reads q too
+    random_state(a, rng)
+    b.f[b.cur][:] = a.f[a.cur]
+    b.vels[b.cur][:] = a.vels[a.cur]
+    a.run(3)
+    b.run(3)
+    assert np.array_equal(a.get_f(), b.get_f())
+
+
+def _with_codes_as_walls(lat, code_test):
+    nbr = lat["nbr"].copy()
+    nbr[code_test(nbr)] = L.WALL
+    out = dict(lat)
+    out["nbr"] = nbr
+    return out
+
+
+def test_zero_velocity_inlet_density_equals_bounce_back():
+    lat = small_pipe()
+    lat["uin"] = np.zeros_like(lat["uin"])
+    walls = _with_codes_as_walls(lat, lambda nbr: nbr == L.INLET)
+    rng = np.random.default_rng(7)
+    a = oracle.OracleSolver(lat, 0.05, bouzidi=False, model="regularised", threads=1)
+    b = oracle.OracleSolver(walls, 0.05, bouzidi=False, model="regularised", threads=1)
+    random_state(a, rng)
+    b.f[b.cur][:] = a.f[a.cur]
+    b.vels[b.cur][:] = a.vels[a.cur]
+    a.step_once(1.0)
+    b.step_once(1.0)
+    # inlet cells then take the equilibrium, but their density is the sum of the streamed values in both cases
+    assert np.array_equal(a.rho, b.rho)
+
+
+def test_outlet_at_rest_is_anti_bounce_back():
+    lat = small_pipe()
+    walls = _with_codes_as_walls(lat, lambda nbr: nbr <= L.OUTLET0)
+    rng = np.random.default_rng(11)
+    a = oracle.OracleSolver(lat, 0.05, bouzidi=False, model="regularised", dtype=np.float64, threads=1)
+    b = oracle.OracleSolver(walls, 0.05, bouzidi=False, model="regularised", dtype=np.float64, threads=1)
+    random_state(a, rng)
+    a.vels[a.cur][:] = 0.0                       # outlet uses the previous velocity; p_out = 0
+    b.f[b.cur][:] = a.f[a.cur]
+    f_in = a.f[a.cur].copy()
+    a.step_once(1.0)
+    b.step_once(1.0)
+    # an outlet link streams -f_o, a half-way wall +f_o: the densities differ by -2 sum f_o over outlet links
+    nbr = lat["nbr"]
+    expect = np.zeros(a.N)
+    for i in range(27):
+        m = nbr[i] <= L.OUTLET0
+        expect[m] += -2.0 * f_in[L.OPP[i], m]
+    assert np.allclose(a.rho - b.rho, expect, atol=1e-15)
+    assert np.any(expect != 0)
+
+
+# --- moment properties pinning the third-order term and the Galilean correction ------------------------------------
+def test_inlet_equilibrium_third_moments():
+    """Inlet cells output w_i (dr + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2 + h3): its third moments are those of the Maxwellian
+    that D3Q27 supports, sum c_a c_b c_c feq = (u_a d_bc + u_b d_ac + u_c d_ab) / 3 + u_a u_b u_c for a, b, c not all
+    equal (and u_a for aaa, where the lattice aliases c^3 = c)."""
+    lat = small_pipe()
+    rng = np.random.default_rng(13)
+    o = oracle.OracleSolver(lat, 0.05, model="regularised", dtype=np.float64, threads=1)
+    cells = np.flatnonzero((lat["nbr"] == L.INLET).any(0))
+    uin = np.zeros_like(lat["uin"], dtype=np.float64)
+    uin[cells] = 0.05 * rng.standard_normal((len(cells), 3))
+    o.uin[:] = uin
+    random_state(o, rng)
+    o.step_once(1.0)
+    f = o.get_f()[:, cells]
+    u = uin[cells]
+    d = np.eye(3)
+    for a in range(3):
+        for b in range(3):
+            for c in range(3):
+                m3 = (C[:, a] * C[:, b] * C[:, c]) @ f
+                lin = (u[:, a] * d[b, c] + u[:, b] * d[a, c] + u[:, c] * d[a, b]) / 3
+                if a == b == c:
+                    expect = u[:, a]
+                else:
+                    expect = lin + u[:, a] * u[:, b] * u[:, c]
+                assert np.allclose(m3, expect, atol=1e-15), (a, b, c)
+    # and the velocity written for inlet cells is the prescribed one
+    assert np.allclose(o.vels[o.cur][cells], u)
+
+
+def test_galilean_correction_second_moment():
+    """Regularised collision, no force: sum c c^T f_out = dr/3 I + u u^T + (1 - om) P with P the non-equilibrium
+    stress minus diag(E), E_a = 3 tau u_a^2 du_a/dx_a (central difference of the previous velocity). A linear previous
+    velocity field makes the gradient exact."""
+    n = 6
+    lat = periodic_box(n)
+    o = oracle.OracleSolver(lat, 0.1, model="regularised", dtype=np.float64, threads=1)
+    rng = np.random.default_rng(17)
+    random_state(o, rng, amp=2e-3)
+    coords = lat["coords"].astype(np.float64)
+    g = np.array([0.003, -0.002, 0.0015])
+    # du_a/dx_a = g_a by central differences everywhere except across the periodic seam
+    o.vels[o.cur][:] = coords * g[None, :] + 0.01
+    f_in = o.f[o.cur].copy()
+    o.step_once(1.0)
+    f_s = np.take_along_axis(f_in, lat["nbr"].astype(np.int64), axis=1)
+    dr = f_s.sum(0)
+    u = C.T @ f_s
+    Pi = np.einsum("ia,ib,in->nab", C, C, f_s)
+    P = Pi - dr[:, None, None] / 3 * np.eye(3) - np.einsum("an,bn->nab", u, u)
+    tau = 1 / o.om[0]
+    E = 3 * tau * u.T ** 2 * g[None, :]
+    interior = ((coords > 0) & (coords < n - 1)).all(1)
+    P[:, [0, 1, 2], [0, 1, 2]] -= E
+    expect = dr[:, None, None] / 3 * np.eye(3) + np.einsum("an,bn->nab", u, u) + (1 - o.om[0]) * P
+    got = np.einsum("ia,ib,in->nab", C, C, o.get_f())
+    assert np.allclose(got[interior], expect[interior], atol=1e-14)
+    # without the correction the diagonal differs by (1 - om) E
+    assert np.abs((1 - o.om[0]) * E[interior]).max() > 1e-9
+
+
+def test_fp32_tracks_fp64():
+    lat = small_pipe()
+    rng = np.random.default_rng(19)
+    runs = []
+    for dt in (np.float32, np.float64):
+        o = oracle.OracleSolver(lat, 0.05, model="regularised", dtype=dt, threads=2)
+        o.set_drive([0.02, 0.0, 0.01], period=40)
+        o.set_outlet_pressure([0.0])
+        o.run(60)
+        runs.append(o.get_f().astype(np.float64))
+    assert np.linalg.norm(runs[0] - runs[1]) / np.linalg.norm(runs[1]) < 1e-5
+
+
+# --- reference's published physics (geometry/solver.json), at its operating point tau = 0.50495 --------------------
+# The reference ran FP32 Warp kernels compiled with fused multiply-adds. At tau = 1/2 + 0.005 the converged profiles
+# depend on the FP32 rounding style at the 1e-3 level of the L2 error (the strict FP32 build and the FP64 build differ
+# from the published values by up to 3e-3 relative; the FMA build by 2e-4). The published values are therefore matched
+# with the FMA build of the oracle to 1e-3 relative (L2) and 3e-5 absolute (flow error). Dropping Bouzidi moves the L2
+# by 6x and a wrong force coefficient moves the flow error by percent-level: both fail.
+def _poiseuille(D, bz, nu, threads=None):
+    lat, r, R = periodic_pipe(D)
+    s = oracle.OracleSolver(lat, nu, bouzidi=bz, model="regularised", threads=threads, fma=True)
+    F = 4 * nu * 0.02 / R ** 2
+    s.set_force([0, 0, F])
+    s.run(int(14 * R * R / (5.78 * nu)))
+    _, u = s.fields()
+    ua = poiseuille_exact(r, R, F, nu)
+    return l2(u[:, 2], ua), float(u[:, 2].sum() / ua.sum() - 1), s.ok()
+
+
+def test_poiseuille_d10_published(nu_healthy):
+    e, fl, ok = _poiseuille(10, True, nu_healthy, threads=1)
+    assert ok
+    assert abs(e / 0.13532128869870114 - 1) < 1e-3, e
+    assert abs(fl - -0.14670134092831366) < 3e-5, fl
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("bz,l2_ref,flow_ref", [(True, 0.03429092846606971, -0.03803448455849523),
+                                                 (False, 0.20496644204979705, -0.22679752766633754)])
+def test_poiseuille_d20_published(nu_healthy, bz, l2_ref, flow_ref):
+    e, fl, ok = _poiseuille(20, bz, nu_healthy, threads=4)
+    assert ok
+    assert abs(e / l2_ref - 1) < 1e-3, e
+    assert abs(fl - flow_ref) < 3e-5, fl
+
+
+@pytest.mark.slow
+def test_womersley_wo56_published(nu_healthy):
+    """validate.py:92-117 at Wo = 5.6 (period 12132 steps): published L2 over the cycle 0.060357."""
+    nu = nu_healthy
+    lat, r, R = periodic_pipe(20)
+    Wo, phases = 5.6, 12
+    T = int(round(2 * math.pi * R * R / (Wo * Wo * nu) / phases)) * phases
+    assert T == 12132
+    w = 2 * math.pi / T
+    Wo = R * math.sqrt(w / nu)
+    rr = np.linspace(0, R, 50)
+    peak = max(np.abs(womersley_exact(rr, R, Wo, 1.0, w, t)).max() for t in np.linspace(0, T, 24))
+    F0 = 0.02 / peak
+    s = oracle.OracleSolver(lat, nu, model="regularised", threads=4, fma=True)
+    s.set_force([0, 0, F0])
+    s.set_drive([0.0, 1.0, 0.0], period=T)
+    s.run(2 * T)
+    num = den = 0.0
+    for _ in range(phases):
+        s.run(T // phases)
+        _, u = s.fields()
+        ua = womersley_exact(r, R, Wo, F0, w, s.steps - 0.5)
+        num += ((u[:, 2] - ua) ** 2).sum()
+        den += (ua ** 2).sum()
+    e = math.sqrt(num / den)
+    assert abs(e / 0.060357378395948014 - 1) < 1e-3, e
