@@ -42,6 +42,30 @@ class GatherArgs(ctypes.Structure):
                 ("n_probe", c_i32)]
 
 
+class LatticeIn(ctypes.Structure):
+    _fields_ = [("n_cells", c_i64), ("shape", c_i32 * 3), ("periodic", c_i32 * 3), ("coords", c_void_p),
+                ("nbr", c_void_p), ("q", c_void_p), ("uin", c_void_p)]
+
+
+class BricksSizes(ctypes.Structure):
+    _fields_ = [("n_bricks", c_i64), ("n_boundary", c_i64), ("n_interior", c_i64), ("n_slots", c_i64)]
+
+
+class BricksStepArgs(ctypes.Structure):
+    _fields_ = [("f", c_void_p), ("nbr", c_void_p), ("list", c_void_p), ("n_list", c_i64), ("bidx", c_void_p),
+                ("lmask", c_void_p), ("smask", c_void_p), ("qc", c_void_p), ("uin", c_void_p), ("side", c_void_p),
+                ("om_cell", c_void_p), ("om_brick", c_void_p), ("omega", c_float), ("p_out", c_void_p),
+                ("drive", c_void_p), ("vel_old", c_void_p), ("vel", c_void_p), ("rho", c_void_p),
+                ("n_slots", c_i64), ("force", c_float * 3), ("magic", c_float), ("bouzidi", c_i32),
+                ("model", c_i32), ("galilean", c_i32), ("parity", c_i32), ("boundary", c_i32)]
+
+
+class BricksConvertArgs(ctypes.Structure):
+    _fields_ = [("f", c_void_p), ("nbr", c_void_p), ("bidx", c_void_p), ("lmask", c_void_p), ("smask", c_void_p),
+                ("side", c_void_p), ("slot_of_cell", c_void_p), ("n_cells", c_i64), ("f_ref", c_void_p),
+                ("parity", c_i32)]
+
+
 # exported symbol -> argument types (restype is int for all but cs_last_error)
 SIGNATURES = {
     "cs_last_error": [],
@@ -59,7 +83,15 @@ SIGNATURES = {
     "cs_flux_bc": [ctypes.POINTER(FluxArgs), c_void_p],
     "cs_gather": [ctypes.POINTER(GatherArgs), c_void_p],
     "cs_reduce_finite": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
+    "cs_bricks_build": [ctypes.POINTER(LatticeIn), ctypes.POINTER(c_void_p)],
+    "cs_bricks_sizes_of": [c_void_p, ctypes.POINTER(BricksSizes)],
+    "cs_bricks_copy": [c_void_p, ctypes.c_char_p, c_void_p],
+    "cs_bricks_free": [c_void_p],
+    "cs_bricks_step": [ctypes.POINTER(BricksStepArgs), c_void_p],
+    "cs_bricks_get_f": [ctypes.POINTER(BricksConvertArgs), c_void_p],
+    "cs_bricks_set_f": [ctypes.POINTER(BricksConvertArgs), c_void_p],
 }
+CS_E_LAYOUT = -3
 
 _lib = None
 

@@ -1,0 +1,87 @@
+"""Host side of the brick layout: converts a lattice in the reference's indexed layout (lbm.py:264-330) into the
+4 x 4 x 4 brick storage used by the AA-pattern kernels (native builder ``cs_bricks_build``)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .lattice import C
+
+BRICK = 4
+ARRAYS = {
+    # name: (dtype, shape as a function of the sizes)
+    "coords": (np.int32, lambda z, N: (z.n_bricks, 3)),
+    "nbr": (np.int32, lambda z, N: (z.n_bricks, 27)),
+    "bidx": (np.int32, lambda z, N: (z.n_bricks,)),
+    "interior": (np.int32, lambda z, N: (z.n_interior,)),
+    "boundary": (np.int32, lambda z, N: (z.n_boundary,)),
+    "slot_of_cell": (np.int32, lambda z, N: (N,)),
+    "cell_of_slot": (np.int32, lambda z, N: (z.n_slots,)),
+    "lmask": (np.uint32, lambda z, N: (z.n_boundary, 64)),
+    "smask": (np.uint32, lambda z, N: (z.n_boundary, 64)),
+    "qc": (np.float32, lambda z, N: (z.n_boundary, 27, 64)),
+    "uin": (np.float32, lambda z, N: (z.n_boundary, 3, 64)),
+}
+
+
+class LayoutError(ValueError):
+    """The lattice's neighbour table is not the implicit neighbourhood of a brick grid."""
+
+
+def infer_periodic(lat):
+    """Axes along which some fluid link wraps around the grid (build_lattice's `periodic`, not stored in the dict)."""
+    nbr, coords = lat["nbr"], lat["coords"].astype(np.int64)
+    periodic = [0, 0, 0]
+    for i in range(1, len(C)):
+        src = nbr[i]
+        m = src >= 0
+        if not m.any():
+            continue
+        delta = coords[src[m]] - coords[m] + C[i]
+        for a in range(3):
+            if np.any(delta[:, a] != 0):
+                periodic[a] = 1
+    return periodic
+
+
+def build(lat):
+    """Brick layout of `lat` as a dict of numpy arrays plus the sizes. Raises LayoutError when the lattice cannot be
+    represented (then the indexed layout serves it)."""
+    N = int(lat["nbr"].shape[1])
+    coords = np.ascontiguousarray(lat["coords"], np.int32)
+    nbr = np.ascontiguousarray(lat["nbr"], np.int32)
+    q = np.ascontiguousarray(lat["q"], np.float32)
+    uin = np.ascontiguousarray(lat["uin"], np.float32)
+    inp = _native.LatticeIn()
+    inp.n_cells = N
+    inp.shape = (ctypes.c_int32 * 3)(*[int(v) for v in lat["shape"]])
+    periodic = infer_periodic(lat)
+    inp.periodic = (ctypes.c_int32 * 3)(*periodic)
+    inp.coords, inp.nbr, inp.q, inp.uin = coords.ctypes.data, nbr.ctypes.data, q.ctypes.data, uin.ctypes.data
+    h = ctypes.c_void_p()
+    lib = _native.load()
+    rc = lib.cs_bricks_build(ctypes.byref(inp), ctypes.byref(h))
+    if rc == _native.CS_E_LAYOUT:
+        raise LayoutError(lib.cs_last_error().decode(errors="replace"))
+    if rc != 0:
+        raise _native.NativeError(f"cs_bricks_build returned {rc}: {lib.cs_last_error().decode(errors='replace')}")
+    try:
+        z = _native.BricksSizes()
+        _native.call("cs_bricks_sizes_of", h, ctypes.byref(z))
+        out = {"n_bricks": z.n_bricks, "n_boundary": z.n_boundary, "n_interior": z.n_interior,
+               "n_slots": z.n_slots, "n_cells": N, "periodic": periodic}
+        for name, (dt, shape) in ARRAYS.items():
+            arr = np.empty(shape(z, N), dt)
+            if arr.size:
+                _native.call("cs_bricks_copy", h, name.encode(), ctypes.c_void_p(arr.ctypes.data))
+            out[name] = arr
+    finally:
+        lib.cs_bricks_free(h)
+    return out
+
+
+def fill_fraction(layout):
+    """Fluid cells / storage slots."""
+    return layout["n_cells"] / max(1, layout["n_slots"])

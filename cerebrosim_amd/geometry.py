@@ -16,15 +16,16 @@ from .lattice import build_lattice
 OFFSET = np.array([0.31, 0.17, 0.5])   # pipe axes off the grid's symmetry lines (validate.py:42)
 
 
-def periodic_pipe(D):
-    """Pipe of diameter D along z, two nodes long and periodic in z (the reference's Poiseuille/Womersley pipe).
-    Returns (lattice, radial distance per cell, R)."""
+def periodic_pipe(D, nz=2):
+    """Pipe of diameter D along z, nz nodes long and periodic in z (the reference's Poiseuille/Womersley pipe has
+    nz = 2; flows that do not vary along z give the same per-cell values for any nz). Returns (lattice, radial
+    distance per cell, R)."""
     R = D / 2.0
     n = int(math.ceil(D)) + 4
     cx, cy = n / 2 - 0.5 + OFFSET[0], n / 2 - 0.5 + OFFSET[1]
     i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
     r2d = np.hypot(i - cx, j - cy)
-    r = np.repeat(r2d[:, :, None], 2, axis=2)
+    r = np.repeat(r2d[:, :, None], nz, axis=2)
     lat = build_lattice(r - R, [0, 0, 0], 1.0, periodic=(2,))
     return lat, r[tuple(lat["coords"].T)], R
 

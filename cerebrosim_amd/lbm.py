@@ -62,13 +62,26 @@ class _IndexedBackend:
         self.inl = torch.from_numpy((lat["nbr"] == INLET).any(0).astype(np.int32)).to(dev)
         self.vels = [torch.zeros((N, 3), dtype=torch.float32, device=dev) for _ in range(2)]
         self.rho = torch.zeros(N, dtype=torch.float32, device=dev)
+        self.om = torch.full((N,), 1.0 / s.tau, dtype=torch.float32, device=dev)
+
+    def set_omega(self, om):
+        self.om.copy_(_torch().from_numpy(np.ascontiguousarray(om, np.float32)))
+
+    def fields(self, cur):
+        return self.rho.cpu().numpy(), self.vels[cur].cpu().numpy()
+
+    def launch_parts(self):
+        return [("k_step_indexed", self.vels[0].shape[0])]
+
+    def step_part(self, s, a, part):
+        self.step(s, a)
 
     def step(self, s, a):
         args = _native.StepArgs()
         args.f_in, args.f_out = self.f[a].data_ptr(), self.f[1 - a].data_ptr()
         args.nbr, args.qw, args.uin, args.inl = (self.nbr.data_ptr(), self.q.data_ptr(), self.uin.data_ptr(),
                                                  self.inl.data_ptr())
-        args.p_out, args.drive, args.om_cell = s.p_out.data_ptr(), s.drv.data_ptr(), s.om.data_ptr()
+        args.p_out, args.drive, args.om_cell = s.p_out.data_ptr(), s.drv.data_ptr(), self.om.data_ptr()
         args.vel_old, args.vel, args.rho = self.vels[a].data_ptr(), self.vels[1 - a].data_ptr(), self.rho.data_ptr()
         args.n_cells = s.N
         args.force = (ctypes.c_float * 3)(*s.force)
@@ -77,10 +90,10 @@ class _IndexedBackend:
         _native.call("cs_step_indexed", ctypes.byref(args), s.stream_handle)
 
     # state access in the reference layout
-    def get_f(self, cur):
+    def get_f(self, s, cur):
         return self.f[cur].cpu().numpy()
 
-    def set_f(self, cur, f):
+    def set_f(self, s, cur, f):
         self.f[cur].copy_(_torch().from_numpy(np.ascontiguousarray(f, np.float32)))
 
     def velocity(self, cur):
@@ -96,13 +109,128 @@ class _IndexedBackend:
         self.vels[cur].copy_(_torch().from_numpy(np.ascontiguousarray(u, np.float32)))
 
 
+class _BricksBackend:
+    """4^3 brick storage with AA in-place streaming (see csrc/k_bricks.hip): populations [bricks][27][64] once,
+    velocity as three planes of n_slots floats (double-buffered), density one plane. Two launches per step: the
+    interior bricks (no boundary data) and the boundary bricks."""
+
+    name = "bricks"
+
+    def __init__(self, s, lat, layout):
+        torch = _torch()
+        dev = s.dev
+        self.L = layout
+        self.S = S = int(layout["n_slots"])
+        self.nb = int(layout["n_bricks"])
+        self.n_cells = s.N
+        up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        self.f = torch.zeros(self.nb * 27 * 64, dtype=torch.float32, device=dev)
+        self.nbr = up(layout["nbr"])
+        self.bidx = up(layout["bidx"])
+        self.interior = up(layout["interior"])
+        self.boundary = up(layout["boundary"])
+        nbb = max(1, int(layout["n_boundary"]))
+        self.lmask = up(layout["lmask"].view(np.int32)) if layout["n_boundary"] else torch.zeros(64, dtype=torch.int32, device=dev)
+        self.smask = up(layout["smask"].view(np.int32)) if layout["n_boundary"] else torch.zeros(64, dtype=torch.int32, device=dev)
+        self.qc = up(layout["qc"]) if layout["n_boundary"] else torch.zeros(27 * 64, dtype=torch.float32, device=dev)
+        self.uin = up(layout["uin"]) if layout["n_boundary"] else torch.zeros(3 * 64, dtype=torch.float32, device=dev)
+        self.side = torch.zeros(nbb * 27 * 64, dtype=torch.float32, device=dev)
+        self.slot_of_cell = up(layout["slot_of_cell"].astype(np.int64))
+        self.slot_of_cell32 = up(layout["slot_of_cell"])
+        self.vels = [torch.zeros((3, S), dtype=torch.float32, device=dev) for _ in range(2)]
+        self.rho = torch.zeros(S, dtype=torch.float32, device=dev)
+        self.omega = float(np.float32(1.0 / s.tau))
+        self.om = torch.full((S,), self.omega, dtype=torch.float32, device=dev)
+        self.om_brick = torch.zeros(self.nb, dtype=torch.uint8, device=dev)
+
+    def set_omega(self, om):
+        torch = _torch()
+        om = np.asarray(om, np.float32)
+        slots = np.full(self.S, self.omega, np.float32)
+        slots[self.L["slot_of_cell"]] = om
+        own = (slots.reshape(self.nb, 64) != np.float32(self.omega)).any(1).astype(np.uint8)
+        self.om.copy_(torch.from_numpy(slots))
+        self.om_brick.copy_(torch.from_numpy(own))
+
+    def _args(self, s, a, boundary):
+        A = _native.BricksStepArgs()
+        lst = self.boundary if boundary else self.interior
+        A.f, A.nbr, A.list, A.n_list = self.f.data_ptr(), self.nbr.data_ptr(), lst.data_ptr(), lst.numel()
+        A.bidx, A.lmask, A.smask = self.bidx.data_ptr(), self.lmask.data_ptr(), self.smask.data_ptr()
+        A.qc, A.uin, A.side = self.qc.data_ptr(), self.uin.data_ptr(), self.side.data_ptr()
+        A.om_cell, A.om_brick, A.omega = self.om.data_ptr(), self.om_brick.data_ptr(), self.omega
+        A.p_out, A.drive = s.p_out.data_ptr(), s.drv.data_ptr()
+        A.vel_old, A.vel, A.rho = self.vels[a].data_ptr(), self.vels[1 - a].data_ptr(), self.rho.data_ptr()
+        A.n_slots = self.S
+        A.force = (ctypes.c_float * 3)(*s.force)
+        A.magic = s.magic
+        A.bouzidi, A.model, A.galilean = s.bouzidi, s.model, s.gal
+        A.parity, A.boundary = a, int(boundary)
+        return A
+
+    def step(self, s, a):
+        for boundary in (False, True):
+            self.step_part(s, a, boundary)
+
+    def step_part(self, s, a, boundary):
+        """One of the step's two launches: interior bricks or boundary bricks."""
+        A = self._args(s, a, boundary)
+        if A.n_list:
+            _native.call("cs_bricks_step", ctypes.byref(A), s.stream_handle)
+
+    def launch_parts(self):
+        """(name, cells processed) of the launches of one step, in order."""
+        return [("k_bricks_interior", int(self.L["n_interior"]) * 64),
+                ("k_bricks_boundary", self.n_cells - int(self.L["n_interior"]) * 64)]
+
+    def _convert(self, s, name, f_ref, parity):
+        A = _native.BricksConvertArgs()
+        A.f, A.nbr, A.bidx = self.f.data_ptr(), self.nbr.data_ptr(), self.bidx.data_ptr()
+        A.lmask, A.smask, A.side = self.lmask.data_ptr(), self.smask.data_ptr(), self.side.data_ptr()
+        A.slot_of_cell, A.n_cells = self.slot_of_cell32.data_ptr(), self.n_cells
+        A.f_ref, A.parity = f_ref.data_ptr(), parity
+        _native.call(name, ctypes.byref(A), s.stream_handle)
+
+    def get_f(self, s, cur):
+        f_ref = _torch().empty((Q, self.n_cells), dtype=_torch().float32, device=s.dev)
+        self._convert(s, "cs_bricks_get_f", f_ref, cur)
+        s.synchronize()
+        return f_ref.cpu().numpy()
+
+    def set_f(self, s, cur, f):
+        f_ref = _torch().from_numpy(np.ascontiguousarray(f, np.float32)).to(s.dev)
+        self._convert(s, "cs_bricks_set_f", f_ref, cur)
+        s.synchronize()
+
+    def velocity(self, cur):
+        return self.vels[cur]
+
+    def velocity_addressing(self):
+        return 1, self.S
+
+    def slots(self, cells):
+        return self.L["slot_of_cell"][np.asarray(cells)].astype(np.int32)
+
+    def fields(self, cur):
+        idx = self.slot_of_cell
+        rho = self.rho.index_select(0, idx)
+        u = self.vels[cur].index_select(1, idx).t()
+        return rho.cpu().numpy(), u.contiguous().cpu().numpy()
+
+    def set_velocity(self, cur, u):
+        torch = _torch()
+        u = torch.from_numpy(np.ascontiguousarray(u, np.float32)).to(self.vels[cur].device)
+        self.vels[cur][:, self.slot_of_cell] = u.t()
+
+
 class Solver:
     """Lattice Boltzmann on a sparse lattice (lbm.py:346-455). ``model`` "trt" (magic = Lambda) or "regularised";
     ``sigma`` < 1 blends the regularised stress with the finite-difference one. ``layout`` selects the device
-    storage: "indexed" (reference (Q, N) layout, any neighbour table)."""
+    storage: "bricks" (4^3 bricks, AA in-place streaming), "indexed" (reference (Q, N) layout, any neighbour table)
+    or "auto" (bricks when the lattice and options allow, else indexed)."""
 
     def __init__(self, lat, nu_lu, magic=3.0 / 16.0, bouzidi=True, model="trt", sigma=1.0, galilean=None,
-                 device=None, layout="indexed"):
+                 device=None, layout="auto"):
         torch = _torch()
         self.dev = _require_gpu(device)
         self.lat = lat
@@ -122,15 +250,11 @@ class Solver:
         self.stream_handle = ctypes.c_void_p(self._tstream.cuda_stream)
         with torch.cuda.stream(self._tstream):
             self.p_out = torch.zeros(max(1, lat["n_outlets"]), dtype=torch.float32, device=dev)
-            self.om = torch.full((N,), 1.0 / self.tau, dtype=torch.float32, device=dev)
             self.tick = torch.zeros(1, dtype=torch.int32, device=dev)
             self.drv = torch.zeros(1, dtype=torch.float32, device=dev)
             self._work = torch.zeros(_native.CS_REDUCE_WORK, dtype=torch.float64, device=dev)
             self._red = torch.zeros(2, dtype=torch.float64, device=dev)
-            if layout == "indexed":
-                self.backend = _IndexedBackend(self, lat)
-            else:
-                raise ValueError(f"unknown layout {layout!r}")
+            self.backend = self._make_backend(lat, layout)
         self._tstream.synchronize()
         self.layout = self.backend.name
         self.force = (0.0, 0.0, 0.0)
@@ -138,6 +262,29 @@ class Solver:
         self.set_drive([1.0])
         self.cur = 0
         self.steps = 0
+
+    def _make_backend(self, lat, layout):
+        from . import bricks
+        if layout not in ("auto", "bricks", "indexed"):
+            raise ValueError(f"unknown layout {layout!r}")
+        if layout == "indexed":
+            return _IndexedBackend(self, lat)
+        reason = None
+        if self.model == 1 and self.sigma < 1.0:
+            reason = "the hybrid blend (sigma < 1) runs on the indexed layout"
+        else:
+            try:
+                return _BricksBackend(self, lat, bricks.build(lat))
+            except bricks.LayoutError as e:
+                reason = str(e)
+        if layout == "bricks":
+            raise ValueError(f"layout 'bricks' unavailable: {reason}")
+        self.layout_note = reason
+        return _IndexedBackend(self, lat)
+
+    @property
+    def om(self):
+        return self.backend.om
 
     # --- reference attributes backed by the device state
     @property
@@ -199,7 +346,10 @@ class Solver:
             w = w * w * (3.0 - 2.0 * w)
             tau = np.maximum(tau, np.where(side, self.tau + (tau_max - self.tau) * w, self.tau))
         self.tau_cell = tau
-        self._host_to_device(self.om, (1.0 / tau).astype(np.float32))
+        self.synchronize()
+        with _torch().cuda.stream(self._tstream):
+            self.backend.set_omega((1.0 / tau).astype(np.float32))
+        self.synchronize()
 
     def set_outlet_pressure(self, p_lu):
         self._host_to_device(self.p_out, np.asarray(p_lu, np.float32).reshape(-1))
@@ -242,8 +392,8 @@ class Solver:
     def fields(self):
         """(pressure, velocity) per cell in lattice units, as numpy, in the lattice's cell order."""
         self.synchronize()
-        p, u = self.backend.fields(self.cur) if hasattr(self.backend, "fields") else (
-            self.backend.rho.cpu().numpy(), self.backend.velocity(self.cur).cpu().numpy())
+        with _torch().cuda.stream(self._tstream):
+            p, u = self.backend.fields(self.cur)
         return p / 3.0, u
 
     def ok(self):
@@ -259,13 +409,14 @@ class Solver:
     def get_f(self):
         """Newest post-collision populations f~ (Q, N) in the reference layout, as numpy."""
         self.synchronize()
-        return self.backend.get_f(self.cur)
+        with _torch().cuda.stream(self._tstream):
+            return self.backend.get_f(self, self.cur)
 
     def set_f(self, f):
         """Replace the newest post-collision populations (Q, N), reference layout."""
         self.synchronize()
         with _torch().cuda.stream(self._tstream):
-            self.backend.set_f(self.cur, f)
+            self.backend.set_f(self, self.cur, f)
         self.synchronize()
 
     def set_velocity(self, u):

@@ -97,6 +97,99 @@ typedef struct cs_step_args {
 int cs_step_indexed(const cs_step_args* a, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Brick layout -- the MI355X storage of the same lattice (no reference counterpart: the reference keeps the
+ * indexed layout of lbm.py:264-330 on the device). Fluid cells are grouped into 4 x 4 x 4 bricks (one wavefront per
+ * brick); populations are stored once, per brick as [27][64] floats, and streamed in place with the AA pattern.
+ * The host builder converts a lattice in the reference layout (nbr/q/uin/coords as returned by build_lattice) and
+ * checks that every fluid link of nbr is the implicit brick neighbour; it fails with CS_E_LAYOUT otherwise (the
+ * caller then uses the indexed layout).
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct cs_lattice_in {
+    int64_t n_cells;
+    int32_t shape[3];
+    int32_t periodic[3];        /* 1 = axis wraps (must then be a multiple of the brick edge) */
+    const int32_t* coords;      /* host, (N, 3) */
+    const int32_t* nbr;         /* host, (Q, N) */
+    const float* q;             /* host, (Q, N) */
+    const float* uin;           /* host, (N, 3) */
+} cs_lattice_in;
+
+typedef struct cs_bricks_sizes {
+    int64_t n_bricks;           /* bricks holding at least one fluid cell */
+    int64_t n_boundary;         /* bricks with a solid cell or a missing link */
+    int64_t n_interior;
+    int64_t n_slots;            /* n_bricks * 64 */
+} cs_bricks_sizes;
+
+/* Build the brick layout on the host; *handle owns the arrays until cs_bricks_free. */
+int cs_bricks_build(const cs_lattice_in* in, void** handle);
+int cs_bricks_sizes_of(const void* handle, cs_bricks_sizes* out);
+/* Copy one named host array into dst (caller-sized by cs_bricks_sizes_of):
+ *  "coords"        int32 [n_bricks][3]     brick coordinates (in bricks)
+ *  "nbr"           int32 [n_bricks][27]    neighbour brick at offset d = (ox+1) + 3 (oy+1) + 9 (oz+1), -1 if none
+ *  "bidx"          int32 [n_bricks]        boundary index of the brick, -1 for interior bricks
+ *  "interior"      int32 [n_interior]      brick ids of interior bricks
+ *  "boundary"      int32 [n_boundary]      brick ids of boundary bricks
+ *  "slot_of_cell"  int32 [N]               storage slot (brick * 64 + x + 4 y + 16 z) of each reference cell
+ *  "cell_of_slot"  int32 [n_slots]         reference cell of each slot, -1 for solid slots
+ *  "lmask"         uint32 [n_boundary][64] bit i: link i is fluid; bit 27: inlet cell; 0 for solid slots
+ *  "smask"         uint32 [n_boundary][64] bit i: wall link i with q >= 1/2 whose opposite link is fluid
+ *  "qc"            float [n_boundary][27][64] wall fraction q (>= 0), -1 inlet, -2 - k outlet k, 0 fluid
+ *  "uin"           float [n_boundary][3][64] inlet profile of inlet cells */
+int cs_bricks_copy(const void* handle, const char* name, void* dst);
+int cs_bricks_free(void* handle);
+
+/* One AA step over a list of bricks (two launches per step: interior bricks, then boundary bricks).
+ * parity 0 = even step (reads and writes the cell's own slots), 1 = odd step (pulls from and pushes to the
+ * neighbours). Velocity is stored as three component planes of n_slots floats; rho as one plane. */
+typedef struct cs_bricks_step_args {
+    float* f;                   /* [n_bricks][27][64] */
+    const int32_t* nbr;         /* [n_bricks][27] */
+    const int32_t* list;        /* brick ids to process */
+    int64_t n_list;
+    const int32_t* bidx;        /* [n_bricks] */
+    const uint32_t* lmask;      /* [n_boundary][64] */
+    const uint32_t* smask;      /* [n_boundary][64] */
+    const float* qc;            /* [n_boundary][27][64] */
+    const float* uin;           /* [n_boundary][3][64] */
+    float* side;                /* [n_boundary][27][64] copies of q >= 1/2 partners */
+    const float* om_cell;       /* [n_slots] per-cell 1 / tau, read only where om_brick != 0 */
+    const uint8_t* om_brick;    /* [n_bricks] 1 = the brick's cells have their own omega */
+    float omega;                /* 1 / tau of every other cell */
+    const float* p_out;
+    const float* drive;
+    const float* vel_old;       /* [3][n_slots] */
+    float* vel;                 /* [3][n_slots] */
+    float* rho;                 /* [n_slots] */
+    int64_t n_slots;
+    float force[3];
+    float magic;
+    int32_t bouzidi;
+    int32_t model;
+    int32_t galilean;
+    int32_t parity;
+    int32_t boundary;           /* 1 = the list holds boundary bricks */
+} cs_bricks_step_args;
+int cs_bricks_step(const cs_bricks_step_args* a, void* stream);
+
+/* Conversions between the AA brick storage and the reference (Q, N) layout of post-collision populations, for the
+ * state that the next step of the given parity reads. set_f also refreshes the side copies. */
+typedef struct cs_bricks_convert_args {
+    float* f;                   /* brick storage */
+    const int32_t* nbr;
+    const int32_t* bidx;
+    const uint32_t* lmask;
+    const uint32_t* smask;
+    float* side;
+    const int32_t* slot_of_cell; /* [N] */
+    int64_t n_cells;
+    float* f_ref;               /* device (Q, N) */
+    int32_t parity;             /* parity of the next step */
+} cs_bricks_convert_args;
+int cs_bricks_get_f(const cs_bricks_convert_args* a, void* stream);
+int cs_bricks_set_f(const cs_bricks_convert_args* a, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Per-step flow measurement and boundary update -- replaces k_flux + k_bc (simulate.py:65-82) launched after each
  * step at simulate.py:111-114. Q[slot] = sum over the M slab cells of vel[idx] . w (deterministic order), then
  * p_out[0] = drive[0]; for k in 1..n_open-1: p_out[k] += alpha (R[k] Q[k] - pref - p_out[k]); q_last = Q.
