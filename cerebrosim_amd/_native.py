@@ -9,7 +9,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcerebro_lbm.so")
+LIB_PATH = os.environ.get("CEREBRO_LBM_LIB") or os.path.join(HERE, "libcerebro_lbm.so")
 
 c_int, c_i32, c_i64, c_float, c_void_p = ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_void_p
 CS_MODEL_TRT, CS_MODEL_REGULARISED = 0, 1
@@ -88,6 +88,8 @@ SIGNATURES = {
     "cs_bricks_copy": [c_void_p, ctypes.c_char_p, c_void_p],
     "cs_bricks_free": [c_void_p],
     "cs_bricks_step": [ctypes.POINTER(BricksStepArgs), c_void_p],
+    "cs_bricks_slot_table": [c_void_p],
+    "cs_bricks_stage_table": [c_void_p],
     "cs_bricks_get_f": [ctypes.POINTER(BricksConvertArgs), c_void_p],
     "cs_bricks_set_f": [ctypes.POINTER(BricksConvertArgs), c_void_p],
 }

@@ -82,6 +82,21 @@ def build(lat):
     return out
 
 
+def slot_table():
+    """(27, 64) uint16: offset of population k of cell l inside its brick's block of 1728 floats."""
+    out = np.empty((27, 64), np.uint16)
+    _native.call("cs_bricks_slot_table", ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
+def stage_table():
+    """(27, 64) uint16: staged position of the population an interior brick's lane l pulls for direction i in an
+    odd step (own group first, then the neighbour records in offset order 0..12, 14..26)."""
+    out = np.empty((27, 64), np.uint16)
+    _native.call("cs_bricks_stage_table", ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
 def fill_fraction(layout):
     """Fluid cells / storage slots."""
     return layout["n_cells"] / max(1, layout["n_slots"])

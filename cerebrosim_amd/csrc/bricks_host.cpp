@@ -71,6 +71,10 @@ int build(const cs_lattice_in* in, BrickHost& h) {
         }
     std::sort(keys.begin(), keys.end());
     const int64_t nb = (int64_t)keys.size();
+    if (nb * BC * Q >= ((int64_t)1 << 32)) {
+        set_error("cs_bricks_build: %lld bricks exceed the 32-bit population offsets of one lattice", (long long)nb);
+        return CS_E_LAYOUT;
+    }
     h.nb = nb;
     h.coords.resize(3 * nb);
     for (int64_t b = 0; b < nb; ++b) {

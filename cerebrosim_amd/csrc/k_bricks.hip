@@ -1,7 +1,15 @@
 // Solver step over the brick layout with AA-pattern in-place streaming (semantics of lbm.py:140-257).
 //
-// Storage: S[b][k][l] = f[(b * 27 + k) * 64 + l], l = x + 4 y + 16 z inside 4^3 brick b. One wavefront per brick,
-// lane l = cell l; four bricks per workgroup.
+// Storage: the 27 x 64 populations of 4^3 brick b occupy the block f[b * 1728 .. b * 1728 + 1728); population k of
+// cell l (l = x + 4 y + 16 z) sits at offset slot[k][l] of the block. One wavefront per brick, lane l = cell l; four
+// bricks per workgroup.
+//
+// Slot order. An even step touches exactly its brick's own block. An odd step reads and then writes slot (m, k) from
+// the cell m - c_k, which lies in m's brick or in one of its 26 neighbours. slot[][] groups a block by that accessing
+// brick: first the 1000 slots the brick's own wavefront reaches, then one contiguous record per neighbour offset (100
+// slots for a face neighbour, 10 for an edge, 1 for a corner), population-major inside each group. In an odd step a
+// wavefront therefore writes its own group and 26 records, each touched by no other wavefront, so cache lines fill
+// from one wavefront instead of collecting partial writes from two.
 //
 // AA pattern. Let P_i(x) be the post-collision population of cell x in direction i.
 //  * even step (parity 0) reads S[n][i] and writes P_i(n) to S[n][opp i]: everything stays in the cell's own slots;
@@ -28,6 +36,133 @@ constexpr int WG = 64 * WAVES;
 constexpr uint32_t kInletBit = 1u << 27;
 constexpr uint32_t kAllLinks = (1u << Q) - 1;
 
+struct SlotTable {
+    uint16_t p[Q][BC];
+};
+
+constexpr SlotTable make_slots() {
+    SlotTable t{};
+    int pos = 0;
+    // group g = brick offset (ox+1) + 3 (oy+1) + 9 (oz+1) of the accessing cell l - c_k; own brick (13) first
+    for (int gi = 0; gi < 27; ++gi) {
+        const int g = gi == 0 ? 13 : (gi <= 13 ? gi - 1 : gi);
+        for (int k = 0; k < Q; ++k)
+            for (int l = 0; l < BC; ++l) {
+                const int x = (l & 3) - Lat::cx[k], y = ((l >> 2) & 3) - Lat::cy[k], z = (l >> 4) - Lat::cz[k];
+                const int o = (x < 0 ? 0 : (x > 3 ? 2 : 1)) + 3 * (y < 0 ? 0 : (y > 3 ? 2 : 1)) +
+                              9 * (z < 0 ? 0 : (z > 3 ? 2 : 1));
+                if (o == g) t.p[k][l] = (uint16_t)pos++;
+            }
+    }
+    return t;
+}
+
+// 32-bit element offsets: the host guarantees n_bricks * 27 * 64 < 2^32 (one base pointer + a 32-bit VGPR offset
+// per access keeps register use low).
+__device__ __forceinline__ uint32_t brick_base(int b) { return (uint32_t)b * (uint32_t)(Q * BC); }
+
+__constant__ SlotTable kSlots = make_slots();
+
+// Brick offset index (ox+1) + 3 (oy+1) + 9 (oz+1) of the position (x, y, z) (components in [-1, 4]).
+constexpr int offset_index(int x, int y, int z) {
+    return (x < 0 ? 0 : (x > 3 ? 2 : 1)) + 3 * (y < 0 ? 0 : (y > 3 ? 2 : 1)) + 9 * (z < 0 ? 0 : (z > 3 ? 2 : 1));
+}
+
+constexpr int group_size(int g) {
+    int n = 0;
+    for (int a = 0, v = g; a < 3; ++a, v /= 3) n += (v % 3 != 1);
+    return n == 0 ? 1000 : (n == 1 ? 100 : (n == 2 ? 10 : 1));
+}
+
+// Start of group g inside a block (the order of make_slots: own group 13, then 0..12, 14..26).
+constexpr int group_start(int g) {
+    int pos = 0;
+    for (int gi = 0; gi < 27; ++gi) {
+        const int h = gi == 0 ? 13 : (gi <= 13 ? gi - 1 : gi);
+        if (h == g) return pos;
+        pos += group_size(h);
+    }
+    return -1;
+}
+
+// The odd-step footprint of a brick, staged in LDS: its own group, then for every neighbour offset o != 13 the
+// record of brick (b + o) that holds the slots accessed from b, i.e. group 26 - o of that brick. Staged order = the
+// block order of the groups (own first, then o = 0..12, 14..26), so stage_start(o) = group_start(26 - o) for o != 13
+// would not hold; it is computed here.
+constexpr int stage_start(int o) {
+    int pos = 0;
+    for (int oi = 0; oi < 27; ++oi) {
+        const int h = oi == 0 ? 13 : (oi <= 13 ? oi - 1 : oi);
+        if (h == o) return pos;
+        pos += group_size(h == 13 ? 13 : 26 - h);
+    }
+    return -1;
+}
+
+// oddpos[i][l]: position in the staged odd-step footprint of the slot lane l reads for direction i,
+// i.e. population opp(i) of the cell l - c_i (in this brick or a neighbour).
+constexpr SlotTable make_oddpos() {
+    SlotTable t{};
+    constexpr SlotTable sl = make_slots();
+    for (int i = 0; i < Q; ++i)
+        for (int l = 0; l < BC; ++l) {
+            const int x = (l & 3) - Lat::cx[i], y = ((l >> 2) & 3) - Lat::cy[i], z = (l >> 4) - Lat::cz[i];
+            const int o = offset_index(x, y, z);
+            const int ml = (x & 3) + 4 * (y & 3) + 16 * (z & 3);
+            const int slot = sl.p[Lat::opp[i]][ml];
+            const int g = o == 13 ? 13 : 26 - o;
+            t.p[i][l] = (uint16_t)(stage_start(o) + slot - group_start(g));
+        }
+    return t;
+}
+
+__constant__ SlotTable kOddPos = make_oddpos();
+
+// Cross-lane LDS hand-over inside one wavefront.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Copy a 27 x 64 uint16 table into LDS (216 x 16 bytes).
+__device__ __forceinline__ void load_table(uint32_t* lds, const SlotTable& src) {
+    if (threadIdx.x < Q * BC * 2 / 16) {
+        const uint4 v = reinterpret_cast<const uint4*>(&src)[threadIdx.x];
+        reinterpret_cast<uint4*>(lds)[threadIdx.x] = v;
+    }
+}
+
+// Move the footprint of brick b between global memory and the wave's LDS stage with full-row accesses.
+// Even steps: the whole own block. Odd steps: own group + the 26 records in the neighbour blocks.
+template <int PAR, bool STORE>
+__device__ __forceinline__ void stage_copy(float* __restrict__ F, float* stage, const int* tbl, uint32_t base, int L) {
+    if constexpr (PAR == 0) {
+        static_for<0, Q>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if constexpr (STORE) F[base + j * BC + L] = stage[j * BC + L];
+            else stage[j * BC + L] = F[base + j * BC + L];
+        });
+    } else {
+        static_for<0, 27>([&](auto oc) {
+            constexpr int o = decltype(oc)::value;
+            constexpr int g = o == 13 ? 13 : 26 - o;
+            constexpr int size = group_size(g), start = group_start(g), st = stage_start(o);
+            const uint32_t nb = o == 13 ? base : brick_base(__builtin_amdgcn_readfirstlane(tbl[o]));
+            static_for<0, (size + BC - 1) / BC>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const int idx = j * BC + L;
+                if (size % BC == 0 || j * BC + BC <= size || idx < size) {
+                    if constexpr (STORE) F[nb + start + idx] = stage[st + idx];
+                    else stage[st + idx] = F[nb + start + idx];
+                }
+            });
+        });
+    }
+}
+
+static_assert(Q * BC * 2 % 16 == 0 && Q * BC * 2 / 16 <= WG, "table copy layout");
+
 // Brick-table index d and local index of the position (x, y, z) + (SX, SY, SZ), SX.. in {-1, 0, 1}.
 template <int SX, int SY, int SZ>
 __device__ __forceinline__ void shift_pos(int x, int y, int z, int& d, int& l) {
@@ -42,7 +177,6 @@ __device__ __forceinline__ void shift_pos(int x, int y, int z, int& d, int& l) {
     l = ((x + SX) & 3) + 4 * ((y + SY) & 3) + 16 * ((z + SZ) & 3);
 }
 
-__device__ __forceinline__ size_t brick_base(int b) { return (size_t)b * (Q * BC); }
 
 // Value streamed into cell n through missing link I (wall / inlet / outlet), lbm.py:163-181.
 //   fo        raw value of slot I: P_opp(n)
@@ -75,6 +209,14 @@ __device__ __forceinline__ float link_value(float code, float fo, bool opp_fluid
 template <int MODEL, bool GAL, bool BND, int PAR>
 __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
     __shared__ int tbl[WAVES][32];
+    // Even steps of interior bricks stage their own block in LDS: 27 full-row loads and stores instead of 27
+    // gathers that each touch many lines of the block. Odd steps and boundary bricks access global memory directly
+    // (staging the 26 small neighbour records of an odd step costs more than the direct gathers).
+    constexpr bool STAGED = !BND && PAR == 0;
+    __shared__ uint32_t slots32[Q * BC / 2];
+    __shared__ float stage_all[STAGED ? WAVES * Q * BC : 1];
+    const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
+    load_table(slots32, STAGED && PAR == 1 ? kOddPos : kSlots);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int L = threadIdx.x & 63;
     // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
@@ -88,16 +230,16 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
     if (!valid) return;
 
     const int x = L & 3, y = (L >> 2) & 3, z = L >> 4;
-    const size_t S = (size_t)a.n_slots;
-    const size_t slot = (size_t)b * BC + L;
-    const size_t base = brick_base(b);
+    const uint32_t S = (uint32_t)a.n_slots;
+    const uint32_t slot = (uint32_t)b * BC + L;
+    const uint32_t base = brick_base(b);
     float* __restrict__ F = a.f;
 
     uint32_t mask = kAllLinks;
     int bb = 0;
     if constexpr (BND) {
         bb = a.bidx[b];
-        mask = a.lmask[(size_t)bb * BC + L];
+        mask = a.lmask[(uint32_t)bb * BC + L];
         if (mask == 0) return;  // solid slot
     }
     const float om = a.om_brick[b] ? a.om_cell[slot] : a.omega;
@@ -105,18 +247,27 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
 
     // ---- streaming: raw values ---------------------------------------------------------------------------
     float f[Q];
+    float* stage = stage_all + (STAGED ? wave * Q * BC : 0);
+    if constexpr (STAGED) {
+        stage_copy<PAR, false>(F, stage, tbl[wave], base, L);
+        wave_sync();
+        static_for<0, Q>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            f[i] = stage[slt[i * BC + L]];
+        });
+    } else
     static_for<0, Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
         constexpr int io = Lat::opp[i];
         if constexpr (PAR == 0 || i == 0) {
-            f[i] = F[base + i * BC + L];
+            f[i] = F[base + slt[i * BC + L]];
         } else {
             int dd, l;
             shift_pos<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z, dd, l);
             if (!BND || ((mask >> i) & 1u)) {
-                f[i] = F[brick_base(tbl[wave][dd]) + io * BC + l];
+                f[i] = F[brick_base(tbl[wave][dd]) + slt[io * BC + l]];
             } else {
-                f[i] = F[base + i * BC + L];
+                f[i] = F[base + slt[i * BC + L]];
             }
         }
     });
@@ -131,13 +282,13 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
             const float upsq = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
             float uind[3] = {0.0f, 0.0f, 0.0f};
             if (mask & kInletBit) {
-                uind[0] = a.uin[((size_t)bb * 3 + 0) * BC + L] * d;
-                uind[1] = a.uin[((size_t)bb * 3 + 1) * BC + L] * d;
-                uind[2] = a.uin[((size_t)bb * 3 + 2) * BC + L] * d;
+                uind[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
+                uind[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
+                uind[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
             }
-            const uint32_t sm = a.bouzidi ? a.smask[(size_t)bb * BC + L] : 0u;
-            const float* qc = a.qc + (size_t)bb * Q * BC + L;
-            const float* side = a.side + (size_t)bb * Q * BC + L;
+            const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
+            const float* qc = a.qc + (uint32_t)bb * Q * BC + L;
+            const float* side = a.side + (uint32_t)bb * Q * BC + L;
             static_for<1, Q>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 if constexpr (pair_head<i>()) {
@@ -179,24 +330,24 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
             shift_pos<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z, d2, l2);
             const float* V = a.vel_old + ax * S;
             if constexpr (!BND) {
-                const float v1 = V[(size_t)tbl[wave][d1] * BC + l1];
-                const float v2 = V[(size_t)tbl[wave][d2] * BC + l2];
+                const float v1 = V[(uint32_t)tbl[wave][d1] * BC + l1];
+                const float v2 = V[(uint32_t)tbl[wave][d2] * BC + l2];
                 p.g[ax] = (v2 - v1) * 0.5f;
             } else {
                 float h1 = 1.0f, v1 = up[ax], h2 = 1.0f, v2 = up[ax];
                 if ((mask >> i1) & 1u) {
-                    v1 = V[(size_t)tbl[wave][d1] * BC + l1];
+                    v1 = V[(uint32_t)tbl[wave][d1] * BC + l1];
                 } else {
-                    const float c = a.qc[((size_t)bb * Q + i1) * BC + L];
+                    const float c = a.qc[((uint32_t)bb * Q + i1) * BC + L];
                     if (c >= 0.0f) {
                         h1 = fmaxf(c, 0.1f);
                         v1 = 0.0f;
                     }
                 }
                 if ((mask >> i2) & 1u) {
-                    v2 = V[(size_t)tbl[wave][d2] * BC + l2];
+                    v2 = V[(uint32_t)tbl[wave][d2] * BC + l2];
                 } else {
-                    const float c = a.qc[((size_t)bb * Q + i2) * BC + L];
+                    const float c = a.qc[((uint32_t)bb * Q + i2) * BC + L];
                     if (c >= 0.0f) {
                         h2 = fmaxf(c, 0.1f);
                         v2 = 0.0f;
@@ -213,9 +364,9 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
     if constexpr (BND) inlet_cell = (mask & kInletBit) != 0u;
     if (inlet_cell) {
         dr = density_deviation(f);
-        u[0] = a.uin[((size_t)bb * 3 + 0) * BC + L] * d;
-        u[1] = a.uin[((size_t)bb * 3 + 1) * BC + L] * d;
-        u[2] = a.uin[((size_t)bb * 3 + 2) * BC + L] * d;
+        u[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
+        u[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
+        u[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
         inlet_equilibrium(f, dr, u[0], u[1], u[2], p.t3);
     } else {
         collide<MODEL, false>(f, p, dr, u);
@@ -227,27 +378,39 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
     a.vel[2 * S + slot] = u[2];
     a.rho[slot] = dr;
     if constexpr (BND) {
-        const uint32_t sm = a.bouzidi ? a.smask[(size_t)bb * BC + L] : 0u;
+        const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
         if (sm) {
-            float* side = a.side + (size_t)bb * Q * BC + L;
+            float* side = a.side + (uint32_t)bb * Q * BC + L;
             static_for<1, Q>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 if ((sm >> i) & 1u) side[i * BC] = f[i];
             });
         }
     }
+    if constexpr (STAGED) {
+        // even: P_i(n) -> own slot opp(i); odd: P_i(n) -> slot i of n + c_i = the slot read for direction opp(i)
+        wave_sync();
+        static_for<0, Q>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            constexpr int io = Lat::opp[i];
+            stage[slt[io * BC + L]] = f[i];
+        });
+        wave_sync();
+        stage_copy<PAR, true>(F, stage, tbl[wave], base, L);
+        return;
+    }
     static_for<0, Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
         constexpr int io = Lat::opp[i];
         if constexpr (PAR == 0 || i == 0) {
-            F[base + io * BC + L] = f[i];
+            F[base + slt[io * BC + L]] = f[i];
         } else {
             if (!BND || ((mask >> io) & 1u)) {
                 int dd, l;
                 shift_pos<Lat::cx[i], Lat::cy[i], Lat::cz[i]>(x, y, z, dd, l);
-                F[brick_base(tbl[wave][dd]) + i * BC + l] = f[i];
+                F[brick_base(tbl[wave][dd]) + slt[i * BC + l]] = f[i];
             } else {
-                F[base + io * BC + L] = f[i];
+                F[base + slt[io * BC + L]] = f[i];
             }
         }
     });
@@ -256,19 +419,19 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
 // ---- conversions between the AA storage and the reference (Q, N) layout --------------------------------------
 __device__ __forceinline__ uint32_t cell_mask(const cs_bricks_convert_args& a, int b, int L, int& bb) {
     bb = a.bidx[b];
-    return bb < 0 ? kAllLinks : a.lmask[(size_t)bb * BC + L];
+    return bb < 0 ? kAllLinks : a.lmask[(uint32_t)bb * BC + L];
 }
 
 // Location of P_i(n) in the AA storage for the state read by a step of parity `par`.
 template <int I>
-__device__ __forceinline__ size_t location(const cs_bricks_convert_args& a, int b, int L, uint32_t mask, int par) {
+__device__ __forceinline__ uint32_t location(const cs_bricks_convert_args& a, int b, int L, uint32_t mask, int par) {
     constexpr int io = Lat::opp[I];
-    if (par == 1 || I == 0 || !((mask >> io) & 1u)) return brick_base(b) + io * BC + L;
+    if (par == 1 || I == 0 || !((mask >> io) & 1u)) return brick_base(b) + kSlots.p[io][L];
     const int x = L & 3, y = (L >> 2) & 3, z = L >> 4;
     int dd, l;
     shift_pos<Lat::cx[I], Lat::cy[I], Lat::cz[I]>(x, y, z, dd, l);
     const int nbk = a.nbr[(size_t)b * 27 + dd];
-    return brick_base(nbk) + I * BC + l;
+    return brick_base(nbk) + kSlots.p[I][l];
 }
 
 template <bool SET>
@@ -279,10 +442,10 @@ __global__ __launch_bounds__(256) void k_convert(cs_bricks_convert_args a) {
     const int b = (int)(s / BC), L = (int)(s % BC);
     int bb;
     const uint32_t mask = cell_mask(a, b, L, bb);
-    const uint32_t sm = bb >= 0 ? a.smask[(size_t)bb * BC + L] : 0u;
+    const uint32_t sm = bb >= 0 ? a.smask[(uint32_t)bb * BC + L] : 0u;
     static_for<0, Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        const size_t loc = location<i>(a, b, L, mask, a.parity);
+        const uint32_t loc = location<i>(a, b, L, mask, a.parity);
         float* ref = a.f_ref + (size_t)i * a.n_cells + n;
         if constexpr (SET) {
             a.f[loc] = *ref;
@@ -321,6 +484,8 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
                a->model);
     CS_REQUIRE(a->parity == 0 || a->parity == 1, "cs_bricks_step: parity %d", a->parity);
     CS_REQUIRE(a->n_slots > 0 && a->n_slots % cs::BC == 0, "cs_bricks_step: n_slots %lld", (long long)a->n_slots);
+    CS_REQUIRE(a->n_slots * cs::Q < ((int64_t)1 << 32), "cs_bricks_step: %lld slots exceed 32-bit population offsets",
+               (long long)a->n_slots);
     if (a->n_list == 0) return CS_OK;
     CS_REQUIRE(a->n_list > 0 && a->n_list <= a->n_slots / cs::BC, "cs_bricks_step: n_list %lld",
                (long long)a->n_list);
@@ -342,6 +507,22 @@ static int convert(const cs_bricks_convert_args* a, void* stream, bool set) {
     if (set) cs::k_convert<true><<<grid, 256, 0, cs::as_stream(stream)>>>(*a);
     else cs::k_convert<false><<<grid, 256, 0, cs::as_stream(stream)>>>(*a);
     CS_LAUNCHED("k_convert");
+    return CS_OK;
+}
+
+extern "C" int cs_bricks_slot_table(uint16_t* out) {
+    CS_REQUIRE(out != nullptr, "cs_bricks_slot_table: null output");
+    constexpr cs::SlotTable t = cs::make_slots();
+    for (int k = 0; k < cs::Q; ++k)
+        for (int l = 0; l < cs::BC; ++l) out[k * cs::BC + l] = t.p[k][l];
+    return CS_OK;
+}
+
+extern "C" int cs_bricks_stage_table(uint16_t* out) {
+    CS_REQUIRE(out != nullptr, "cs_bricks_stage_table: null output");
+    constexpr cs::SlotTable t = cs::make_oddpos();
+    for (int k = 0; k < cs::Q; ++k)
+        for (int l = 0; l < cs::BC; ++l) out[k * cs::BC + l] = t.p[k][l];
     return CS_OK;
 }
 

@@ -99,7 +99,8 @@ int cs_step_indexed(const cs_step_args* a, void* stream);
 /* ---------------------------------------------------------------------------------------------------------------
  * Brick layout -- the MI355X storage of the same lattice (no reference counterpart: the reference keeps the
  * indexed layout of lbm.py:264-330 on the device). Fluid cells are grouped into 4 x 4 x 4 bricks (one wavefront per
- * brick); populations are stored once, per brick as [27][64] floats, and streamed in place with the AA pattern.
+ * brick); populations are stored once, per brick as a block of 27 x 64 floats, and streamed in place with the AA
+ * pattern.
  * The host builder converts a lattice in the reference layout (nbr/q/uin/coords as returned by build_lattice) and
  * checks that every fluid link of nbr is the implicit brick neighbour; it fails with CS_E_LAYOUT otherwise (the
  * caller then uses the indexed layout).
@@ -139,11 +140,18 @@ int cs_bricks_sizes_of(const void* handle, cs_bricks_sizes* out);
 int cs_bricks_copy(const void* handle, const char* name, void* dst);
 int cs_bricks_free(void* handle);
 
+/* Slot order inside a brick's block of 27 x 64 populations: population k of cell l (x + 4 y + 16 z) is at offset
+ * out[k * 64 + l] of the block (27 x 64 uint16). Slots are grouped by the brick whose odd step accesses them. */
+int cs_bricks_slot_table(uint16_t* out);
+/* Staging order of an interior brick's odd-step footprint (its own group, then the record of each neighbour in the
+ * order o = 0..12, 14..26): position of the population lane l reads for direction i is out[i * 64 + l]. */
+int cs_bricks_stage_table(uint16_t* out);
+
 /* One AA step over a list of bricks (two launches per step: interior bricks, then boundary bricks).
  * parity 0 = even step (reads and writes the cell's own slots), 1 = odd step (pulls from and pushes to the
  * neighbours). Velocity is stored as three component planes of n_slots floats; rho as one plane. */
 typedef struct cs_bricks_step_args {
-    float* f;                   /* [n_bricks][27][64] */
+    float* f;                   /* [n_bricks][1728], slot order of cs_bricks_slot_table */
     const int32_t* nbr;         /* [n_bricks][27] */
     const int32_t* list;        /* brick ids to process */
     int64_t n_list;

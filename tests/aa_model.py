@@ -30,6 +30,8 @@ class AAModel:
         self.uin = lat["uin"].astype(np.float64)
         self.N = self.nbr.shape[1]
         self.slot = layout["slot_of_cell"].astype(np.int64)
+        from cerebrosim_amd import bricks
+        self.slots = bricks.slot_table().astype(np.int64)
         self.S = np.zeros((layout["n_slots"] // 64 * 27 * 64,))
         self.side = np.zeros((27, self.N))
         self.om = np.full(self.N, 1.0 / (3 * nu + 0.5))
@@ -44,7 +46,7 @@ class AAModel:
 
     def addr(self, cells, k):
         s = self.slot[cells]
-        return (s // 64) * 27 * 64 + k * 64 + s % 64
+        return (s // 64) * 27 * 64 + self.slots[k][s % 64]
 
     def loc(self, i, par):
         """Address of P_i(n) for all n in the state read by a step of parity `par`."""

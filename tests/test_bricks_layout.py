@@ -111,3 +111,61 @@ def test_aa_stepping_matches_oracle(start_par):
         got = m.get_f(par)
         assert np.abs(got - ref).max() < 1e-13 * max(1.0, np.abs(ref).max()) * 100, step
         assert np.allclose(m.vel, o.vels[o.cur], atol=1e-15)
+
+
+def test_slot_table_groups():
+    """The slot table is a permutation of the 1728 block offsets; the odd-step accesses of each neighbour brick form
+    one contiguous record (1000 own slots, 100 per face, 10 per edge, 1 per corner)."""
+    t = bricks.slot_table().astype(np.int64)
+    assert sorted(t.ravel().tolist()) == list(range(27 * 64))
+    l = np.arange(64)
+    pos = np.stack([l % 4, (l // 4) % 4, l // 16], 1)
+    groups = {}
+    for k in range(27):
+        src = pos - L.C[k]
+        o = np.where(src < 0, 0, np.where(src > 3, 2, 1))
+        g = o[:, 0] + 3 * o[:, 1] + 9 * o[:, 2]
+        for gg, off in zip(g, t[k]):
+            groups.setdefault(int(gg), []).append(int(off))
+    sizes = {g: len(v) for g, v in groups.items()}
+    assert sizes[13] == 1000
+    for g, v in groups.items():
+        v = sorted(v)
+        assert v == list(range(v[0], v[0] + len(v))), g          # contiguous
+        nz = sum(1 for a in (g % 3, (g // 3) % 3, g // 9) if a != 1)
+        assert len(v) == {0: 1000, 1: 100, 2: 10, 3: 1}[nz]
+
+
+def test_odd_step_staging_maps_to_direct_slots():
+    """Interior bricks stage their odd-step footprint (own group + one record from each of the 26 neighbour blocks)
+    in LDS. Every staged position must stand for exactly the global slot the direct path accesses."""
+    slots = bricks.slot_table().astype(np.int64)
+    stage = bricks.stage_table().astype(np.int64)
+    order = [13] + [o for o in range(27) if o != 13]
+    # group g of a block: its slots, in block order
+    l = np.arange(64)
+    pos = np.stack([l % 4, (l // 4) % 4, l // 16], 1)
+    members = {}
+    for k in range(27):
+        src = pos - L.C[k]
+        o = np.where(src < 0, 0, np.where(src > 3, 2, 1))
+        g = o[:, 0] + 3 * o[:, 1] + 9 * o[:, 2]
+        for gg, off in zip(g, slots[k]):
+            members.setdefault(int(gg), []).append(int(off))
+    start = {g: min(v) for g, v in members.items()}
+    # staged footprint: (neighbour offset o, block offset)
+    foot = []
+    for o in order:
+        g = 13 if o == 13 else 26 - o
+        foot += [(o, start[g] + j) for j in range(len(members[g]))]
+    assert len(foot) == 27 * 64
+    for i in range(27):
+        src = pos - L.C[i]
+        o = np.where(src < 0, 0, np.where(src > 3, 2, 1))
+        oi = o[:, 0] + 3 * o[:, 1] + 9 * o[:, 2]
+        ml = (src[:, 0] % 4) + 4 * (src[:, 1] % 4) + 16 * (src[:, 2] % 4)
+        direct = [(int(a), int(slots[L.OPP[i]][m])) for a, m in zip(oi, ml)]
+        staged = [foot[p] for p in stage[i]]
+        assert staged == direct, i
+    # a bijection: the 27 x 64 reads cover the footprint once
+    assert sorted(stage.ravel().tolist()) == list(range(27 * 64))
