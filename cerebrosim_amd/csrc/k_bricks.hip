@@ -208,15 +208,15 @@ __device__ __forceinline__ float link_value(float code, float fo, bool opp_fluid
 
 template <int MODEL, bool GAL, bool BND, int PAR>
 __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
+    // Even steps stage the brick's own block in LDS: 27 full-row loads and stores instead of 27 gathers that each
+    // touch many lines of the block (in an even step no other wavefront touches the block). Odd steps access
+    // global memory directly (staging the 26 small neighbour records of an odd step costs more than the gathers).
+    constexpr bool STAGED = PAR == 0;
     __shared__ int tbl[WAVES][32];
-    // Even steps of interior bricks stage their own block in LDS: 27 full-row loads and stores instead of 27
-    // gathers that each touch many lines of the block. Odd steps and boundary bricks access global memory directly
-    // (staging the 26 small neighbour records of an odd step costs more than the direct gathers).
-    constexpr bool STAGED = !BND && PAR == 0;
     __shared__ uint32_t slots32[Q * BC / 2];
     __shared__ float stage_all[STAGED ? WAVES * Q * BC : 1];
     const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
-    load_table(slots32, STAGED && PAR == 1 ? kOddPos : kSlots);
+    load_table(slots32, kSlots);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int L = threadIdx.x & 63;
     // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
@@ -227,193 +227,199 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
     const int b = valid ? a.list[li] : 0;
     if (L < 27) tbl[wave][L] = valid ? a.nbr[(int64_t)b * 27 + L] : -1;
     __syncthreads();
-    if (!valid) return;
+    if (!valid) return;  // wave-uniform
 
     const int x = L & 3, y = (L >> 2) & 3, z = L >> 4;
     const uint32_t S = (uint32_t)a.n_slots;
     const uint32_t slot = (uint32_t)b * BC + L;
     const uint32_t base = brick_base(b);
     float* __restrict__ F = a.f;
+    float* stage = stage_all + (STAGED ? wave * Q * BC : 0);
 
     uint32_t mask = kAllLinks;
     int bb = 0;
     if constexpr (BND) {
-        bb = a.bidx[b];
+        bb = (int)li;  // the boundary list is in boundary-index order: bidx[list[li]] == li
         mask = a.lmask[(uint32_t)bb * BC + L];
-        if (mask == 0) return;  // solid slot
     }
-    const float om = a.om_brick[b] ? a.om_cell[slot] : a.omega;
-    const float d = a.drive[0];
+    const bool live = mask != 0u;  // solid slots of boundary bricks only join the staging copies
 
     // ---- streaming: raw values ---------------------------------------------------------------------------
     float f[Q];
-    float* stage = stage_all + (STAGED ? wave * Q * BC : 0);
     if constexpr (STAGED) {
         stage_copy<PAR, false>(F, stage, tbl[wave], base, L);
         wave_sync();
+        if (live)
+            static_for<0, Q>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                f[i] = stage[slt[i * BC + L]];
+            });
+    } else if (live) {
         static_for<0, Q>([&](auto ic) {
             constexpr int i = decltype(ic)::value;
-            f[i] = stage[slt[i * BC + L]];
-        });
-    } else
-    static_for<0, Q>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        constexpr int io = Lat::opp[i];
-        if constexpr (PAR == 0 || i == 0) {
-            f[i] = F[base + slt[i * BC + L]];
-        } else {
-            int dd, l;
-            shift_pos<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z, dd, l);
-            if (!BND || ((mask >> i) & 1u)) {
-                f[i] = F[brick_base(tbl[wave][dd]) + slt[io * BC + l]];
+            constexpr int io = Lat::opp[i];
+            if constexpr (i == 0) {
+                f[i] = F[base + slt[L]];
             } else {
-                f[i] = F[base + slt[i * BC + L]];
+                int dd, l;
+                shift_pos<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z, dd, l);
+                if (!BND || ((mask >> i) & 1u)) {
+                    f[i] = F[brick_base(tbl[wave][dd]) + slt[io * BC + l]];
+                } else {
+                    f[i] = F[base + slt[i * BC + L]];
+                }
+            }
+        });
+    }
+
+    if (live) {
+        const float om = a.om_brick[b] ? a.om_cell[slot] : a.omega;
+        const float d = a.drive[0];
+        // ---- boundary links and the previous velocity --------------------------------------------------------
+        float up[3] = {0.0f, 0.0f, 0.0f};
+        if constexpr (BND) {
+            up[0] = a.vel_old[slot];
+            up[1] = a.vel_old[S + slot];
+            up[2] = a.vel_old[2 * S + slot];
+            if ((mask & kAllLinks) != kAllLinks) {
+                const float upsq = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
+                float uind[3] = {0.0f, 0.0f, 0.0f};
+                if (mask & kInletBit) {
+                    uind[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
+                    uind[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
+                    uind[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
+                }
+                const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
+                const float* qc = a.qc + (uint32_t)bb * Q * BC + L;
+                const float* side = a.side + (uint32_t)bb * Q * BC + L;
+                static_for<1, Q>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    if constexpr (pair_head<i>()) {
+                        constexpr int j = i + 1;
+                        const bool fi = (mask >> i) & 1u, fj = (mask >> j) & 1u;
+                        const float ri = f[i], rj = f[j];
+                        float vi = ri, vj = rj;
+                        if (!fi) {
+                            const float ps = ((sm >> i) & 1u) ? side[i * BC] : rj;
+                            vi = link_value<i>(qc[i * BC], ri, fj, rj, ps, a.bouzidi, uind, up, upsq, a.p_out);
+                        }
+                        if (!fj) {
+                            const float ps = ((sm >> j) & 1u) ? side[j * BC] : ri;
+                            vj = link_value<j>(qc[j * BC], rj, fi, ri, ps, a.bouzidi, uind, up, upsq, a.p_out);
+                        }
+                        f[i] = vi;
+                        f[j] = vj;
+                    }
+                });
             }
         }
-    });
 
-    // ---- boundary links and the previous velocity ------------------------------------------------------------
-    float up[3] = {0.0f, 0.0f, 0.0f};
-    if constexpr (BND) {
-        up[0] = a.vel_old[slot];
-        up[1] = a.vel_old[S + slot];
-        up[2] = a.vel_old[2 * S + slot];
-        if ((mask & kAllLinks) != kAllLinks) {
-            const float upsq = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
-            float uind[3] = {0.0f, 0.0f, 0.0f};
-            if (mask & kInletBit) {
-                uind[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
-                uind[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
-                uind[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
-            }
-            const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
-            const float* qc = a.qc + (uint32_t)bb * Q * BC + L;
-            const float* side = a.side + (uint32_t)bb * Q * BC + L;
-            static_for<1, Q>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                if constexpr (pair_head<i>()) {
-                    constexpr int j = i + 1;
-                    const bool fi = (mask >> i) & 1u, fj = (mask >> j) & 1u;
-                    const float ri = f[i], rj = f[j];
-                    float vi = ri, vj = rj;
-                    if (!fi) {
-                        const float ps = ((sm >> i) & 1u) ? side[i * BC] : rj;
-                        vi = link_value<i>(qc[i * BC], ri, fj, rj, ps, a.bouzidi, uind, up, upsq, a.p_out);
+        // ---- Galilean correction: d u_a / d x_a of the previous velocity (lbm.py:105-126, 196-202) ------------
+        CollideParams p;
+        p.om = om;
+        p.om_m = MODEL == 0 ? 1.0f / (0.5f + a.magic / (1.0f / om - 0.5f)) : 0.0f;
+        p.F[0] = a.force[0] * d;
+        p.F[1] = a.force[1] * d;
+        p.F[2] = a.force[2] * d;
+        p.t3 = GAL ? 1.0f : 0.0f;
+        p.sigma = 1.0f;
+        p.g[0] = p.g[1] = p.g[2] = 0.0f;
+        if constexpr (GAL) {
+            static_for<0, 3>([&](auto ac) {
+                constexpr int ax = decltype(ac)::value;
+                constexpr int i1 = 2 * ax + 1, i2 = 2 * ax + 2;
+                int d1, l1, d2, l2;
+                shift_pos<ax == 0 ? -1 : 0, ax == 1 ? -1 : 0, ax == 2 ? -1 : 0>(x, y, z, d1, l1);
+                shift_pos<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z, d2, l2);
+                const float* V = a.vel_old + ax * S;
+                if constexpr (!BND) {
+                    const float v1 = V[(uint32_t)tbl[wave][d1] * BC + l1];
+                    const float v2 = V[(uint32_t)tbl[wave][d2] * BC + l2];
+                    p.g[ax] = (v2 - v1) * 0.5f;
+                } else {
+                    float h1 = 1.0f, v1 = up[ax], h2 = 1.0f, v2 = up[ax];
+                    if ((mask >> i1) & 1u) {
+                        v1 = V[(uint32_t)tbl[wave][d1] * BC + l1];
+                    } else {
+                        const float c = a.qc[((uint32_t)bb * Q + i1) * BC + L];
+                        if (c >= 0.0f) {
+                            h1 = fmaxf(c, 0.1f);
+                            v1 = 0.0f;
+                        }
                     }
-                    if (!fj) {
-                        const float ps = ((sm >> j) & 1u) ? side[j * BC] : ri;
-                        vj = link_value<j>(qc[j * BC], rj, fi, ri, ps, a.bouzidi, uind, up, upsq, a.p_out);
+                    if ((mask >> i2) & 1u) {
+                        v2 = V[(uint32_t)tbl[wave][d2] * BC + l2];
+                    } else {
+                        const float c = a.qc[((uint32_t)bb * Q + i2) * BC + L];
+                        if (c >= 0.0f) {
+                            h2 = fmaxf(c, 0.1f);
+                            v2 = 0.0f;
+                        }
                     }
-                    f[i] = vi;
-                    f[j] = vj;
+                    p.g[ax] = (h1 * h1 * (v2 - up[ax]) + h2 * h2 * (up[ax] - v1)) / (h1 * h2 * (h1 + h2));
                 }
             });
         }
-    }
 
-    // ---- Galilean correction: d u_a / d x_a of the previous velocity (lbm.py:105-126, 196-202) ----------------
-    CollideParams p;
-    p.om = om;
-    p.om_m = MODEL == 0 ? 1.0f / (0.5f + a.magic / (1.0f / om - 0.5f)) : 0.0f;
-    p.F[0] = a.force[0] * d;
-    p.F[1] = a.force[1] * d;
-    p.F[2] = a.force[2] * d;
-    p.t3 = GAL ? 1.0f : 0.0f;
-    p.sigma = 1.0f;
-    p.g[0] = p.g[1] = p.g[2] = 0.0f;
-    if constexpr (GAL) {
-        static_for<0, 3>([&](auto ac) {
-            constexpr int ax = decltype(ac)::value;
-            constexpr int i1 = 2 * ax + 1, i2 = 2 * ax + 2;
-            int d1, l1, d2, l2;
-            shift_pos<ax == 0 ? -1 : 0, ax == 1 ? -1 : 0, ax == 2 ? -1 : 0>(x, y, z, d1, l1);
-            shift_pos<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z, d2, l2);
-            const float* V = a.vel_old + ax * S;
-            if constexpr (!BND) {
-                const float v1 = V[(uint32_t)tbl[wave][d1] * BC + l1];
-                const float v2 = V[(uint32_t)tbl[wave][d2] * BC + l2];
-                p.g[ax] = (v2 - v1) * 0.5f;
-            } else {
-                float h1 = 1.0f, v1 = up[ax], h2 = 1.0f, v2 = up[ax];
-                if ((mask >> i1) & 1u) {
-                    v1 = V[(uint32_t)tbl[wave][d1] * BC + l1];
-                } else {
-                    const float c = a.qc[((uint32_t)bb * Q + i1) * BC + L];
-                    if (c >= 0.0f) {
-                        h1 = fmaxf(c, 0.1f);
-                        v1 = 0.0f;
-                    }
-                }
-                if ((mask >> i2) & 1u) {
-                    v2 = V[(uint32_t)tbl[wave][d2] * BC + l2];
-                } else {
-                    const float c = a.qc[((uint32_t)bb * Q + i2) * BC + L];
-                    if (c >= 0.0f) {
-                        h2 = fmaxf(c, 0.1f);
-                        v2 = 0.0f;
-                    }
-                }
-                p.g[ax] = (h1 * h1 * (v2 - up[ax]) + h2 * h2 * (up[ax] - v1)) / (h1 * h2 * (h1 + h2));
+        // ---- collision -----------------------------------------------------------------------------------------
+        float dr, u[3];
+        bool inlet_cell = false;
+        if constexpr (BND) inlet_cell = (mask & kInletBit) != 0u;
+        if (inlet_cell) {
+            dr = density_deviation(f);
+            u[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
+            u[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
+            u[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
+            inlet_equilibrium(f, dr, u[0], u[1], u[2], p.t3);
+        } else {
+            collide<MODEL, false>(f, p, dr, u);
+        }
+
+        // ---- outputs -------------------------------------------------------------------------------------------
+        a.vel[slot] = u[0];
+        a.vel[S + slot] = u[1];
+        a.vel[2 * S + slot] = u[2];
+        a.rho[slot] = dr;
+        if constexpr (BND) {
+            const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
+            if (sm) {
+                float* side = a.side + (uint32_t)bb * Q * BC + L;
+                static_for<1, Q>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    if ((sm >> i) & 1u) side[i * BC] = f[i];
+                });
             }
-        });
-    }
-
-    // ---- collision ---------------------------------------------------------------------------------------------
-    float dr, u[3];
-    bool inlet_cell = false;
-    if constexpr (BND) inlet_cell = (mask & kInletBit) != 0u;
-    if (inlet_cell) {
-        dr = density_deviation(f);
-        u[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
-        u[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
-        u[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
-        inlet_equilibrium(f, dr, u[0], u[1], u[2], p.t3);
-    } else {
-        collide<MODEL, false>(f, p, dr, u);
-    }
-
-    // ---- outputs -----------------------------------------------------------------------------------------------
-    a.vel[slot] = u[0];
-    a.vel[S + slot] = u[1];
-    a.vel[2 * S + slot] = u[2];
-    a.rho[slot] = dr;
-    if constexpr (BND) {
-        const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
-        if (sm) {
-            float* side = a.side + (uint32_t)bb * Q * BC + L;
-            static_for<1, Q>([&](auto ic) {
+        }
+        if constexpr (!STAGED) {
+            // odd: P_i(n) -> slot i of n + c_i (where the next even step of n + c_i reads it), or back into the own
+            // slot opp(i) when n + c_i is not fluid
+            static_for<0, Q>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
-                if ((sm >> i) & 1u) side[i * BC] = f[i];
+                constexpr int io = Lat::opp[i];
+                if constexpr (i == 0) {
+                    F[base + slt[L]] = f[i];
+                } else if (!BND || ((mask >> io) & 1u)) {
+                    int dd, l;
+                    shift_pos<Lat::cx[i], Lat::cy[i], Lat::cz[i]>(x, y, z, dd, l);
+                    F[brick_base(tbl[wave][dd]) + slt[i * BC + l]] = f[i];
+                } else {
+                    F[base + slt[io * BC + L]] = f[i];
+                }
             });
         }
     }
     if constexpr (STAGED) {
-        // even: P_i(n) -> own slot opp(i); odd: P_i(n) -> slot i of n + c_i = the slot read for direction opp(i)
+        // even: P_i(n) -> own slot opp(i), through the stage
         wave_sync();
-        static_for<0, Q>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            constexpr int io = Lat::opp[i];
-            stage[slt[io * BC + L]] = f[i];
-        });
+        if (live)
+            static_for<0, Q>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                constexpr int io = Lat::opp[i];
+                stage[slt[io * BC + L]] = f[i];
+            });
         wave_sync();
         stage_copy<PAR, true>(F, stage, tbl[wave], base, L);
-        return;
     }
-    static_for<0, Q>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        constexpr int io = Lat::opp[i];
-        if constexpr (PAR == 0 || i == 0) {
-            F[base + slt[io * BC + L]] = f[i];
-        } else {
-            if (!BND || ((mask >> io) & 1u)) {
-                int dd, l;
-                shift_pos<Lat::cx[i], Lat::cy[i], Lat::cz[i]>(x, y, z, dd, l);
-                F[brick_base(tbl[wave][dd]) + slt[i * BC + l]] = f[i];
-            } else {
-                F[base + slt[io * BC + L]] = f[i];
-            }
-        }
-    });
 }
 
 // ---- conversions between the AA storage and the reference (Q, N) layout --------------------------------------

@@ -153,7 +153,8 @@ int cs_bricks_stage_table(uint16_t* out);
 typedef struct cs_bricks_step_args {
     float* f;                   /* [n_bricks][1728], slot order of cs_bricks_slot_table */
     const int32_t* nbr;         /* [n_bricks][27] */
-    const int32_t* list;        /* brick ids to process */
+    const int32_t* list;        /* brick ids to process; for boundary bricks the "boundary" list of the builder
+                                   (its position li is the brick's boundary index) */
     int64_t n_list;
     const int32_t* bidx;        /* [n_bricks] */
     const uint32_t* lmask;      /* [n_boundary][64] */

@@ -1,0 +1,35 @@
+"""Copies the numbers the tests compare against out of the reference's committed data files (read as JSON/text, no
+reference code is run) into tests/golden/reference_data.json:
+
+* geometry/physics.json: the inflow waveform over one beat, 200 samples rounded to 5 decimals (vessel.py:253-265
+  generated it; physics.json:"inlet");
+* reports/step4_report.md:28: the calibrated circuit's resistances (GPa s/m^3).
+
+    python tests/golden/extract_reference_data.py [/path/to/reference]
+"""
+import json
+import os
+import re
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def main(ref):
+    phys = json.load(open(os.path.join(ref, "geometry", "physics.json")))
+    inlet = phys["inlet"]
+    report = open(os.path.join(ref, "reports", "step4_report.md")).read()
+    m = re.search(r"outlet resistances MCA ([\d.]+), ACA ([\d.]+) GPa s/m\^3; segment \(incl. extensions\) ([\d.]+)",
+                  report)
+    out = {"inflow_waveform": {"t_s": inlet["waveform_t_s"], "q_over_mean": inlet["waveform_Q_over_mean"],
+                               "mean_flow_mL_min": inlet["mean_flow_mL_min"],
+                               "pulsatility_index": inlet["pulsatility_index"],
+                               "source": "geometry/physics.json: inlet"},
+           "circuit_GPa_s_m3": {"R_mca": float(m.group(1)), "R_aca": float(m.group(2)), "R_seg": float(m.group(3)),
+                                "source": "reports/step4_report.md:28"}}
+    json.dump(out, open(os.path.join(HERE, "reference_data.json"), "w"), indent=1)
+    print("wrote reference_data.json")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
