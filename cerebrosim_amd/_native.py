@@ -42,6 +42,12 @@ class GatherArgs(ctypes.Structure):
                 ("n_probe", c_i32)]
 
 
+class WssArgs(ctypes.Structure):
+    _fields_ = [("idx", c_void_p), ("c", c_void_p), ("normal", c_void_p), ("n_vert", c_i64), ("k", c_i32),
+                ("vel", c_void_p), ("vel_elem", c_i64), ("vel_comp", c_i64), ("scale", c_float), ("tau", c_void_p),
+                ("acc_tau", c_void_p), ("acc_mag", c_void_p)]
+
+
 class LatticeIn(ctypes.Structure):
     _fields_ = [("n_cells", c_i64), ("shape", c_i32 * 3), ("periodic", c_i32 * 3), ("coords", c_void_p),
                 ("nbr", c_void_p), ("q", c_void_p), ("uin", c_void_p)]
@@ -83,6 +89,7 @@ SIGNATURES = {
     "cs_flux_bc": [ctypes.POINTER(FluxArgs), c_void_p],
     "cs_gather": [ctypes.POINTER(GatherArgs), c_void_p],
     "cs_reduce_finite": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
+    "cs_wss": [ctypes.POINTER(WssArgs), c_void_p],
     "cs_bricks_build": [ctypes.POINTER(LatticeIn), ctypes.POINTER(c_void_p)],
     "cs_bricks_sizes_of": [c_void_p, ctypes.POINTER(BricksSizes)],
     "cs_bricks_copy": [c_void_p, ctypes.c_char_p, c_void_p],

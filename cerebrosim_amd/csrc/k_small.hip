@@ -127,8 +127,49 @@ __global__ __launch_bounds__(64) void k_finite_final(const double* work, int nbl
     result[1] = b;
 }
 
+// ---- wall shear stress: one thread per wall vertex, K cells summed in order ------------------------------------
+__global__ __launch_bounds__(256) void k_wss(cs_wss_args a) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= a.n_vert) return;
+    const float n0 = a.normal[3 * v], n1 = a.normal[3 * v + 1], n2 = a.normal[3 * v + 2];
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+    for (int j = 0; j < a.k; ++j) {
+        const float cj = a.c[v * a.k + j];
+        if (cj == 0.0f) continue;
+        const float* u = a.vel + (int64_t)a.idx[v * a.k + j] * a.vel_elem;
+        const float u0 = u[0], u1 = u[a.vel_comp], u2 = u[2 * a.vel_comp];
+        const float un = u0 * n0 + u1 * n1 + u2 * n2;
+        t0 += cj * (u0 - un * n0);
+        t1 += cj * (u1 - un * n1);
+        t2 += cj * (u2 - un * n2);
+    }
+    t0 *= a.scale;
+    t1 *= a.scale;
+    t2 *= a.scale;
+    if (a.tau) {
+        a.tau[3 * v] = t0;
+        a.tau[3 * v + 1] = t1;
+        a.tau[3 * v + 2] = t2;
+    }
+    if (a.acc_tau) {
+        a.acc_tau[3 * v] += t0;
+        a.acc_tau[3 * v + 1] += t1;
+        a.acc_tau[3 * v + 2] += t2;
+    }
+    if (a.acc_mag) a.acc_mag[v] += sqrtf(t0 * t0 + t1 * t1 + t2 * t2);
+}
+
 }  // namespace
 }  // namespace cs
+
+extern "C" int cs_wss(const cs_wss_args* a, void* stream) {
+    CS_REQUIRE(a && a->idx && a->c && a->normal && a->vel, "cs_wss: null argument");
+    CS_REQUIRE(a->n_vert >= 0 && a->k > 0, "cs_wss: n_vert %lld, k %d", (long long)a->n_vert, a->k);
+    if (a->n_vert == 0) return CS_OK;
+    cs::k_wss<<<(unsigned)((a->n_vert + 255) / 256), 256, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_wss");
+    return CS_OK;
+}
 
 extern "C" int cs_clock(const cs_clock_args* a, void* stream) {
     CS_REQUIRE(a && a->tick && a->coef && a->drive, "cs_clock: null argument");

@@ -1,0 +1,92 @@
+"""Wall shear stress on the device (counterpart of wss.py).
+
+``weights`` builds, once per wall surface, the sparse least-squares coefficients of wss.py:35-50 on the host (K
+nearest lumen cells per wall vertex, Gaussian side weight, depth window, quadratic fit u_t = a d + b d^2 with a
+linear fallback below 6 cells). ``WallShear`` then evaluates tau = mu * 1e3 * sum c u_t for every frame with the
+``cs_wss`` kernel on the solver's live velocity (no field copies to the host) and accumulates the time-averaged WSS
+and the oscillatory shear index (wss.py:120-122).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+from . import _native
+
+MU = 3.5e-3       # Pa s (geometry/physics.json: blood viscosity)
+K = 48            # nearest lumen cells per wall vertex (wss.py:30)
+D_MIN, D_MAX = 0.02, 0.45      # mm: cells used in the fit, by distance to the wall (wss.py:31)
+SIGMA = 0.15      # mm: sideways fall-off of the weights (wss.py:32)
+
+
+def weights(verts, normals, cells, depth, k=K, radius=0.8, d_min=D_MIN, d_max=D_MAX, sigma=SIGMA):
+    """Per vertex: its k nearest cells and their coefficients c, so that the wall-normal derivative of the
+    tangential velocity is a = sum_j c_j u_t,j (per mm). Returns (idx (V, k), c (V, k) float32, quadratic, any)."""
+    dist, idx = cKDTree(cells).query(verts, k=k, distance_upper_bound=radius)
+    ok = np.isfinite(dist)
+    idx = np.where(ok, idx, 0)
+    d = depth[idx]
+    rel = cells[idx] - verts[:, None]
+    side = rel - (rel * normals[:, None]).sum(-1, keepdims=True) * normals[:, None]
+    w = np.exp(-(np.linalg.norm(side, axis=-1) / sigma) ** 2) * ok * (d > d_min) * (d < d_max)
+    s11, s12, s22 = (w * d ** 2).sum(1), (w * d ** 3).sum(1), (w * d ** 4).sum(1)
+    det = s11 * s22 - s12 ** 2
+    quad = (det > 1e-12 * np.maximum(s11 * s22, 1e-30)) & ((w > 0.05).sum(1) >= 6)
+    c = np.where(quad[:, None], w * (s22[:, None] * d - s12[:, None] * d ** 2) / np.where(quad, det, 1)[:, None],
+                 w * d / np.maximum(s11, 1e-30)[:, None])
+    c[s11 <= 0] = 0.0
+    return idx, c.astype(np.float32), quad, s11 > 0
+
+
+class WallShear:
+    """Device WSS over a fixed wall surface for one solver (or any velocity storage with the same addressing).
+
+    idx: (V, K) cell ids of the lattice (converted to storage slots), c: (V, K) coefficients, normals: (V, 3) into
+    the lumen, u_scale: physical velocity per solver velocity unit (m/s), mu: dynamic viscosity (Pa s)."""
+
+    def __init__(self, solver, idx, c, normals, u_scale=1.0, mu=MU):
+        import torch
+        self.s = solver
+        dev = solver.dev
+        V, k = idx.shape
+        slots = solver.backend.slots(np.asarray(idx).ravel()).reshape(V, k)
+        with torch.cuda.stream(solver._tstream):
+            self.idx = torch.from_numpy(np.ascontiguousarray(slots, np.int32)).to(dev)
+            self.c = torch.from_numpy(np.ascontiguousarray(c, np.float32)).to(dev)
+            self.n = torch.from_numpy(np.ascontiguousarray(normals, np.float32)).to(dev)
+            self.tau = torch.zeros((V, 3), dtype=torch.float32, device=dev)
+            self.acc_tau = torch.zeros((V, 3), dtype=torch.float32, device=dev)
+            self.acc_mag = torch.zeros(V, dtype=torch.float32, device=dev)
+        solver.synchronize()
+        self.V, self.k = V, k
+        self.scale = float(mu * 1e3 * u_scale)
+        self.frames = 0
+
+    def frame(self, accumulate=True):
+        """WSS (Pa) of the solver's newest velocity field at every vertex (device tensor (V, 3))."""
+        s = self.s
+        elem, comp = s.backend.velocity_addressing()
+        A = _native.WssArgs()
+        A.idx, A.c, A.normal, A.n_vert, A.k = self.idx.data_ptr(), self.c.data_ptr(), self.n.data_ptr(), self.V, self.k
+        A.vel, A.vel_elem, A.vel_comp = s.backend.velocity(s.cur).data_ptr(), elem, comp
+        A.scale = self.scale
+        A.tau = self.tau.data_ptr()
+        A.acc_tau = self.acc_tau.data_ptr() if accumulate else None
+        A.acc_mag = self.acc_mag.data_ptr() if accumulate else None
+        _native.call("cs_wss", ctypes.byref(A), s.stream_handle)
+        if accumulate:
+            self.frames += 1
+        return self.tau
+
+    def tawss(self):
+        self.s.synchronize()
+        return (self.acc_mag / max(1, self.frames)).cpu().numpy()
+
+    def osi(self):
+        self.s.synchronize()
+        n = max(1, self.frames)
+        mean = (self.acc_tau / n).norm(dim=1)
+        tawss = self.acc_mag / n
+        return (0.5 * (1 - mean / tawss.clamp_min(1e-9))).cpu().numpy()

@@ -242,6 +242,29 @@ typedef struct cs_gather_args {
 int cs_gather(const cs_gather_args* a, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Wall shear stress -- replaces shear() of wss.py:53-57 (numpy, per saved frame) with a device reduction on the live
+ * velocity field. Per wall vertex v: tau_v = scale * sum_k c[v,k] (u_k - (u_k . n_v) n_v) over its K cells (fixed
+ * order), u_k read at storage slot idx[v,k]. scale = mu * 1e3 * (m/s per velocity unit) turns the fit coefficients
+ * (per mm) into Pa. Optionally writes tau (V, 3) and accumulates sum tau (V, 3) and sum |tau| (V) for the
+ * time-averaged WSS and the oscillatory shear index of wss.py:120-122.
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct cs_wss_args {
+    const int32_t* idx;         /* device, (V, K) storage slots */
+    const float* c;             /* device, (V, K) fit coefficients, 0 for unused entries */
+    const float* normal;        /* device, (V, 3) unit normals into the lumen */
+    int64_t n_vert;
+    int32_t k;
+    const float* vel;
+    int64_t vel_elem;
+    int64_t vel_comp;
+    float scale;
+    float* tau;                 /* device, (V, 3), or NULL */
+    float* acc_tau;             /* device, (V, 3) += tau, or NULL */
+    float* acc_mag;             /* device, (V) += |tau|, or NULL */
+} cs_wss_args;
+int cs_wss(const cs_wss_args* a, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Health check -- replaces Solver.ok() (lbm.py:454-455), which copies rho to the host and tests isfinite. A
  * deterministic two-pass device reduction over x[0..n) writes result[0] = sum of the finite values (FP64) and
  * result[1] = number of non-finite values, so ok() reads back 16 bytes instead of 4 N. `work` is caller-provided
