@@ -88,6 +88,22 @@ def load_traffic(layout):
     return None
 
 
+def pmc_probe(layout):
+    """Short run for hardware-counter passes: the bench workload, 4 eager steps (2 even, 2 odd) and a 1 GiB device
+    copy whose counter readings calibrate FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    s = make_solver(build_workload(), layout)
+    s.run(4, chunk=1000)
+    s.synchronize()
+    x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print("pmc probe done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,11 +112,15 @@ def main():
     ap.add_argument("--layout", default="auto")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (the step of one lattice "
               f"runs on one device; independent cases shard across GPUs without exchange)", file=sys.stderr)
         sys.exit(2)
+    if args.pmc_probe:
+        pmc_probe(args.layout)
+        return
     import torch
     t0 = time.perf_counter()
     lat = build_workload()

@@ -78,6 +78,8 @@ class WallShear:
         _native.call("cs_wss", ctypes.byref(A), s.stream_handle)
         if accumulate:
             self.frames += 1
+        import torch
+        torch.cuda.current_stream(s.dev).wait_stream(s._tstream)   # later torch work on tau sees the result
         return self.tau
 
     def tawss(self):
