@@ -13,6 +13,7 @@ LIB_PATH = os.environ.get("CEREBRO_LBM_LIB") or os.path.join(HERE, "libcerebro_l
 
 c_int, c_i32, c_i64, c_float, c_void_p = ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_void_p
 CS_MODEL_TRT, CS_MODEL_REGULARISED = 0, 1
+CS_STORAGE_FP32, CS_STORAGE_FP16 = 0, 1
 CS_REDUCE_BLOCKS = 512
 CS_REDUCE_WORK = 2 * CS_REDUCE_BLOCKS
 
@@ -62,14 +63,15 @@ class BricksStepArgs(ctypes.Structure):
                 ("lmask", c_void_p), ("smask", c_void_p), ("qc", c_void_p), ("uin", c_void_p), ("side", c_void_p),
                 ("om_cell", c_void_p), ("om_brick", c_void_p), ("omega", c_float), ("p_out", c_void_p),
                 ("drive", c_void_p), ("vel_old", c_void_p), ("vel", c_void_p), ("rho", c_void_p),
-                ("n_slots", c_i64), ("force", c_float * 3), ("magic", c_float), ("bouzidi", c_i32),
-                ("model", c_i32), ("galilean", c_i32), ("parity", c_i32), ("boundary", c_i32)]
+                ("rho_old", c_void_p), ("n_slots", c_i64), ("force", c_float * 3), ("magic", c_float),
+                ("bouzidi", c_i32), ("model", c_i32), ("galilean", c_i32), ("parity", c_i32), ("boundary", c_i32),
+                ("storage", c_i32)]
 
 
 class BricksConvertArgs(ctypes.Structure):
     _fields_ = [("f", c_void_p), ("nbr", c_void_p), ("bidx", c_void_p), ("lmask", c_void_p), ("smask", c_void_p),
                 ("side", c_void_p), ("slot_of_cell", c_void_p), ("n_cells", c_i64), ("f_ref", c_void_p),
-                ("parity", c_i32)]
+                ("parity", c_i32), ("storage", c_i32), ("vel", c_void_p), ("rho", c_void_p), ("n_slots", c_i64)]
 
 
 # exported symbol -> argument types (restype is int for all but cs_last_error)

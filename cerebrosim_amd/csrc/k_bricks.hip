@@ -24,6 +24,8 @@
 //    (the missing link opp i's own raw value), otherwise it lives in a slot another thread rewrites during the
 //    step, so each such link keeps a copy in the side buffer, written by n at the end of the previous step.
 // Interior bricks (all cells fluid, all links fluid) run a separate instantiation without any boundary data.
+#include <type_traits>
+
 #include "collide.hpp"
 #include "common.hpp"
 
@@ -135,8 +137,8 @@ __device__ __forceinline__ void load_table(uint32_t* lds, const SlotTable& src) 
 
 // Move the footprint of brick b between global memory and the wave's LDS stage with full-row accesses.
 // Even steps: the whole own block. Odd steps: own group + the 26 records in the neighbour blocks.
-template <int PAR, bool STORE>
-__device__ __forceinline__ void stage_copy(float* __restrict__ F, float* stage, const int* tbl, uint32_t base, int L) {
+template <int PAR, bool STORE, typename T>
+__device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int* tbl, uint32_t base, int L) {
     if constexpr (PAR == 0) {
         static_for<0, Q>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
@@ -177,6 +179,53 @@ __device__ __forceinline__ void shift_pos(int x, int y, int z, int& d, int& l) {
     l = ((x + SX) & 3) + 4 * ((y + SY) & 3) + 16 * ((z + SZ) & 3);
 }
 
+// ---- FP16 storage -----------------------------------------------------------------------------------------------
+// A stored population is h = (f~ - feq2(rho', u of the cell that wrote it)) / w * kH16Scale in float16, where
+// feq2 = w (rho' + 3 c.u + 4.5 (c.u)^2) and rho' = drho - 1.5 |u|^2: the non-equilibrium part, which is small, keeps
+// the float16's 11 significant bits. Every slot is written by the cell x that owns the population P_i(x); the reader
+// decodes it with x's moments of the previous step (the velocity / density planes that step wrote). The writer of
+// the value a cell n reads for direction i is n - c_i when link i is fluid and n itself otherwise (both parities).
+constexpr float kH16Scale = 1024.0f;
+constexpr int kHalo = 6 * 6 * 6;  // the brick and a one-cell shell
+
+// Halo index of the position (x, y, z) + (SX, SY, SZ) in the 6^3 shell around the brick.
+template <int SX, int SY, int SZ>
+__device__ __forceinline__ int halo_index(int x, int y, int z) {
+    return (x + 1 + SX) + 6 * (y + 1 + SY) + 36 * (z + 1 + SZ);
+}
+
+template <int I>
+__device__ __forceinline__ _Float16 h16_encode(float fv, float rho_p, float u0, float u1, float u2) {
+    const float cu = cdot<I>(u0, u1, u2);
+    const float v = (fv * (1.0f / Lat::w[I]) - rho_p - cu * (3.0f + 4.5f * cu)) * kH16Scale;
+    return (_Float16)v;
+}
+
+template <int I>
+__device__ __forceinline__ float h16_decode(_Float16 h, const float4& m) {
+    const float cu = cdot<I>(m.y, m.z, m.w);
+    return Lat::w[I] * ((float)h * (1.0f / kH16Scale) + m.x + cu * (3.0f + 4.5f * cu));
+}
+
+// Stage the previous-step moments {drho - 1.5 |u|^2, ux, uy, uz} of the brick and its one-cell shell in LDS.
+__device__ __forceinline__ void load_halo(float4* halo, const int* tbl, const float* rho_old, const float* vel_old,
+                                          uint32_t S, int L) {
+    for (int e = L; e < kHalo; e += BC) {
+        const int hx = e % 6, hy = (e / 6) % 6, hz = e / 36;
+        const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
+                      9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
+        const int l = ((hx - 1) & 3) + 4 * ((hy - 1) & 3) + 16 * ((hz - 1) & 3);
+        const int nb = tbl[o];
+        float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (nb >= 0) {
+            const uint32_t sl = (uint32_t)nb * BC + l;
+            const float u0 = vel_old[sl], u1 = vel_old[S + sl], u2 = vel_old[2 * S + sl];
+            m = make_float4(rho_old[sl] - 1.5f * (u0 * u0 + u1 * u1 + u2 * u2), u0, u1, u2);
+        }
+        halo[e] = m;
+    }
+}
+
 
 // Value streamed into cell n through missing link I (wall / inlet / outlet), lbm.py:163-181.
 //   fo        raw value of slot I: P_opp(n)
@@ -206,15 +255,17 @@ __device__ __forceinline__ float link_value(float code, float fo, bool opp_fluid
     return -fo + 2.0f * Lat::w[I] * (3.0f * p_out[k] + 4.5f * cu * cu - 1.5f * upsq);
 }
 
-template <int MODEL, bool GAL, bool BND, int PAR>
+template <int MODEL, bool GAL, bool BND, int PAR, bool H16>
 __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
+    using T = std::conditional_t<H16, _Float16, float>;
     // Even steps stage the brick's own block in LDS: 27 full-row loads and stores instead of 27 gathers that each
     // touch many lines of the block (in an even step no other wavefront touches the block). Odd steps access
     // global memory directly (staging the 26 small neighbour records of an odd step costs more than the gathers).
     constexpr bool STAGED = PAR == 0;
     __shared__ int tbl[WAVES][32];
-    __shared__ uint32_t slots32[Q * BC / 2];
-    __shared__ float stage_all[STAGED ? WAVES * Q * BC : 1];
+    __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
+    __shared__ T stage_all[STAGED ? WAVES * Q * BC : 1];
+    __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
     const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
     load_table(slots32, kSlots);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -233,8 +284,13 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
     const uint32_t S = (uint32_t)a.n_slots;
     const uint32_t slot = (uint32_t)b * BC + L;
     const uint32_t base = brick_base(b);
-    float* __restrict__ F = a.f;
-    float* stage = stage_all + (STAGED ? wave * Q * BC : 0);
+    T* __restrict__ F = reinterpret_cast<T*>(a.f);
+    T* stage = stage_all + (STAGED ? wave * Q * BC : 0);
+    float4* halo = halo_all + (H16 ? wave * kHalo : 0);
+    if constexpr (H16) {
+        load_halo(halo, tbl[wave], a.rho_old, a.vel_old, S, L);
+        wave_sync();
+    }
 
     uint32_t mask = kAllLinks;
     int bb = 0;
@@ -246,27 +302,39 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
 
     // ---- streaming: raw values ---------------------------------------------------------------------------
     float f[Q];
+    // value of direction i as stored -> float. FP16: a fluid link holds P_i(n - c_i), a non-fluid one the bounced
+    // P_opp(i)(n); each is decoded with its writer's moments and its own direction.
+    auto value = [&](auto ic, T raw) -> float {
+        constexpr int i = decltype(ic)::value;
+        if constexpr (!H16) {
+            return raw;
+        } else {
+            if (!BND || ((mask >> i) & 1u))
+                return h16_decode<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
+            return h16_decode<Lat::opp[i]>(raw, halo[halo_index<0, 0, 0>(x, y, z)]);
+        }
+    };
     if constexpr (STAGED) {
         stage_copy<PAR, false>(F, stage, tbl[wave], base, L);
         wave_sync();
         if (live)
             static_for<0, Q>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
-                f[i] = stage[slt[i * BC + L]];
+                f[i] = value(ic, stage[slt[i * BC + L]]);
             });
     } else if (live) {
         static_for<0, Q>([&](auto ic) {
             constexpr int i = decltype(ic)::value;
             constexpr int io = Lat::opp[i];
             if constexpr (i == 0) {
-                f[i] = F[base + slt[L]];
+                f[i] = value(ic, F[base + slt[L]]);
             } else {
                 int dd, l;
                 shift_pos<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z, dd, l);
                 if (!BND || ((mask >> i) & 1u)) {
-                    f[i] = F[brick_base(tbl[wave][dd]) + slt[io * BC + l]];
+                    f[i] = value(ic, F[brick_base(tbl[wave][dd]) + slt[io * BC + l]]);
                 } else {
-                    f[i] = F[base + slt[i * BC + L]];
+                    f[i] = value(ic, F[base + slt[i * BC + L]]);
                 }
             }
         });
@@ -332,14 +400,25 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
                 shift_pos<ax == 0 ? -1 : 0, ax == 1 ? -1 : 0, ax == 2 ? -1 : 0>(x, y, z, d1, l1);
                 shift_pos<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z, d2, l2);
                 const float* V = a.vel_old + ax * S;
+                // neighbour velocity component ax of the previous step at x -+ e_ax
+                auto nbv = [&](bool minus) -> float {
+                    if constexpr (H16) {
+                        const int e = minus ? halo_index<ax == 0 ? -1 : 0, ax == 1 ? -1 : 0, ax == 2 ? -1 : 0>(x, y, z)
+                                            : halo_index<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z);
+                        const float4 m = halo[e];
+                        return ax == 0 ? m.y : (ax == 1 ? m.z : m.w);
+                    } else {
+                        return minus ? V[(uint32_t)tbl[wave][d1] * BC + l1] : V[(uint32_t)tbl[wave][d2] * BC + l2];
+                    }
+                };
                 if constexpr (!BND) {
-                    const float v1 = V[(uint32_t)tbl[wave][d1] * BC + l1];
-                    const float v2 = V[(uint32_t)tbl[wave][d2] * BC + l2];
+                    const float v1 = nbv(true);
+                    const float v2 = nbv(false);
                     p.g[ax] = (v2 - v1) * 0.5f;
                 } else {
                     float h1 = 1.0f, v1 = up[ax], h2 = 1.0f, v2 = up[ax];
                     if ((mask >> i1) & 1u) {
-                        v1 = V[(uint32_t)tbl[wave][d1] * BC + l1];
+                        v1 = nbv(true);
                     } else {
                         const float c = a.qc[((uint32_t)bb * Q + i1) * BC + L];
                         if (c >= 0.0f) {
@@ -348,7 +427,7 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
                         }
                     }
                     if ((mask >> i2) & 1u) {
-                        v2 = V[(uint32_t)tbl[wave][d2] * BC + l2];
+                        v2 = nbv(false);
                     } else {
                         const float c = a.qc[((uint32_t)bb * Q + i2) * BC + L];
                         if (c >= 0.0f) {
@@ -390,6 +469,14 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
                 });
             }
         }
+        if constexpr (H16) {
+            // from here on f holds the stored form: the deviation from this cell's new equilibrium
+            const float rp = dr - 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            static_for<0, Q>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                f[i] = (float)h16_encode<i>(f[i], rp, u[0], u[1], u[2]);
+            });
+        }
         if constexpr (!STAGED) {
             // odd: P_i(n) -> slot i of n + c_i (where the next even step of n + c_i reads it), or back into the own
             // slot opp(i) when n + c_i is not fluid
@@ -397,13 +484,13 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
                 constexpr int i = decltype(ic)::value;
                 constexpr int io = Lat::opp[i];
                 if constexpr (i == 0) {
-                    F[base + slt[L]] = f[i];
+                    F[base + slt[L]] = (T)f[i];
                 } else if (!BND || ((mask >> io) & 1u)) {
                     int dd, l;
                     shift_pos<Lat::cx[i], Lat::cy[i], Lat::cz[i]>(x, y, z, dd, l);
-                    F[brick_base(tbl[wave][dd]) + slt[i * BC + l]] = f[i];
+                    F[brick_base(tbl[wave][dd]) + slt[i * BC + l]] = (T)f[i];
                 } else {
-                    F[base + slt[io * BC + L]] = f[i];
+                    F[base + slt[io * BC + L]] = (T)f[i];
                 }
             });
         }
@@ -415,7 +502,7 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
             static_for<0, Q>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 constexpr int io = Lat::opp[i];
-                stage[slt[io * BC + L]] = f[i];
+                stage[slt[io * BC + L]] = (T)f[i];
             });
         wave_sync();
         stage_copy<PAR, true>(F, stage, tbl[wave], base, L);
@@ -440,8 +527,10 @@ __device__ __forceinline__ uint32_t location(const cs_bricks_convert_args& a, in
     return brick_base(nbk) + kSlots.p[I][l];
 }
 
-template <bool SET>
+// P_i(n) is written by n itself, so the FP16 form is encoded and decoded with n's own newest moments.
+template <bool SET, bool H16>
 __global__ __launch_bounds__(256) void k_convert(cs_bricks_convert_args a) {
+    using T = std::conditional_t<H16, _Float16, float>;
     const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (n >= a.n_cells) return;
     const int64_t s = a.slot_of_cell[n];
@@ -449,15 +538,23 @@ __global__ __launch_bounds__(256) void k_convert(cs_bricks_convert_args a) {
     int bb;
     const uint32_t mask = cell_mask(a, b, L, bb);
     const uint32_t sm = bb >= 0 ? a.smask[(uint32_t)bb * BC + L] : 0u;
+    T* F = reinterpret_cast<T*>(a.f);
+    float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if constexpr (H16) {
+        const float u0 = a.vel[s], u1 = a.vel[a.n_slots + s], u2 = a.vel[2 * a.n_slots + s];
+        m = make_float4(a.rho[s] - 1.5f * (u0 * u0 + u1 * u1 + u2 * u2), u0, u1, u2);
+    }
     static_for<0, Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
         const uint32_t loc = location<i>(a, b, L, mask, a.parity);
         float* ref = a.f_ref + (size_t)i * a.n_cells + n;
         if constexpr (SET) {
-            a.f[loc] = *ref;
+            if constexpr (H16) F[loc] = h16_encode<i>(*ref, m.x, m.y, m.z, m.w);
+            else F[loc] = *ref;
             if ((sm >> i) & 1u) a.side[((size_t)bb * Q + i) * BC + L] = *ref;
         } else {
-            *ref = a.f[loc];
+            if constexpr (H16) *ref = h16_decode<i>(F[loc], m);
+            else *ref = F[loc];
         }
     });
 }
@@ -466,8 +563,14 @@ template <int MODEL, bool GAL, bool BND>
 hipError_t launch_par(const cs_bricks_step_args& a, hipStream_t s) {
     const int64_t groups = (a.n_list + WAVES - 1) / WAVES;
     const unsigned grid = (unsigned)(((groups + 7) / 8) * 8);
-    if (a.parity == 0) k_bricks<MODEL, GAL, BND, 0><<<grid, WG, 0, s>>>(a);
-    else k_bricks<MODEL, GAL, BND, 1><<<grid, WG, 0, s>>>(a);
+    const bool h = a.storage == CS_STORAGE_FP16;
+    if (a.parity == 0) {
+        if (h) k_bricks<MODEL, GAL, BND, 0, true><<<grid, WG, 0, s>>>(a);
+        else k_bricks<MODEL, GAL, BND, 0, false><<<grid, WG, 0, s>>>(a);
+    } else {
+        if (h) k_bricks<MODEL, GAL, BND, 1, true><<<grid, WG, 0, s>>>(a);
+        else k_bricks<MODEL, GAL, BND, 1, false><<<grid, WG, 0, s>>>(a);
+    }
     return hipGetLastError();
 }
 
@@ -489,6 +592,10 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
     CS_REQUIRE(a->model == CS_MODEL_TRT || a->model == CS_MODEL_REGULARISED, "cs_bricks_step: unknown model %d",
                a->model);
     CS_REQUIRE(a->parity == 0 || a->parity == 1, "cs_bricks_step: parity %d", a->parity);
+    CS_REQUIRE(a->storage == CS_STORAGE_FP32 || a->storage == CS_STORAGE_FP16, "cs_bricks_step: storage %d",
+               a->storage);
+    CS_REQUIRE(a->storage == CS_STORAGE_FP32 || (a->rho_old && a->rho_old != a->rho),
+               "cs_bricks_step: FP16 storage needs the previous density plane rho_old (distinct from rho)");
     CS_REQUIRE(a->n_slots > 0 && a->n_slots % cs::BC == 0, "cs_bricks_step: n_slots %lld", (long long)a->n_slots);
     CS_REQUIRE(a->n_slots * cs::Q < ((int64_t)1 << 32), "cs_bricks_step: %lld slots exceed 32-bit population offsets",
                (long long)a->n_slots);
@@ -508,10 +615,20 @@ static int convert(const cs_bricks_convert_args* a, void* stream, bool set) {
     CS_REQUIRE(a && a->f && a->nbr && a->bidx && a->slot_of_cell && a->f_ref && a->lmask && a->smask && a->side,
                "cs_bricks convert: null argument");
     CS_REQUIRE(a->parity == 0 || a->parity == 1, "cs_bricks convert: parity %d", a->parity);
+    CS_REQUIRE(a->storage == CS_STORAGE_FP32 || a->storage == CS_STORAGE_FP16, "cs_bricks convert: storage %d",
+               a->storage);
+    CS_REQUIRE(a->storage == CS_STORAGE_FP32 || (a->vel && a->rho && a->n_slots > 0),
+               "cs_bricks convert: FP16 storage needs the newest vel/rho planes and n_slots");
     if (a->n_cells == 0) return CS_OK;
     const unsigned grid = (unsigned)((a->n_cells + 255) / 256);
-    if (set) cs::k_convert<true><<<grid, 256, 0, cs::as_stream(stream)>>>(*a);
-    else cs::k_convert<false><<<grid, 256, 0, cs::as_stream(stream)>>>(*a);
+    hipStream_t st = cs::as_stream(stream);
+    if (a->storage == CS_STORAGE_FP16) {
+        if (set) cs::k_convert<true, true><<<grid, 256, 0, st>>>(*a);
+        else cs::k_convert<false, true><<<grid, 256, 0, st>>>(*a);
+    } else {
+        if (set) cs::k_convert<true, false><<<grid, 256, 0, st>>>(*a);
+        else cs::k_convert<false, false><<<grid, 256, 0, st>>>(*a);
+    }
     CS_LAUNCHED("k_convert");
     return CS_OK;
 }

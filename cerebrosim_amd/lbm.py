@@ -67,6 +67,9 @@ class _IndexedBackend:
     def set_omega(self, om):
         self.om.copy_(_torch().from_numpy(np.ascontiguousarray(om, np.float32)))
 
+    def density(self, cur):
+        return self.rho
+
     def fields(self, cur):
         return self.rho.cpu().numpy(), self.vels[cur].cpu().numpy()
 
@@ -105,26 +108,32 @@ class _IndexedBackend:
     def slots(self, cells):
         return np.asarray(cells, np.int32)
 
-    def set_velocity(self, cur, u):
+    def set_velocity(self, cur, u, s=None):
         self.vels[cur].copy_(_torch().from_numpy(np.ascontiguousarray(u, np.float32)))
 
 
 class _BricksBackend:
     """4^3 brick storage with AA in-place streaming (see csrc/k_bricks.hip): populations [bricks][27][64] once,
-    velocity as three planes of n_slots floats (double-buffered), density one plane. Two launches per step: the
-    interior bricks (no boundary data) and the boundary bricks."""
+    velocity as three planes of n_slots floats (double-buffered), density one plane (two with FP16 storage, whose
+    decoding reads the previous step's density of the neighbours). Two launches per step: the interior bricks (no
+    boundary data) and the boundary bricks.
+
+    storage "fp32": populations stored as float. "fp16": stored as float16 deviations from the writing cell's
+    second-order equilibrium (scaled by 1024 / w_i); the arithmetic stays float32, the side buffer float32."""
 
     name = "bricks"
 
-    def __init__(self, s, lat, layout):
+    def __init__(self, s, lat, layout, storage="fp32"):
         torch = _torch()
         dev = s.dev
         self.L = layout
         self.S = S = int(layout["n_slots"])
         self.nb = int(layout["n_bricks"])
         self.n_cells = s.N
+        self.storage = {"fp32": _native.CS_STORAGE_FP32, "fp16": _native.CS_STORAGE_FP16}[storage]
         up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-        self.f = torch.zeros(self.nb * 27 * 64, dtype=torch.float32, device=dev)
+        dt = torch.float16 if self.storage == _native.CS_STORAGE_FP16 else torch.float32
+        self.f = torch.zeros(self.nb * 27 * 64, dtype=dt, device=dev)
         self.nbr = up(layout["nbr"])
         self.bidx = up(layout["bidx"])
         self.interior = up(layout["interior"])
@@ -138,7 +147,8 @@ class _BricksBackend:
         self.slot_of_cell = up(layout["slot_of_cell"].astype(np.int64))
         self.slot_of_cell32 = up(layout["slot_of_cell"])
         self.vels = [torch.zeros((3, S), dtype=torch.float32, device=dev) for _ in range(2)]
-        self.rho = torch.zeros(S, dtype=torch.float32, device=dev)
+        r = torch.zeros(S, dtype=torch.float32, device=dev)
+        self.rhos = [r, torch.zeros_like(r)] if self.storage == _native.CS_STORAGE_FP16 else [r, r]
         self.omega = float(np.float32(1.0 / s.tau))
         self.om = torch.full((S,), self.omega, dtype=torch.float32, device=dev)
         self.om_brick = torch.zeros(self.nb, dtype=torch.uint8, device=dev)
@@ -160,8 +170,9 @@ class _BricksBackend:
         A.qc, A.uin, A.side = self.qc.data_ptr(), self.uin.data_ptr(), self.side.data_ptr()
         A.om_cell, A.om_brick, A.omega = self.om.data_ptr(), self.om_brick.data_ptr(), self.omega
         A.p_out, A.drive = s.p_out.data_ptr(), s.drv.data_ptr()
-        A.vel_old, A.vel, A.rho = self.vels[a].data_ptr(), self.vels[1 - a].data_ptr(), self.rho.data_ptr()
-        A.n_slots = self.S
+        A.vel_old, A.vel = self.vels[a].data_ptr(), self.vels[1 - a].data_ptr()
+        A.rho, A.rho_old = self.rhos[1 - a].data_ptr(), self.rhos[a].data_ptr()
+        A.n_slots, A.storage = self.S, self.storage
         A.force = (ctypes.c_float * 3)(*s.force)
         A.magic = s.magic
         A.bouzidi, A.model, A.galilean = s.bouzidi, s.model, s.gal
@@ -189,6 +200,9 @@ class _BricksBackend:
         A.lmask, A.smask, A.side = self.lmask.data_ptr(), self.smask.data_ptr(), self.side.data_ptr()
         A.slot_of_cell, A.n_cells = self.slot_of_cell32.data_ptr(), self.n_cells
         A.f_ref, A.parity = f_ref.data_ptr(), parity
+        # the newest moments: those of the step that left this state (parity = the next step's parity)
+        A.storage, A.vel, A.rho, A.n_slots = self.storage, self.vels[parity].data_ptr(), \
+            self.rhos[parity].data_ptr(), self.S
         _native.call(name, ctypes.byref(A), s.stream_handle)
 
     def get_f(self, s, cur):
@@ -211,26 +225,34 @@ class _BricksBackend:
     def slots(self, cells):
         return self.L["slot_of_cell"][np.asarray(cells)].astype(np.int32)
 
+    def density(self, cur):
+        return self.rhos[cur]
+
     def fields(self, cur):
         idx = self.slot_of_cell
-        rho = self.rho.index_select(0, idx)
+        rho = self.rhos[cur].index_select(0, idx)
         u = self.vels[cur].index_select(1, idx).t()
         return rho.cpu().numpy(), u.contiguous().cpu().numpy()
 
-    def set_velocity(self, cur, u):
+    def set_velocity(self, cur, u, s=None):
         torch = _torch()
+        # FP16: the stored deviations are relative to these moments -- re-encode the populations around the new ones
+        f = self.get_f(s, cur) if self.storage == _native.CS_STORAGE_FP16 else None
         u = torch.from_numpy(np.ascontiguousarray(u, np.float32)).to(self.vels[cur].device)
         self.vels[cur][:, self.slot_of_cell] = u.t()
+        if f is not None:
+            self.set_f(s, cur, f)
 
 
 class Solver:
     """Lattice Boltzmann on a sparse lattice (lbm.py:346-455). ``model`` "trt" (magic = Lambda) or "regularised";
     ``sigma`` < 1 blends the regularised stress with the finite-difference one. ``layout`` selects the device
     storage: "bricks" (4^3 bricks, AA in-place streaming), "indexed" (reference (Q, N) layout, any neighbour table)
-    or "auto" (bricks when the lattice and options allow, else indexed)."""
+    or "auto" (bricks when the lattice and options allow, else indexed). ``storage`` "fp16" keeps the populations
+    in float16 (bricks layout only; float32 arithmetic and reductions)."""
 
     def __init__(self, lat, nu_lu, magic=3.0 / 16.0, bouzidi=True, model="trt", sigma=1.0, galilean=None,
-                 device=None, layout="auto"):
+                 device=None, layout="auto", storage="fp32"):
         torch = _torch()
         self.dev = _require_gpu(device)
         self.lat = lat
@@ -254,6 +276,9 @@ class Solver:
             self.drv = torch.zeros(1, dtype=torch.float32, device=dev)
             self._work = torch.zeros(_native.CS_REDUCE_WORK, dtype=torch.float64, device=dev)
             self._red = torch.zeros(2, dtype=torch.float64, device=dev)
+            if storage not in ("fp32", "fp16"):
+                raise ValueError(f"unknown storage {storage!r}")
+            self.storage = storage
             self.backend = self._make_backend(lat, layout)
         self._tstream.synchronize()
         self.layout = self.backend.name
@@ -268,17 +293,19 @@ class Solver:
         if layout not in ("auto", "bricks", "indexed"):
             raise ValueError(f"unknown layout {layout!r}")
         if layout == "indexed":
+            if self.storage != "fp32":
+                raise ValueError("storage 'fp16' needs the bricks layout")
             return _IndexedBackend(self, lat)
         reason = None
         if self.model == 1 and self.sigma < 1.0:
             reason = "the hybrid blend (sigma < 1) runs on the indexed layout"
         else:
             try:
-                return _BricksBackend(self, lat, bricks.build(lat))
+                return _BricksBackend(self, lat, bricks.build(lat), self.storage)
             except bricks.LayoutError as e:
                 reason = str(e)
-        if layout == "bricks":
-            raise ValueError(f"layout 'bricks' unavailable: {reason}")
+        if layout == "bricks" or self.storage != "fp32":
+            raise ValueError(f"layout 'bricks' (needed for storage {self.storage!r}) unavailable: {reason}")
         self.layout_note = reason
         return _IndexedBackend(self, lat)
 
@@ -293,7 +320,7 @@ class Solver:
 
     @property
     def rho(self):
-        return self.backend.rho
+        return self.backend.density(self.cur)
 
     @property
     def f(self):
@@ -398,7 +425,7 @@ class Solver:
 
     def ok(self):
         """All densities finite (device reduction; reads back 16 bytes)."""
-        x = self.backend.rho
+        x = self.backend.density(self.cur)
         _native.call("cs_reduce_finite", ctypes.c_void_p(x.data_ptr()), x.numel(),
                      ctypes.c_void_p(self._work.data_ptr()), ctypes.c_void_p(self._red.data_ptr()),
                      self.stream_handle)
@@ -423,5 +450,5 @@ class Solver:
         """Replace the newest velocity field (N, 3) (the previous-step velocity of the next step)."""
         self.synchronize()
         with _torch().cuda.stream(self._tstream):
-            self.backend.set_velocity(self.cur, u)
+            self.backend.set_velocity(self.cur, u, self)
         self.synchronize()

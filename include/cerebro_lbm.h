@@ -150,8 +150,10 @@ int cs_bricks_stage_table(uint16_t* out);
 /* One AA step over a list of bricks (two launches per step: interior bricks, then boundary bricks).
  * parity 0 = even step (reads and writes the cell's own slots), 1 = odd step (pulls from and pushes to the
  * neighbours). Velocity is stored as three component planes of n_slots floats; rho as one plane. */
+#define CS_STORAGE_FP32 0       /* populations stored as float32 deviations f - w */
+#define CS_STORAGE_FP16 1       /* populations stored as float16 of (f - feq2(writer's moments)) / w * 2^10 */
 typedef struct cs_bricks_step_args {
-    float* f;                   /* [n_bricks][1728], slot order of cs_bricks_slot_table */
+    void* f;                    /* [n_bricks][1728] float or half, slot order of cs_bricks_slot_table */
     const int32_t* nbr;         /* [n_bricks][27] */
     const int32_t* list;        /* brick ids to process; for boundary bricks the "boundary" list of the builder
                                    (its position li is the brick's boundary index) */
@@ -170,6 +172,7 @@ typedef struct cs_bricks_step_args {
     const float* vel_old;       /* [3][n_slots] */
     float* vel;                 /* [3][n_slots] */
     float* rho;                 /* [n_slots] */
+    const float* rho_old;       /* [n_slots] density of the previous step (FP16 storage only, else NULL) */
     int64_t n_slots;
     float force[3];
     float magic;
@@ -178,13 +181,14 @@ typedef struct cs_bricks_step_args {
     int32_t galilean;
     int32_t parity;
     int32_t boundary;           /* 1 = the list holds boundary bricks */
+    int32_t storage;            /* CS_STORAGE_* */
 } cs_bricks_step_args;
 int cs_bricks_step(const cs_bricks_step_args* a, void* stream);
 
 /* Conversions between the AA brick storage and the reference (Q, N) layout of post-collision populations, for the
  * state that the next step of the given parity reads. set_f also refreshes the side copies. */
 typedef struct cs_bricks_convert_args {
-    float* f;                   /* brick storage */
+    void* f;                    /* brick storage (float or half) */
     const int32_t* nbr;
     const int32_t* bidx;
     const uint32_t* lmask;
@@ -194,6 +198,10 @@ typedef struct cs_bricks_convert_args {
     int64_t n_cells;
     float* f_ref;               /* device (Q, N) */
     int32_t parity;             /* parity of the next step */
+    int32_t storage;            /* CS_STORAGE_* */
+    const float* vel;           /* FP16 storage: newest velocity planes [3][n_slots] (the cells' own moments) */
+    const float* rho;           /* FP16 storage: newest density plane [n_slots] */
+    int64_t n_slots;
 } cs_bricks_convert_args;
 int cs_bricks_get_f(const cs_bricks_convert_args* a, void* stream);
 int cs_bricks_set_f(const cs_bricks_convert_args* a, void* stream);

@@ -4,6 +4,10 @@ forms, redirected writes at boundaries, the side copies for Bouzidi q >= 1/2, an
 It uses the same host layout arrays as the HIP kernels (cerebrosim_amd.bricks.build) and the same rules as
 csrc/k_bricks.hip; addressing goes through the reference neighbour table and slot_of_cell (the builder verifies
 that these agree with the brick neighbour table). The collision is a vectorised FP64 statement of lbm.py:183-240.
+
+storage="fp16" models the FP16 form: a slot holds float16(1024 (P - feq2(writer's moments)) / w) of the population
+P it carries, feq2 / w = drho - 1.5 |u|^2 + 3 c.u + 4.5 (c.u)^2; a fluid link i of n carries P_i(n - c_i), a
+missing one the bounced P_opp(i)(n), each decoded with its writer's moments of the previous step.
 """
 import numpy as np
 
@@ -23,7 +27,9 @@ def h3(c, u):
 
 
 class AAModel:
-    def __init__(self, lat, layout, nu, force=(0, 0, 0), p_out=(0.0,), bouzidi=True):
+    def __init__(self, lat, layout, nu, force=(0, 0, 0), p_out=(0.0,), bouzidi=True, storage="fp64"):
+        self.fp16 = storage == "fp16"
+        self.bounced_dir = True
         self.lat, self.Lay = lat, layout
         self.nbr = lat["nbr"]
         self.q = lat["q"].astype(np.float64)
@@ -58,13 +64,31 @@ class AAModel:
             a[m] = self.addr(self.nbr[io, m], i)
         return a
 
+    # FP16 form: population value <-> stored value for direction d with the writer cells' moments
+    def _eq(self, d, cells):
+        u = self.vel[cells]
+        cu = u @ C[d]
+        return self.rho[cells] - 1.5 * (u ** 2).sum(1) + cu * (3 + 4.5 * cu)
+
+    def encode(self, v, d, cells):
+        if not self.fp16:
+            return v
+        return ((v / W[d] - self._eq(d, cells)) * 1024.0).astype(np.float32).astype(np.float16).astype(np.float64)
+
+    def decode(self, h, d, cells):
+        if not self.fp16:
+            return h
+        return W[d] * (h / 1024.0 + self._eq(d, cells))
+
     def set_f(self, f, par):
+        n = np.arange(self.N)
         for i in range(27):
-            self.S[self.loc(i, par)] = f[i]
+            self.S[self.loc(i, par)] = self.encode(f[i], i, n)
         self.side[self.sm] = f[self.sm]
 
     def get_f(self, par):
-        return np.stack([self.S[self.loc(i, par)] for i in range(27)])
+        n = np.arange(self.N)
+        return np.stack([self.decode(self.S[self.loc(i, par)], i, n) for i in range(27)])
 
     def step(self, par, drive=1.0):
         N, n = self.N, np.arange(self.N)
@@ -75,6 +99,13 @@ class AAModel:
                 m = self.fluid[i]
                 a[m] = self.addr(self.nbr[i, m], OPP[i])
             raw[i] = self.S[a]
+            if self.fp16:
+                fl = self.fluid[i] if i != 0 else np.ones(N, bool)
+                wr = np.where(fl, self.nbr[i], n) if i != 0 else n
+                dec_i = self.decode(raw[i], i, wr)
+                # bounced_dir False plants a mistake for the tests: decode a bounced population with direction i
+                dec_o = self.decode(raw[i], OPP[i] if self.bounced_dir else i, n)
+                raw[i] = np.where(fl, dec_i, dec_o)
         f = raw.copy()
         up = self.vel
         upsq = (up ** 2).sum(1)
@@ -109,7 +140,7 @@ class AAModel:
             if par == 1 and i != 0:
                 m = self.fluid[io]
                 a[m] = self.addr(self.nbr[io, m], i)
-            self.S[a] = post[i]
+            self.S[a] = self.encode(post[i], i, n)
         self.side[self.sm] = post[self.sm]
 
     def ddx(self, ax, comp):

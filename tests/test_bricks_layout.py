@@ -169,3 +169,31 @@ def test_odd_step_staging_maps_to_direct_slots():
         assert staged == direct, i
     # a bijection: the 27 x 64 reads cover the footprint once
     assert sorted(stage.ravel().tolist()) == list(range(27 * 64))
+
+
+def run_fp16_model(steps, bounced_dir=True):
+    """FP16-form AA model and FP64 oracle on the tilted pipe from rest, inlet drive 0.03, outlet p = 0."""
+    lat = tilted_pipe()
+    lay = bricks.build(lat)
+    nu = 0.05
+    o = oracle.OracleSolver(lat, nu, model="regularised", dtype=np.float64, threads=1)
+    o.set_outlet_pressure([0.0])
+    m = AAModel(lat, lay, nu, p_out=[0.0], storage="fp16")
+    m.bounced_dir = bounced_dir
+    par = 0
+    for _ in range(steps):
+        o.step_once(0.03)
+        m.step(par, drive=0.03)
+        par = 1 - par
+    uo = o.vels[o.cur]
+    return float(np.linalg.norm(m.vel - uo) / np.linalg.norm(uo))
+
+
+def test_aa_fp16_form_matches_fp64_oracle():
+    """The FP16 equilibrium-deviation storage rule (writer's moments, bounced links decoded with their own
+    direction) keeps u within 1e-3 of the FP64 oracle; decoding bounced populations with the link's direction (a
+    planted mistake) does not."""
+    e = run_fp16_model(120)
+    assert e <= 1e-3, e
+    bad = run_fp16_model(120, bounced_dir=False)
+    assert bad > 10 * max(e, 1e-4), (e, bad)
