@@ -140,10 +140,17 @@ __device__ __forceinline__ void load_table(uint32_t* lds, const SlotTable& src) 
 template <int PAR, bool STORE, typename T>
 __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int* tbl, uint32_t base, int L) {
     if constexpr (PAR == 0) {
-        static_for<0, Q>([&](auto jc) {
+        // the whole block as 16-byte vectors (a block is Q * BC * sizeof(T) bytes, a multiple of 16)
+        constexpr int NV = Q * BC * (int)sizeof(T) / 16;
+        uint4* g = reinterpret_cast<uint4*>(F + base);
+        uint4* l = reinterpret_cast<uint4*>(stage);
+        static_for<0, (NV + BC - 1) / BC>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            if constexpr (STORE) F[base + j * BC + L] = stage[j * BC + L];
-            else stage[j * BC + L] = F[base + j * BC + L];
+            const int k = j * BC + L;
+            if (NV % BC == 0 || (j + 1) * BC <= NV || k < NV) {
+                if constexpr (STORE) g[k] = l[k];
+                else l[k] = g[k];
+            }
         });
     } else {
         static_for<0, 27>([&](auto oc) {
@@ -164,6 +171,7 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
 }
 
 static_assert(Q * BC * 2 % 16 == 0 && Q * BC * 2 / 16 <= WG, "table copy layout");
+static_assert((Q * BC * 2) % 16 == 0, "a block of float16 populations is a whole number of 16-byte vectors");
 
 // Brick-table index d and local index of the position (x, y, z) + (SX, SY, SZ), SX.. in {-1, 0, 1}.
 template <int SX, int SY, int SZ>
@@ -264,7 +272,7 @@ __global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
     constexpr bool STAGED = PAR == 0;
     __shared__ int tbl[WAVES][32];
     __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
-    __shared__ T stage_all[STAGED ? WAVES * Q * BC : 1];
+    __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
     __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
     const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
     load_table(slots32, kSlots);
