@@ -14,6 +14,7 @@ LIB_PATH = os.environ.get("CEREBRO_LBM_LIB") or os.path.join(HERE, "libcerebro_l
 c_int, c_i32, c_i64, c_float, c_void_p = ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_void_p
 CS_MODEL_TRT, CS_MODEL_REGULARISED = 0, 1
 CS_STORAGE_FP32, CS_STORAGE_FP16 = 0, 1
+CS_CASE_SHIFT, CS_MAX_CASES, CS_LIST_SKIP = 22, 1024, 0xFFFFFFFF
 CS_REDUCE_BLOCKS = 512
 CS_REDUCE_WORK = 2 * CS_REDUCE_BLOCKS
 
@@ -29,6 +30,18 @@ class StepArgs(ctypes.Structure):
                 ("vel_old", c_void_p), ("vel", c_void_p), ("rho", c_void_p), ("n_cells", c_i64),
                 ("force", c_float * 3), ("magic", c_float), ("sigma", c_float), ("bouzidi", c_i32),
                 ("model", c_i32), ("galilean", c_i32)]
+
+
+class ClockCasesArgs(ctypes.Structure):
+    _fields_ = [("tick", c_void_p), ("coef", c_void_p), ("nh", c_void_p), ("period", c_void_p), ("ramp", c_void_p),
+                ("drive", c_void_p), ("n_cases", c_i32), ("coef_stride", c_i32)]
+
+
+class FluxCasesArgs(ctypes.Structure):
+    _fields_ = [("idx", c_void_p), ("w", c_void_p), ("slot", c_void_p), ("start", c_void_p), ("vel", c_void_p),
+                ("vel_elem", c_i64), ("vel_comp", c_i64), ("R", c_void_p), ("drive", c_void_p), ("p_out", c_void_p),
+                ("q_last", c_void_p), ("n_open", c_void_p), ("pref", c_void_p), ("alpha", c_void_p),
+                ("n_cases", c_i32), ("open_stride", c_i32)]
 
 
 class FluxArgs(ctypes.Structure):
@@ -65,7 +78,8 @@ class BricksStepArgs(ctypes.Structure):
                 ("drive", c_void_p), ("vel_old", c_void_p), ("vel", c_void_p), ("rho", c_void_p),
                 ("rho_old", c_void_p), ("n_slots", c_i64), ("force", c_float * 3), ("magic", c_float),
                 ("bouzidi", c_i32), ("model", c_i32), ("galilean", c_i32), ("parity", c_i32), ("boundary", c_i32),
-                ("storage", c_i32)]
+                ("storage", c_i32), ("case_omega", c_void_p), ("case_force", c_void_p), ("p_out_stride", c_i32),
+                ("variant", c_i32)]
 
 
 class BricksConvertArgs(ctypes.Structure):
@@ -89,6 +103,9 @@ SIGNATURES = {
     "cs_clock": [ctypes.POINTER(ClockArgs), c_void_p],
     "cs_step_indexed": [ctypes.POINTER(StepArgs), c_void_p],
     "cs_flux_bc": [ctypes.POINTER(FluxArgs), c_void_p],
+    "cs_clock_cases": [ctypes.POINTER(ClockCasesArgs), c_void_p],
+    "cs_flux_cases": [ctypes.POINTER(FluxCasesArgs), c_void_p],
+    "cs_bricks_variant_info": [c_i32, c_void_p, c_void_p],
     "cs_gather": [ctypes.POINTER(GatherArgs), c_void_p],
     "cs_reduce_finite": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
     "cs_wss": [ctypes.POINTER(WssArgs), c_void_p],

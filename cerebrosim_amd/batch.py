@@ -1,0 +1,682 @@
+"""Batching form: many lattices (cases) of different sizes stepped together by one launch per brick arena.
+
+Design (MI355X-first):
+
+* An ``Arena`` is one packed brick storage on the device: populations ``[bricks][1728]`` (float or half), neighbour
+  table, velocity / density planes, per-cell omega, and the boundary records. Cases take a contiguous range of
+  bricks and a contiguous range of boundary records from first-fit allocators; their neighbour ids are offset by
+  the range start on upload, so the kernels address the arena exactly as they address one lattice.
+* The brick lists of an arena (interior, boundary) hold ``brick | case << 22``; the kernel reads the case's drive,
+  base omega, body force and opening pressures from per-case tables. The boundary list is indexed by boundary
+  record (unused records hold a skip marker), so a boundary brick's record index is its list position.
+* One step = one clock launch for all cases + two step launches per arena (interior, boundary) + one flow launch
+  per arena when cases measure flows (simulate.py's k_flux + k_bc). ``run`` captures chunks of steps in a graph,
+  re-captured only when cases join or leave.
+* An arena holds at most 2^32 / 1728 - 1 bricks (32-bit population offsets, about 159 M cells); a batch holds
+  several arenas (one storage type each). Cases are independent, so results do not depend on what else is in the
+  batch: the tests check that a case gives the same bits alone and in a mix.
+
+The case handle (``Case``) has the ``lbm.Solver`` surface used by the callers: ``set_drive``, ``set_force``,
+``set_sponge``, ``set_outlet_pressure``, ``fields``, ``get_f``/``set_f``, ``set_velocity``, ``ok``, plus the
+resistance-outlet flow measurement of ``artery.ArterySolver`` (``set_outlets``, ``flows``, ``opening_pressures``).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .lbm import _require_gpu, sponge_tau
+
+BC = 64
+BLOCK = 27 * BC
+MAX_BRICKS = (2 ** 32 - 1) // BLOCK          # 32-bit population offsets per arena
+STORAGES = {"fp32": _native.CS_STORAGE_FP32, "fp16": _native.CS_STORAGE_FP16}
+MODELS = {"trt": 0, "regularised": 1}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class BatchFull(RuntimeError):
+    """No arena has room for the case (bricks, boundary records or case slots)."""
+
+
+class _Ranges:
+    """First-fit allocator of contiguous ranges in [0, capacity)."""
+
+    def __init__(self, capacity):
+        self.capacity = int(capacity)
+        self.free = [(0, self.capacity)] if capacity > 0 else []
+
+    def alloc(self, n):
+        if n == 0:
+            return 0
+        for k, (s, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(k)
+                else:
+                    self.free[k] = (s + n, ln - n)
+                return s
+        return None
+
+    def release(self, start, n):
+        if n == 0:
+            return
+        self.free.append((start, n))
+        self.free.sort()
+        merged = []
+        for s, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == s:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((s, ln))
+        self.free = merged
+
+    def high(self):
+        """End of the highest allocated range."""
+        if not self.free or self.free[-1][0] + self.free[-1][1] != self.capacity:
+            return self.capacity
+        return self.free[-1][0]
+
+
+def arena_bytes(bricks, records, storage):
+    """Device bytes of an arena with the given capacities."""
+    elem = 2 if storage == "fp16" else 4
+    per_brick = BLOCK * elem + 27 * 4 + 4 + 1 + BC * 4 + 2 * 3 * BC * 4 + BC * 4 * (2 if storage == "fp16" else 1) + 4
+    per_record = BC * (4 + 4 + 27 * 4 + 3 * 4 + 27 * 4) + 4
+    return int(bricks) * per_brick + int(records) * per_record
+
+
+def case_bytes(layout, storage):
+    """Device bytes a case occupies inside an arena (its bricks and boundary records)."""
+    return arena_bytes(layout["n_bricks"], layout["n_boundary"], storage)
+
+
+class Arena:
+    """Packed brick storage for the cases of one storage type."""
+
+    def __init__(self, batch, bricks, records):
+        torch = _torch()
+        if not 0 < bricks <= MAX_BRICKS:
+            raise ValueError(f"arena of {bricks} bricks: 1 .. {MAX_BRICKS} (32-bit population offsets)")
+        dev = batch.dev
+        self.batch = batch
+        self.cap_b, self.cap_r = int(bricks), max(1, int(records))
+        self.S = self.cap_b * BC
+        dt = torch.float16 if batch.storage == "fp16" else torch.float32
+        z = lambda n, d=torch.float32: torch.zeros(n, dtype=d, device=dev)
+        self.f = z(self.cap_b * BLOCK, dt)
+        self.nbr = torch.full((self.cap_b * 27,), -1, dtype=torch.int32, device=dev)
+        self.bidx = torch.full((self.cap_b,), -1, dtype=torch.int32, device=dev)
+        self.om_brick = z(self.cap_b, torch.uint8)
+        self.om = z(self.S)
+        self.vels = [z((3, self.S)) for _ in range(2)]
+        r = z(self.S)
+        self.rhos = [r, z(self.S)] if batch.storage == "fp16" else [r, r]
+        self.lmask = z(self.cap_r * BC, torch.int32)
+        self.smask = z(self.cap_r * BC, torch.int32)
+        self.qc = z(self.cap_r * 27 * BC)
+        self.uin = z(self.cap_r * 3 * BC)
+        self.side = z(self.cap_r * 27 * BC)
+        self.n_open = z(batch.max_cases, torch.int32)       # flow-measuring cases of this arena
+        self.bricks = _Ranges(self.cap_b)
+        self.records = _Ranges(self.cap_r if records > 0 else 0)
+        self.cases = []
+        self.interior = self.boundary = None
+        self.n_interior = self.n_boundary = 0
+        self.flux = None
+
+    def nbytes(self):
+        return arena_bytes(self.cap_b, self.cap_r, self.batch.storage)
+
+    def place(self, lay):
+        """(first brick, first record) for the layout, or None."""
+        b0 = self.bricks.alloc(int(lay["n_bricks"]))
+        if b0 is None:
+            return None
+        r0 = self.records.alloc(int(lay["n_boundary"])) if lay["n_boundary"] else 0
+        if r0 is None:
+            self.bricks.release(b0, int(lay["n_bricks"]))
+            return None
+        return b0, r0
+
+    def rebuild_lists(self):
+        """Device lists from the cases now in the arena (host arrays, one upload each)."""
+        torch = _torch()
+        dev = self.batch.dev
+        shift = _native.CS_CASE_SHIFT
+        inner = [(c.lay["interior"].astype(np.uint32) + np.uint32(c.b0)) | np.uint32(c.cid << shift)
+                 for c in self.cases]
+        inner = np.concatenate(inner) if inner else np.zeros(0, np.uint32)
+        hi = self.records.high() if self.records.capacity else 0
+        bnd = np.full(hi, _native.CS_LIST_SKIP, np.uint32)
+        for c in self.cases:
+            nbd = int(c.lay["n_boundary"])
+            if nbd:
+                bnd[c.r0:c.r0 + nbd] = ((c.lay["boundary"].astype(np.uint32) + np.uint32(c.b0)) |
+                                        np.uint32(c.cid << shift))
+        up = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).to(dev)
+        self.interior = up(inner) if len(inner) else torch.zeros(1, dtype=torch.int32, device=dev)
+        self.boundary = up(bnd) if len(bnd) else torch.zeros(1, dtype=torch.int32, device=dev)
+        self.n_interior, self.n_boundary = len(inner), len(bnd)
+        # flow-measurement lists (cases with openings under measurement)
+        fl = [c for c in self.cases if c.flux is not None]
+        nopen = np.zeros(self.batch.max_cases, np.int32)
+        if fl:
+            start = np.zeros(self.batch.max_cases + 1, np.int64)
+            idx, w, slot = [], [], []
+            pos = 0
+            for cid in range(self.batch.max_cases):
+                start[cid] = pos
+                c = self.batch.case_by_id.get(cid)
+                if c is not None and c.arena is self and c.flux is not None:
+                    idx.append(c.flux_slots)
+                    w.append(c.flux["w"])
+                    slot.append(c.flux["slot"])
+                    pos += len(c.flux_slots)
+                    nopen[cid] = c.n_open
+            start[self.batch.max_cases] = pos
+            cat = lambda xs, d: torch.from_numpy(np.ascontiguousarray(np.concatenate(xs), d)).to(dev)
+            self.flux = {"idx": cat(idx, np.int32), "w": cat(w, np.float32), "slot": cat(slot, np.int32),
+                         "start": torch.from_numpy(start).to(dev)}
+        else:
+            self.flux = None
+        self.n_open.copy_(torch.from_numpy(nopen))
+
+
+class Batch:
+    """A set of cases stepped together (see the module docstring).
+
+    storage "fp32" / "fp16"; model, magic, bouzidi, galilean as in ``lbm.Solver`` (shared by all cases). Arenas
+    are created on demand with ``arena_bricks`` / ``arena_records`` capacity while the total stays within
+    ``budget_bytes`` (default: the device's free memory less 2 GiB)."""
+
+    def __init__(self, storage="fp32", model="regularised", magic=3.0 / 16.0, bouzidi=True, galilean=None,
+                 device=None, arena_bricks=65536, arena_records=16384, budget_bytes=None, max_cases=256,
+                 max_open=8, max_harmonics=16, variant=0):
+        torch = _torch()
+        if storage not in STORAGES:
+            raise ValueError(f"unknown storage {storage!r}")
+        if not 1 <= max_cases <= _native.CS_MAX_CASES:
+            raise ValueError(f"max_cases {max_cases}: 1 .. {_native.CS_MAX_CASES}")
+        self.dev = _require_gpu(device)
+        self.storage = storage
+        self.model = MODELS[model]
+        self.magic = float(magic)
+        self.bouzidi = int(bool(bouzidi))
+        self.gal = int(True if galilean is None else bool(galilean))
+        self.arena_bricks, self.arena_records = int(arena_bricks), int(arena_records)
+        if budget_bytes is None:
+            free, _ = torch.cuda.mem_get_info(self.dev)
+            budget_bytes = max(0, free - 2 * 2 ** 30)
+        self.budget = int(budget_bytes)
+        self.max_cases, self.max_open, self.max_harm = int(max_cases), int(max_open), int(max_harmonics)
+        self.variant = int(variant)
+        self._tstream = torch.cuda.Stream(device=self.dev)          # default priority
+        self.stream_handle = ctypes.c_void_p(self._tstream.cuda_stream)
+        M, K = self.max_cases, self.max_open
+        with torch.cuda.stream(self._tstream):
+            z = lambda shape, d=torch.float32: torch.zeros(shape, dtype=d, device=self.dev)
+            self.tick = z(M, torch.int32)
+            self.coef = z((M, 2 * self.max_harm + 1))
+            self.nh = z(M, torch.int32)
+            self.period = z(M, torch.int32)
+            self.ramp = z(M, torch.int32)
+            self.drive = z(M)
+            self.omega = z(M)
+            self.force = z((M, 3))
+            self.p_out = z((M, K))
+            self.q_last = z((M, K))
+            self.R = z((M, K))
+            self.pref = z(M)
+            self.alpha = z(M)
+            self._work = z(_native.CS_REDUCE_WORK, torch.float64)
+            self._red = z(2, torch.float64)
+        self._tstream.synchronize()
+        self.arenas = []
+        self.case_by_id = {}
+        self.cur = 0
+        self.steps = 0
+        self.graphs = {}
+
+    # --- capacity -------------------------------------------------------------------------------------------
+    def nbytes(self):
+        return sum(a.nbytes() for a in self.arenas)
+
+    def _new_arena(self, lay):
+        nb = max(self.arena_bricks, int(lay["n_bricks"]))
+        nr = max(self.arena_records, int(lay["n_boundary"]))
+        nb = min(nb, MAX_BRICKS)
+        if lay["n_bricks"] > nb:
+            raise ValueError(f"case of {lay['n_bricks']} bricks exceeds one arena ({MAX_BRICKS} bricks)")
+        if self.nbytes() + arena_bytes(nb, nr, self.storage) > self.budget:
+            return None
+        with _torch().cuda.stream(self._tstream):
+            a = Arena(self, nb, nr)
+        self.arenas.append(a)
+        return a
+
+    def can_admit(self, lay):
+        """Whether a case of this layout fits now (free case slot and room in an arena or the budget)."""
+        if len(self.case_by_id) >= self.max_cases:
+            return False
+        for a in self.arenas:
+            b0 = a.bricks.alloc(int(lay["n_bricks"]))
+            if b0 is None:
+                continue
+            ok = True
+            r0 = None
+            if lay["n_boundary"]:
+                r0 = a.records.alloc(int(lay["n_boundary"]))
+                ok = r0 is not None
+                if ok:
+                    a.records.release(r0, int(lay["n_boundary"]))
+            a.bricks.release(b0, int(lay["n_bricks"]))
+            if ok:
+                return True
+        nb = max(self.arena_bricks, int(lay["n_bricks"]))
+        nr = max(self.arena_records, int(lay["n_boundary"]))
+        return self.nbytes() + arena_bytes(min(nb, MAX_BRICKS), nr, self.storage) <= self.budget
+
+    # --- cases ----------------------------------------------------------------------------------------------
+    def add(self, lat, nu_lu, layout=None):
+        """Admit a lattice as a new case at rest (f~ = 0, u = 0); returns its ``Case`` handle. Raises BatchFull."""
+        from . import bricks
+        lay = layout if layout is not None else bricks.build(lat)
+        if len(self.case_by_id) >= self.max_cases:
+            raise BatchFull("no free case slot")
+        cid = min(set(range(self.max_cases)) - set(self.case_by_id))
+        spot = None
+        for a in self.arenas:
+            p = a.place(lay)
+            if p is not None:
+                spot = (a, p)
+                break
+        if spot is None:
+            a = self._new_arena(lay)
+            if a is None:
+                raise BatchFull("HBM budget exhausted")
+            p = a.place(lay)
+            if p is None:
+                raise BatchFull("case does not fit a fresh arena")
+            spot = (a, p)
+        self.synchronize()
+        c = Case(self, cid, lat, nu_lu, lay, spot[0], *spot[1])
+        self.case_by_id[cid] = c
+        spot[0].cases.append(c)
+        self._topology_changed(spot[0])
+        return c
+
+    def remove(self, case):
+        """Release a case's bricks, records and slot."""
+        torch = _torch()
+        self.synchronize()
+        a = case.arena
+        a.cases.remove(case)
+        a.bricks.release(case.b0, int(case.lay["n_bricks"]))
+        if case.lay["n_boundary"]:
+            a.records.release(case.r0, int(case.lay["n_boundary"]))
+        with torch.cuda.stream(self._tstream):
+            self.period[case.cid] = 0
+        del self.case_by_id[case.cid]
+        case.arena = None
+        self._topology_changed(a)
+        if not a.cases:
+            self.synchronize()
+            self.arenas.remove(a)
+        self.synchronize()
+
+    def _topology_changed(self, arena):
+        self.synchronize()
+        with _torch().cuda.stream(self._tstream):
+            arena.rebuild_lists()
+        self.synchronize()
+        self.graphs = {}
+
+    # --- stepping -------------------------------------------------------------------------------------------
+    def _clock(self):
+        a = _native.ClockCasesArgs()
+        a.tick, a.coef, a.nh = self.tick.data_ptr(), self.coef.data_ptr(), self.nh.data_ptr()
+        a.period, a.ramp, a.drive = self.period.data_ptr(), self.ramp.data_ptr(), self.drive.data_ptr()
+        a.n_cases, a.coef_stride = self.max_cases, self.coef.shape[1]
+        _native.call("cs_clock_cases", ctypes.byref(a), self.stream_handle)
+
+    def step_args(self, arena, a, boundary):
+        A = _native.BricksStepArgs()
+        lst = arena.boundary if boundary else arena.interior
+        A.f, A.nbr, A.list = arena.f.data_ptr(), arena.nbr.data_ptr(), lst.data_ptr()
+        A.n_list = arena.n_boundary if boundary else arena.n_interior
+        A.bidx, A.lmask, A.smask = arena.bidx.data_ptr(), arena.lmask.data_ptr(), arena.smask.data_ptr()
+        A.qc, A.uin, A.side = arena.qc.data_ptr(), arena.uin.data_ptr(), arena.side.data_ptr()
+        A.om_cell, A.om_brick, A.omega = arena.om.data_ptr(), arena.om_brick.data_ptr(), 1.0
+        A.p_out, A.drive = self.p_out.data_ptr(), self.drive.data_ptr()
+        A.vel_old, A.vel = arena.vels[a].data_ptr(), arena.vels[1 - a].data_ptr()
+        A.rho, A.rho_old = arena.rhos[1 - a].data_ptr(), arena.rhos[a].data_ptr()
+        A.n_slots, A.storage = arena.S, STORAGES[self.storage]
+        A.magic, A.bouzidi, A.model, A.galilean = self.magic, self.bouzidi, self.model, self.gal
+        A.parity, A.boundary = a, int(boundary)
+        A.case_omega, A.case_force = self.omega.data_ptr(), self.force.data_ptr()
+        A.p_out_stride, A.variant = self.max_open, self.variant
+        return A
+
+    def _flux(self, arena, a):
+        fx = arena.flux
+        A = _native.FluxCasesArgs()
+        A.idx, A.w, A.slot, A.start = fx["idx"].data_ptr(), fx["w"].data_ptr(), fx["slot"].data_ptr(), \
+            fx["start"].data_ptr()
+        A.vel, A.vel_elem, A.vel_comp = arena.vels[1 - a].data_ptr(), 1, arena.S
+        A.R, A.drive, A.p_out, A.q_last = self.R.data_ptr(), self.drive.data_ptr(), self.p_out.data_ptr(), \
+            self.q_last.data_ptr()
+        A.n_open, A.pref, A.alpha = arena.n_open.data_ptr(), self.pref.data_ptr(), self.alpha.data_ptr()
+        A.n_cases, A.open_stride = self.max_cases, self.max_open
+        _native.call("cs_flux_cases", ctypes.byref(A), self.stream_handle)
+
+    def launch_parts(self):
+        """(arena index, boundary flag, cells) of the step launches of one step, in order."""
+        out = []
+        for k, ar in enumerate(self.arenas):
+            ni = sum(int(c.lay["n_interior"]) * BC for c in ar.cases)
+            nf = sum(c.N for c in ar.cases)
+            out += [(k, False, ni), (k, True, nf - ni)]
+        return out
+
+    def step_part(self, k, boundary, a):
+        ar = self.arenas[k]
+        A = self.step_args(ar, a, boundary)
+        if A.n_list:
+            _native.call("cs_bricks_step", ctypes.byref(A), self.stream_handle)
+
+    def _launch(self, a):
+        self._clock()
+        for ar in self.arenas:
+            for boundary in (False, True):
+                A = self.step_args(ar, a, boundary)
+                if A.n_list:
+                    _native.call("cs_bricks_step", ctypes.byref(A), self.stream_handle)
+            if ar.flux is not None:
+                self._flux(ar, a)
+
+    def run(self, steps, chunk=200):
+        """Advance every case `steps` steps: graphs of `chunk` steps (made even), remainder step by step."""
+        if not self.case_by_id:
+            return
+        chunk += chunk % 2
+        full, rest = divmod(int(steps), chunk)
+        if full:
+            key = (self.cur, chunk, self.variant)
+            if key not in self.graphs:
+                with _native.capture(self.stream_handle) as cap:
+                    a = self.cur
+                    for _ in range(chunk):
+                        self._launch(a)
+                        a = 1 - a
+                self.graphs[key] = cap.graph
+            g = self.graphs[key]
+            for _ in range(full):
+                g.launch(self.stream_handle)
+        for _ in range(rest):
+            self._launch(self.cur)
+            self.cur = 1 - self.cur
+        self.steps += int(steps)
+        for c in self.case_by_id.values():
+            c.steps += int(steps)
+
+    def synchronize(self):
+        self._tstream.synchronize()
+
+    # --- autotuning -----------------------------------------------------------------------------------------
+    def variants(self):
+        out = []
+        v = 0
+        while True:
+            w, x = ctypes.c_int32(), ctypes.c_int32()
+            if _native.load().cs_bricks_variant_info(v, ctypes.byref(w), ctypes.byref(x)) != 0:
+                break
+            out.append({"variant": v, "waves_per_group": w.value, "xcd_order": x.value})
+            v += 1
+        return out
+
+    def autotune(self, steps=20, repeats=2):
+        """Time every launch variant on this batch (HIP events on the batch stream, `steps` eager steps each, best of
+        `repeats`), keep the fastest, and restore the state the timing runs changed. Returns {variant: ms/step}."""
+        torch = _torch()
+        snap = self._snapshot()
+        times = {}
+        try:
+            for var in self.variants():
+                self.variant = var["variant"]
+                best = None
+                for _ in range(repeats):
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record(self._tstream)
+                    a = self.cur
+                    for _ in range(steps):
+                        self._launch(a)
+                        a = 1 - a
+                    e1.record(self._tstream)
+                    self.synchronize()
+                    ms = e0.elapsed_time(e1) / steps
+                    best = ms if best is None else min(best, ms)
+                times[var["variant"]] = best
+        finally:
+            self._restore(snap)
+        self.variant = min(times, key=times.get)
+        self.graphs = {}
+        return times
+
+    def _snapshot(self):
+        self.synchronize()
+        with _torch().cuda.stream(self._tstream):
+            s = {"tables": [t.clone() for t in (self.tick, self.drive, self.p_out, self.q_last)],
+                 "arenas": [[ar.f.clone(), ar.side.clone()] + [v.clone() for v in ar.vels] +
+                            [r.clone() for r in ar.rhos] for ar in self.arenas]}
+        self.synchronize()
+        return s
+
+    def _restore(self, s):
+        self.synchronize()
+        with _torch().cuda.stream(self._tstream):
+            for dst, src in zip((self.tick, self.drive, self.p_out, self.q_last), s["tables"]):
+                dst.copy_(src)
+            for ar, saved in zip(self.arenas, s["arenas"]):
+                ar.f.copy_(saved[0])
+                ar.side.copy_(saved[1])
+                ar.vels[0].copy_(saved[2])
+                ar.vels[1].copy_(saved[3])
+                ar.rhos[0].copy_(saved[4])
+                ar.rhos[1].copy_(saved[5])
+        self.synchronize()
+
+
+class Case:
+    """One lattice inside a Batch, with the lbm.Solver surface (lbm.py:346-455) and the resistance-outlet flow
+    measurement of simulate.py (ArterySolver)."""
+
+    def __init__(self, batch, cid, lat, nu_lu, lay, arena, b0, r0):
+        torch = _torch()
+        self.batch, self.cid, self.lat, self.lay, self.arena, self.b0, self.r0 = batch, cid, lat, lay, arena, b0, r0
+        self.N = int(lat["nbr"].shape[1])
+        self.tau = 3.0 * nu_lu + 0.5
+        self.steps = 0
+        self.flux = None
+        self.n_open = 0
+        nb, nr = int(lay["n_bricks"]), int(lay["n_boundary"])
+        dev = batch.dev
+        self.slots = (lay["slot_of_cell"].astype(np.int64) + b0 * BC)
+        with torch.cuda.stream(batch._tstream):
+            up = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+            self.slot_dev = up(self.slots)
+            self.slot32 = up(self.slots.astype(np.int32))
+            nbr = lay["nbr"].astype(np.int64)
+            nbr = np.where(nbr >= 0, nbr + b0, -1).astype(np.int32)
+            arena.nbr[b0 * 27:(b0 + nb) * 27].copy_(up(nbr.reshape(-1)))
+            bidx = lay["bidx"].astype(np.int64)
+            arena.bidx[b0:b0 + nb].copy_(up(np.where(bidx >= 0, bidx + r0, -1).astype(np.int32)))
+            arena.om_brick[b0:b0 + nb].zero_()
+            arena.om[b0 * BC:(b0 + nb) * BC].zero_()
+            arena.f[b0 * BLOCK:(b0 + nb) * BLOCK].zero_()
+            for v in arena.vels:
+                v[:, b0 * BC:(b0 + nb) * BC].zero_()
+            for r in arena.rhos:
+                r[b0 * BC:(b0 + nb) * BC].zero_()
+            if nr:
+                arena.lmask[r0 * BC:(r0 + nr) * BC].copy_(up(lay["lmask"].view(np.int32).reshape(-1)))
+                arena.smask[r0 * BC:(r0 + nr) * BC].copy_(up(lay["smask"].view(np.int32).reshape(-1)))
+                arena.qc[r0 * 27 * BC:(r0 + nr) * 27 * BC].copy_(up(lay["qc"].reshape(-1)))
+                arena.uin[r0 * 3 * BC:(r0 + nr) * 3 * BC].copy_(up(lay["uin"].reshape(-1)))
+                arena.side[r0 * 27 * BC:(r0 + nr) * 27 * BC].zero_()
+            c = cid
+            batch.tick[c] = 0
+            batch.drive[c] = 0.0
+            batch.omega[c] = float(np.float32(1.0 / self.tau))
+            batch.force[c].zero_()
+            batch.p_out[c].zero_()
+            batch.q_last[c].zero_()
+        batch.synchronize()
+        self.set_drive([1.0])
+
+    # --- reference surface ----------------------------------------------------------------------------------
+    @property
+    def cur(self):
+        return self.batch.cur
+
+    def _set(self, tensor, value):
+        torch = _torch()
+        self.batch.synchronize()
+        with torch.cuda.stream(self.batch._tstream):
+            tensor.copy_(torch.as_tensor(np.asarray(value), dtype=tensor.dtype).reshape(tensor.shape).to(self.batch.dev))
+        self.batch.synchronize()
+
+    def set_drive(self, coef, period=1, ramp=0):
+        """As lbm.Solver.set_drive (lbm.py:384-392): the case's clock restarts from its current tick."""
+        coef = np.asarray(coef, np.float32)
+        if len(coef) % 2 == 0:
+            coef = np.append(coef, 0.0).astype(np.float32)
+        nh = (len(coef) - 1) // 2
+        if nh > self.batch.max_harm:
+            raise ValueError(f"{nh} harmonics > max_harmonics {self.batch.max_harm}")
+        row = np.zeros(self.batch.coef.shape[1], np.float32)
+        row[:len(coef)] = coef
+        b, c = self.batch, self.cid
+        self._set(b.coef[c], row)
+        self._set(b.nh[c:c + 1], [nh])
+        self._set(b.ramp[c:c + 1], [int(ramp)])
+        self._set(b.period[c:c + 1], [int(period)])
+
+    def set_force(self, F):
+        self._set(self.batch.force[self.cid], [float(np.float32(x)) for x in F])
+
+    def set_sponge(self, length, tau_max, kinds=("inlet", "outlet")):
+        torch = _torch()
+        tau = sponge_tau(self.lat, self.tau, length, tau_max, kinds)
+        self.tau_cell = tau
+        om = (1.0 / tau).astype(np.float32)
+        omega = np.float32(1.0 / self.tau)
+        nb = int(self.lay["n_bricks"])
+        slots = np.full(nb * BC, omega, np.float32)
+        slots[self.lay["slot_of_cell"]] = om
+        own = (slots.reshape(nb, BC) != omega).any(1).astype(np.uint8)
+        ar, b0 = self.arena, self.b0
+        self.batch.synchronize()
+        with torch.cuda.stream(self.batch._tstream):
+            ar.om[b0 * BC:(b0 + nb) * BC].copy_(torch.from_numpy(slots))
+            ar.om_brick[b0:b0 + nb].copy_(torch.from_numpy(own))
+        self.batch.synchronize()
+
+    def set_outlet_pressure(self, p_lu):
+        p = np.zeros(self.batch.max_open, np.float32)
+        v = np.asarray(p_lu, np.float32).reshape(-1)
+        if len(v) > self.batch.max_open:
+            raise ValueError(f"{len(v)} openings > max_open {self.batch.max_open}")
+        p[:len(v)] = v
+        self._set(self.batch.p_out[self.cid], p)
+
+    def set_outlets(self, flux, R_lu, pref_lu, alpha):
+        """Resistance outlets with per-step flow measurement (simulate.py:93-114, as artery.ArterySolver)."""
+        n = len(R_lu)
+        if not 1 <= n <= self.batch.max_open:
+            raise ValueError(f"{n} openings: 1 .. {self.batch.max_open}")
+        R = np.zeros(self.batch.max_open, np.float32)
+        R[:n] = R_lu
+        self._set(self.batch.R[self.cid], R)
+        self._set(self.batch.pref[self.cid:self.cid + 1], [pref_lu])
+        self._set(self.batch.alpha[self.cid:self.cid + 1], [alpha])
+        self.flux = {"w": np.ascontiguousarray(flux["w"], np.float32),
+                     "slot": np.ascontiguousarray(flux["slot"], np.int32)}
+        self.flux_slots = self.slots[np.asarray(flux["idx"])].astype(np.int32)
+        self.n_open = n
+        self.batch._topology_changed(self.arena)
+
+    def flows(self):
+        self.batch.synchronize()
+        return self.batch.q_last[self.cid, :max(1, self.n_open)].cpu().numpy()
+
+    def opening_pressures(self):
+        self.batch.synchronize()
+        n = max(1, self.n_open, self.lat["n_outlets"])
+        return self.batch.p_out[self.cid, :n].cpu().numpy()
+
+    def fields(self):
+        """(pressure, velocity) per cell in the lattice's order."""
+        self.batch.synchronize()
+        with _torch().cuda.stream(self.batch._tstream):
+            ar, cur = self.arena, self.batch.cur
+            rho = ar.rhos[cur].index_select(0, self.slot_dev)
+            u = ar.vels[cur].index_select(1, self.slot_dev).t().contiguous()
+            out = rho.cpu().numpy() / 3.0, u.cpu().numpy()
+        return out
+
+    def _convert(self, name, f_ref):
+        ar, cur = self.arena, self.batch.cur
+        A = _native.BricksConvertArgs()
+        A.f, A.nbr, A.bidx = ar.f.data_ptr(), ar.nbr.data_ptr(), ar.bidx.data_ptr()
+        A.lmask, A.smask, A.side = ar.lmask.data_ptr(), ar.smask.data_ptr(), ar.side.data_ptr()
+        A.slot_of_cell, A.n_cells = self.slot32.data_ptr(), self.N
+        A.f_ref, A.parity = f_ref.data_ptr(), cur
+        A.storage, A.vel, A.rho, A.n_slots = STORAGES[self.batch.storage], ar.vels[cur].data_ptr(), \
+            ar.rhos[cur].data_ptr(), ar.S
+        _native.call(name, ctypes.byref(A), self.batch.stream_handle)
+
+    def get_f(self):
+        torch = _torch()
+        self.batch.synchronize()
+        with torch.cuda.stream(self.batch._tstream):
+            f_ref = torch.empty((27, self.N), dtype=torch.float32, device=self.batch.dev)
+            self._convert("cs_bricks_get_f", f_ref)
+        self.batch.synchronize()
+        return f_ref.cpu().numpy()
+
+    def set_f(self, f):
+        torch = _torch()
+        self.batch.synchronize()
+        with torch.cuda.stream(self.batch._tstream):
+            f_ref = torch.from_numpy(np.ascontiguousarray(f, np.float32)).to(self.batch.dev)
+            self._convert("cs_bricks_set_f", f_ref)
+        self.batch.synchronize()
+
+    def set_velocity(self, u):
+        torch = _torch()
+        f = self.get_f() if self.batch.storage == "fp16" else None
+        self.batch.synchronize()
+        with torch.cuda.stream(self.batch._tstream):
+            uu = torch.from_numpy(np.ascontiguousarray(u, np.float32)).to(self.batch.dev)
+            self.arena.vels[self.batch.cur][:, self.slot_dev] = uu.t()
+        self.batch.synchronize()
+        if f is not None:
+            self.set_f(f)
+
+    def ok(self):
+        """All densities of the case finite (device reduction)."""
+        ar, b0, nb = self.arena, self.b0, int(self.lay["n_bricks"])
+        x = ar.rhos[self.batch.cur][b0 * BC:(b0 + nb) * BC]
+        _native.call("cs_reduce_finite", ctypes.c_void_p(x.data_ptr()), x.numel(),
+                     ctypes.c_void_p(self.batch._work.data_ptr()), ctypes.c_void_p(self.batch._red.data_ptr()),
+                     self.batch.stream_handle)
+        self.batch.synchronize()
+        return bool(self.batch._red[1].item() == 0)

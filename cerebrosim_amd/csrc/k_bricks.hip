@@ -1,521 +1,17 @@
-// Solver step over the brick layout with AA-pattern in-place streaming (semantics of lbm.py:140-257).
-//
-// Storage: the 27 x 64 populations of 4^3 brick b occupy the block f[b * 1728 .. b * 1728 + 1728); population k of
-// cell l (l = x + 4 y + 16 z) sits at offset slot[k][l] of the block. One wavefront per brick, lane l = cell l; four
-// bricks per workgroup.
-//
-// Slot order. An even step touches exactly its brick's own block. An odd step reads and then writes slot (m, k) from
-// the cell m - c_k, which lies in m's brick or in one of its 26 neighbours. slot[][] groups a block by that accessing
-// brick: first the 1000 slots the brick's own wavefront reaches, then one contiguous record per neighbour offset (100
-// slots for a face neighbour, 10 for an edge, 1 for a corner), population-major inside each group. In an odd step a
-// wavefront therefore writes its own group and 26 records, each touched by no other wavefront, so cache lines fill
-// from one wavefront instead of collecting partial writes from two.
-//
-// AA pattern. Let P_i(x) be the post-collision population of cell x in direction i.
-//  * even step (parity 0) reads S[n][i] and writes P_i(n) to S[n][opp i]: everything stays in the cell's own slots;
-//  * odd step (parity 1) reads S[n - c_i][opp i] (the even step's output of the upstream cell) and writes P_i(n) to
-//    S[n + c_i][i], which is where the next even step of n + c_i reads it. Every slot is read and then written by one
-//    thread only, so one copy of the populations suffices.
-//  * A missing link i of n (wall, inlet, outlet: n - c_i not fluid) reads S[n][i]; the step that wrote it stored
-//    there P_opp(n), the population that left n towards the boundary (odd steps redirect their write to a
-//    non-fluid n + c_i into S[n][opp i]). That is the reference's f_o = f_in[opp i, n] in both parities.
-//  * Bouzidi with q < 1/2 needs f_in[opp i, n + c_i]: the population n pulls for direction opp i, already in hand.
-//    Bouzidi with q >= 1/2 needs P_i(n) of the previous step: when n + c_i is not fluid it sits in S[n][opp i]
-//    (the missing link opp i's own raw value), otherwise it lives in a slot another thread rewrites during the
-//    step, so each such link keeps a copy in the side buffer, written by n at the end of the previous step.
-// Interior bricks (all cells fluid, all links fluid) run a separate instantiation without any boundary data.
-#include <type_traits>
-
-#include "collide.hpp"
-#include "common.hpp"
+// C entry points of the brick layout: the step (dispatch over the launch variants), the conversions between the
+// AA storage and the reference (Q, N) layout, and the slot tables.
+#include "k_bricks.hpp"
 
 namespace cs {
 namespace {
 
-constexpr int BC = 64;
-constexpr int WAVES = 4;
-constexpr int WG = 64 * WAVES;
-constexpr uint32_t kInletBit = 1u << 27;
-constexpr uint32_t kAllLinks = (1u << Q) - 1;
-
-struct SlotTable {
-    uint16_t p[Q][BC];
+// Launch variants the autotuner chooses from: bricks (wavefronts) per workgroup and the XCD-aware workgroup order.
+struct Variant {
+    int waves;
+    int xcd;
 };
-
-constexpr SlotTable make_slots() {
-    SlotTable t{};
-    int pos = 0;
-    // group g = brick offset (ox+1) + 3 (oy+1) + 9 (oz+1) of the accessing cell l - c_k; own brick (13) first
-    for (int gi = 0; gi < 27; ++gi) {
-        const int g = gi == 0 ? 13 : (gi <= 13 ? gi - 1 : gi);
-        for (int k = 0; k < Q; ++k)
-            for (int l = 0; l < BC; ++l) {
-                const int x = (l & 3) - Lat::cx[k], y = ((l >> 2) & 3) - Lat::cy[k], z = (l >> 4) - Lat::cz[k];
-                const int o = (x < 0 ? 0 : (x > 3 ? 2 : 1)) + 3 * (y < 0 ? 0 : (y > 3 ? 2 : 1)) +
-                              9 * (z < 0 ? 0 : (z > 3 ? 2 : 1));
-                if (o == g) t.p[k][l] = (uint16_t)pos++;
-            }
-    }
-    return t;
-}
-
-// 32-bit element offsets: the host guarantees n_bricks * 27 * 64 < 2^32 (one base pointer + a 32-bit VGPR offset
-// per access keeps register use low).
-__device__ __forceinline__ uint32_t brick_base(int b) { return (uint32_t)b * (uint32_t)(Q * BC); }
-
-__constant__ SlotTable kSlots = make_slots();
-
-// Brick offset index (ox+1) + 3 (oy+1) + 9 (oz+1) of the position (x, y, z) (components in [-1, 4]).
-constexpr int offset_index(int x, int y, int z) {
-    return (x < 0 ? 0 : (x > 3 ? 2 : 1)) + 3 * (y < 0 ? 0 : (y > 3 ? 2 : 1)) + 9 * (z < 0 ? 0 : (z > 3 ? 2 : 1));
-}
-
-constexpr int group_size(int g) {
-    int n = 0;
-    for (int a = 0, v = g; a < 3; ++a, v /= 3) n += (v % 3 != 1);
-    return n == 0 ? 1000 : (n == 1 ? 100 : (n == 2 ? 10 : 1));
-}
-
-// Start of group g inside a block (the order of make_slots: own group 13, then 0..12, 14..26).
-constexpr int group_start(int g) {
-    int pos = 0;
-    for (int gi = 0; gi < 27; ++gi) {
-        const int h = gi == 0 ? 13 : (gi <= 13 ? gi - 1 : gi);
-        if (h == g) return pos;
-        pos += group_size(h);
-    }
-    return -1;
-}
-
-// The odd-step footprint of a brick, staged in LDS: its own group, then for every neighbour offset o != 13 the
-// record of brick (b + o) that holds the slots accessed from b, i.e. group 26 - o of that brick. Staged order = the
-// block order of the groups (own first, then o = 0..12, 14..26), so stage_start(o) = group_start(26 - o) for o != 13
-// would not hold; it is computed here.
-constexpr int stage_start(int o) {
-    int pos = 0;
-    for (int oi = 0; oi < 27; ++oi) {
-        const int h = oi == 0 ? 13 : (oi <= 13 ? oi - 1 : oi);
-        if (h == o) return pos;
-        pos += group_size(h == 13 ? 13 : 26 - h);
-    }
-    return -1;
-}
-
-// oddpos[i][l]: position in the staged odd-step footprint of the slot lane l reads for direction i,
-// i.e. population opp(i) of the cell l - c_i (in this brick or a neighbour).
-constexpr SlotTable make_oddpos() {
-    SlotTable t{};
-    constexpr SlotTable sl = make_slots();
-    for (int i = 0; i < Q; ++i)
-        for (int l = 0; l < BC; ++l) {
-            const int x = (l & 3) - Lat::cx[i], y = ((l >> 2) & 3) - Lat::cy[i], z = (l >> 4) - Lat::cz[i];
-            const int o = offset_index(x, y, z);
-            const int ml = (x & 3) + 4 * (y & 3) + 16 * (z & 3);
-            const int slot = sl.p[Lat::opp[i]][ml];
-            const int g = o == 13 ? 13 : 26 - o;
-            t.p[i][l] = (uint16_t)(stage_start(o) + slot - group_start(g));
-        }
-    return t;
-}
-
-__constant__ SlotTable kOddPos = make_oddpos();
-
-// Cross-lane LDS hand-over inside one wavefront.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Copy a 27 x 64 uint16 table into LDS (216 x 16 bytes).
-__device__ __forceinline__ void load_table(uint32_t* lds, const SlotTable& src) {
-    if (threadIdx.x < Q * BC * 2 / 16) {
-        const uint4 v = reinterpret_cast<const uint4*>(&src)[threadIdx.x];
-        reinterpret_cast<uint4*>(lds)[threadIdx.x] = v;
-    }
-}
-
-// Move the footprint of brick b between global memory and the wave's LDS stage with full-row accesses.
-// Even steps: the whole own block. Odd steps: own group + the 26 records in the neighbour blocks.
-template <int PAR, bool STORE, typename T>
-__device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int* tbl, uint32_t base, int L) {
-    if constexpr (PAR == 0) {
-        // the whole block as 16-byte vectors (a block is Q * BC * sizeof(T) bytes, a multiple of 16)
-        constexpr int NV = Q * BC * (int)sizeof(T) / 16;
-        uint4* g = reinterpret_cast<uint4*>(F + base);
-        uint4* l = reinterpret_cast<uint4*>(stage);
-        static_for<0, (NV + BC - 1) / BC>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            const int k = j * BC + L;
-            if (NV % BC == 0 || (j + 1) * BC <= NV || k < NV) {
-                if constexpr (STORE) g[k] = l[k];
-                else l[k] = g[k];
-            }
-        });
-    } else {
-        static_for<0, 27>([&](auto oc) {
-            constexpr int o = decltype(oc)::value;
-            constexpr int g = o == 13 ? 13 : 26 - o;
-            constexpr int size = group_size(g), start = group_start(g), st = stage_start(o);
-            const uint32_t nb = o == 13 ? base : brick_base(__builtin_amdgcn_readfirstlane(tbl[o]));
-            static_for<0, (size + BC - 1) / BC>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                const int idx = j * BC + L;
-                if (size % BC == 0 || j * BC + BC <= size || idx < size) {
-                    if constexpr (STORE) F[nb + start + idx] = stage[st + idx];
-                    else stage[st + idx] = F[nb + start + idx];
-                }
-            });
-        });
-    }
-}
-
-static_assert(Q * BC * 2 % 16 == 0 && Q * BC * 2 / 16 <= WG, "table copy layout");
-static_assert((Q * BC * 2) % 16 == 0, "a block of float16 populations is a whole number of 16-byte vectors");
-
-// Brick-table index d and local index of the position (x, y, z) + (SX, SY, SZ), SX.. in {-1, 0, 1}.
-template <int SX, int SY, int SZ>
-__device__ __forceinline__ void shift_pos(int x, int y, int z, int& d, int& l) {
-    int o = 13;
-    if constexpr (SX < 0) o -= (x == 0) ? 1 : 0;
-    if constexpr (SX > 0) o += (x == 3) ? 1 : 0;
-    if constexpr (SY < 0) o -= (y == 0) ? 3 : 0;
-    if constexpr (SY > 0) o += (y == 3) ? 3 : 0;
-    if constexpr (SZ < 0) o -= (z == 0) ? 9 : 0;
-    if constexpr (SZ > 0) o += (z == 3) ? 9 : 0;
-    d = o;
-    l = ((x + SX) & 3) + 4 * ((y + SY) & 3) + 16 * ((z + SZ) & 3);
-}
-
-// ---- FP16 storage -----------------------------------------------------------------------------------------------
-// A stored population is h = (f~ - feq2(rho', u of the cell that wrote it)) / w * kH16Scale in float16, where
-// feq2 = w (rho' + 3 c.u + 4.5 (c.u)^2) and rho' = drho - 1.5 |u|^2: the non-equilibrium part, which is small, keeps
-// the float16's 11 significant bits. Every slot is written by the cell x that owns the population P_i(x); the reader
-// decodes it with x's moments of the previous step (the velocity / density planes that step wrote). The writer of
-// the value a cell n reads for direction i is n - c_i when link i is fluid and n itself otherwise (both parities).
-constexpr float kH16Scale = 1024.0f;
-constexpr int kHalo = 6 * 6 * 6;  // the brick and a one-cell shell
-
-// Halo index of the position (x, y, z) + (SX, SY, SZ) in the 6^3 shell around the brick.
-template <int SX, int SY, int SZ>
-__device__ __forceinline__ int halo_index(int x, int y, int z) {
-    return (x + 1 + SX) + 6 * (y + 1 + SY) + 36 * (z + 1 + SZ);
-}
-
-template <int I>
-__device__ __forceinline__ _Float16 h16_encode(float fv, float rho_p, float u0, float u1, float u2) {
-    const float cu = cdot<I>(u0, u1, u2);
-    const float v = (fv * (1.0f / Lat::w[I]) - rho_p - cu * (3.0f + 4.5f * cu)) * kH16Scale;
-    return (_Float16)v;
-}
-
-template <int I>
-__device__ __forceinline__ float h16_decode(_Float16 h, const float4& m) {
-    const float cu = cdot<I>(m.y, m.z, m.w);
-    return Lat::w[I] * ((float)h * (1.0f / kH16Scale) + m.x + cu * (3.0f + 4.5f * cu));
-}
-
-// Stage the previous-step moments {drho - 1.5 |u|^2, ux, uy, uz} of the brick and its one-cell shell in LDS.
-__device__ __forceinline__ void load_halo(float4* halo, const int* tbl, const float* rho_old, const float* vel_old,
-                                          uint32_t S, int L) {
-    for (int e = L; e < kHalo; e += BC) {
-        const int hx = e % 6, hy = (e / 6) % 6, hz = e / 36;
-        const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
-                      9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
-        const int l = ((hx - 1) & 3) + 4 * ((hy - 1) & 3) + 16 * ((hz - 1) & 3);
-        const int nb = tbl[o];
-        float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (nb >= 0) {
-            const uint32_t sl = (uint32_t)nb * BC + l;
-            const float u0 = vel_old[sl], u1 = vel_old[S + sl], u2 = vel_old[2 * S + sl];
-            m = make_float4(rho_old[sl] - 1.5f * (u0 * u0 + u1 * u1 + u2 * u2), u0, u1, u2);
-        }
-        halo[e] = m;
-    }
-}
-
-
-// Value streamed into cell n through missing link I (wall / inlet / outlet), lbm.py:163-181.
-//   fo        raw value of slot I: P_opp(n)
-//   opp_fluid link opp(I) is fluid; f_opp its pulled population f_in[opp I, n + c_I]
-//   p_self    P_I(n) of the previous step (for q >= 1/2)
-template <int I>
-__device__ __forceinline__ float link_value(float code, float fo, bool opp_fluid, float f_opp, float p_self, int bouzidi,
-                                            const float (&uind)[3], const float (&up)[3], float upsq,
-                                            const float* p_out) {
-    if (code >= 0.0f) {
-        float v = fo;
-        if (bouzidi) {
-            const float q = code;
-            if (q < 0.5f) {
-                if (opp_fluid) v = 2.0f * q * fo + (1.0f - 2.0f * q) * f_opp;
-            } else {
-                v = fo / (2.0f * q) + (2.0f * q - 1.0f) / (2.0f * q) * p_self;
-            }
-        }
-        return v;
-    }
-    if (code > -1.5f) {  // inlet: bounce-back with the prescribed velocity
-        return fo + 6.0f * Lat::w[I] * cdot<I>(uind[0], uind[1], uind[2]);
-    }
-    const int k = (int)(-2.0f - code);  // outlet k: anti-bounce-back at the prescribed pressure
-    const float cu = cdot<I>(up[0], up[1], up[2]);
-    return -fo + 2.0f * Lat::w[I] * (3.0f * p_out[k] + 4.5f * cu * cu - 1.5f * upsq);
-}
-
-template <int MODEL, bool GAL, bool BND, int PAR, bool H16>
-__global__ __launch_bounds__(WG) void k_bricks(cs_bricks_step_args a) {
-    using T = std::conditional_t<H16, _Float16, float>;
-    // Even steps stage the brick's own block in LDS: 27 full-row loads and stores instead of 27 gathers that each
-    // touch many lines of the block (in an even step no other wavefront touches the block). Odd steps access
-    // global memory directly (staging the 26 small neighbour records of an odd step costs more than the gathers).
-    constexpr bool STAGED = PAR == 0;
-    __shared__ int tbl[WAVES][32];
-    __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
-    __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
-    __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
-    const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
-    load_table(slots32, kSlots);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int L = threadIdx.x & 63;
-    // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
-    const int per = gridDim.x >> 3;
-    const int wg = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    const int64_t li = (int64_t)wg * WAVES + wave;
-    const bool valid = li < a.n_list;
-    const int b = valid ? a.list[li] : 0;
-    if (L < 27) tbl[wave][L] = valid ? a.nbr[(int64_t)b * 27 + L] : -1;
-    __syncthreads();
-    if (!valid) return;  // wave-uniform
-
-    const int x = L & 3, y = (L >> 2) & 3, z = L >> 4;
-    const uint32_t S = (uint32_t)a.n_slots;
-    const uint32_t slot = (uint32_t)b * BC + L;
-    const uint32_t base = brick_base(b);
-    T* __restrict__ F = reinterpret_cast<T*>(a.f);
-    T* stage = stage_all + (STAGED ? wave * Q * BC : 0);
-    float4* halo = halo_all + (H16 ? wave * kHalo : 0);
-    if constexpr (H16) {
-        load_halo(halo, tbl[wave], a.rho_old, a.vel_old, S, L);
-        wave_sync();
-    }
-
-    uint32_t mask = kAllLinks;
-    int bb = 0;
-    if constexpr (BND) {
-        bb = (int)li;  // the boundary list is in boundary-index order: bidx[list[li]] == li
-        mask = a.lmask[(uint32_t)bb * BC + L];
-    }
-    const bool live = mask != 0u;  // solid slots of boundary bricks only join the staging copies
-
-    // ---- streaming: raw values ---------------------------------------------------------------------------
-    float f[Q];
-    // value of direction i as stored -> float. FP16: a fluid link holds P_i(n - c_i), a non-fluid one the bounced
-    // P_opp(i)(n); each is decoded with its writer's moments and its own direction.
-    auto value = [&](auto ic, T raw) -> float {
-        constexpr int i = decltype(ic)::value;
-        if constexpr (!H16) {
-            return raw;
-        } else {
-            if (!BND || ((mask >> i) & 1u))
-                return h16_decode<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
-            return h16_decode<Lat::opp[i]>(raw, halo[halo_index<0, 0, 0>(x, y, z)]);
-        }
-    };
-    if constexpr (STAGED) {
-        stage_copy<PAR, false>(F, stage, tbl[wave], base, L);
-        wave_sync();
-        if (live)
-            static_for<0, Q>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                f[i] = value(ic, stage[slt[i * BC + L]]);
-            });
-    } else if (live) {
-        static_for<0, Q>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            constexpr int io = Lat::opp[i];
-            if constexpr (i == 0) {
-                f[i] = value(ic, F[base + slt[L]]);
-            } else {
-                int dd, l;
-                shift_pos<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z, dd, l);
-                if (!BND || ((mask >> i) & 1u)) {
-                    f[i] = value(ic, F[brick_base(tbl[wave][dd]) + slt[io * BC + l]]);
-                } else {
-                    f[i] = value(ic, F[base + slt[i * BC + L]]);
-                }
-            }
-        });
-    }
-
-    if (live) {
-        const float om = a.om_brick[b] ? a.om_cell[slot] : a.omega;
-        const float d = a.drive[0];
-        // ---- boundary links and the previous velocity --------------------------------------------------------
-        float up[3] = {0.0f, 0.0f, 0.0f};
-        if constexpr (BND) {
-            up[0] = a.vel_old[slot];
-            up[1] = a.vel_old[S + slot];
-            up[2] = a.vel_old[2 * S + slot];
-            if ((mask & kAllLinks) != kAllLinks) {
-                const float upsq = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
-                float uind[3] = {0.0f, 0.0f, 0.0f};
-                if (mask & kInletBit) {
-                    uind[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
-                    uind[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
-                    uind[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
-                }
-                const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
-                const float* qc = a.qc + (uint32_t)bb * Q * BC + L;
-                const float* side = a.side + (uint32_t)bb * Q * BC + L;
-                static_for<1, Q>([&](auto ic) {
-                    constexpr int i = decltype(ic)::value;
-                    if constexpr (pair_head<i>()) {
-                        constexpr int j = i + 1;
-                        const bool fi = (mask >> i) & 1u, fj = (mask >> j) & 1u;
-                        const float ri = f[i], rj = f[j];
-                        float vi = ri, vj = rj;
-                        if (!fi) {
-                            const float ps = ((sm >> i) & 1u) ? side[i * BC] : rj;
-                            vi = link_value<i>(qc[i * BC], ri, fj, rj, ps, a.bouzidi, uind, up, upsq, a.p_out);
-                        }
-                        if (!fj) {
-                            const float ps = ((sm >> j) & 1u) ? side[j * BC] : ri;
-                            vj = link_value<j>(qc[j * BC], rj, fi, ri, ps, a.bouzidi, uind, up, upsq, a.p_out);
-                        }
-                        f[i] = vi;
-                        f[j] = vj;
-                    }
-                });
-            }
-        }
-
-        // ---- Galilean correction: d u_a / d x_a of the previous velocity (lbm.py:105-126, 196-202) ------------
-        CollideParams p;
-        p.om = om;
-        p.om_m = MODEL == 0 ? 1.0f / (0.5f + a.magic / (1.0f / om - 0.5f)) : 0.0f;
-        p.F[0] = a.force[0] * d;
-        p.F[1] = a.force[1] * d;
-        p.F[2] = a.force[2] * d;
-        p.t3 = GAL ? 1.0f : 0.0f;
-        p.sigma = 1.0f;
-        p.g[0] = p.g[1] = p.g[2] = 0.0f;
-        if constexpr (GAL) {
-            static_for<0, 3>([&](auto ac) {
-                constexpr int ax = decltype(ac)::value;
-                constexpr int i1 = 2 * ax + 1, i2 = 2 * ax + 2;
-                int d1, l1, d2, l2;
-                shift_pos<ax == 0 ? -1 : 0, ax == 1 ? -1 : 0, ax == 2 ? -1 : 0>(x, y, z, d1, l1);
-                shift_pos<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z, d2, l2);
-                const float* V = a.vel_old + ax * S;
-                // neighbour velocity component ax of the previous step at x -+ e_ax
-                auto nbv = [&](bool minus) -> float {
-                    if constexpr (H16) {
-                        const int e = minus ? halo_index<ax == 0 ? -1 : 0, ax == 1 ? -1 : 0, ax == 2 ? -1 : 0>(x, y, z)
-                                            : halo_index<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z);
-                        const float4 m = halo[e];
-                        return ax == 0 ? m.y : (ax == 1 ? m.z : m.w);
-                    } else {
-                        return minus ? V[(uint32_t)tbl[wave][d1] * BC + l1] : V[(uint32_t)tbl[wave][d2] * BC + l2];
-                    }
-                };
-                if constexpr (!BND) {
-                    const float v1 = nbv(true);
-                    const float v2 = nbv(false);
-                    p.g[ax] = (v2 - v1) * 0.5f;
-                } else {
-                    float h1 = 1.0f, v1 = up[ax], h2 = 1.0f, v2 = up[ax];
-                    if ((mask >> i1) & 1u) {
-                        v1 = nbv(true);
-                    } else {
-                        const float c = a.qc[((uint32_t)bb * Q + i1) * BC + L];
-                        if (c >= 0.0f) {
-                            h1 = fmaxf(c, 0.1f);
-                            v1 = 0.0f;
-                        }
-                    }
-                    if ((mask >> i2) & 1u) {
-                        v2 = nbv(false);
-                    } else {
-                        const float c = a.qc[((uint32_t)bb * Q + i2) * BC + L];
-                        if (c >= 0.0f) {
-                            h2 = fmaxf(c, 0.1f);
-                            v2 = 0.0f;
-                        }
-                    }
-                    p.g[ax] = (h1 * h1 * (v2 - up[ax]) + h2 * h2 * (up[ax] - v1)) / (h1 * h2 * (h1 + h2));
-                }
-            });
-        }
-
-        // ---- collision -----------------------------------------------------------------------------------------
-        float dr, u[3];
-        bool inlet_cell = false;
-        if constexpr (BND) inlet_cell = (mask & kInletBit) != 0u;
-        if (inlet_cell) {
-            dr = density_deviation(f);
-            u[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
-            u[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
-            u[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
-            inlet_equilibrium(f, dr, u[0], u[1], u[2], p.t3);
-        } else {
-            collide<MODEL, false>(f, p, dr, u);
-        }
-
-        // ---- outputs -------------------------------------------------------------------------------------------
-        a.vel[slot] = u[0];
-        a.vel[S + slot] = u[1];
-        a.vel[2 * S + slot] = u[2];
-        a.rho[slot] = dr;
-        if constexpr (BND) {
-            const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
-            if (sm) {
-                float* side = a.side + (uint32_t)bb * Q * BC + L;
-                static_for<1, Q>([&](auto ic) {
-                    constexpr int i = decltype(ic)::value;
-                    if ((sm >> i) & 1u) side[i * BC] = f[i];
-                });
-            }
-        }
-        if constexpr (H16) {
-            // from here on f holds the stored form: the deviation from this cell's new equilibrium
-            const float rp = dr - 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-            static_for<0, Q>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                f[i] = (float)h16_encode<i>(f[i], rp, u[0], u[1], u[2]);
-            });
-        }
-        if constexpr (!STAGED) {
-            // odd: P_i(n) -> slot i of n + c_i (where the next even step of n + c_i reads it), or back into the own
-            // slot opp(i) when n + c_i is not fluid
-            static_for<0, Q>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                constexpr int io = Lat::opp[i];
-                if constexpr (i == 0) {
-                    F[base + slt[L]] = (T)f[i];
-                } else if (!BND || ((mask >> io) & 1u)) {
-                    int dd, l;
-                    shift_pos<Lat::cx[i], Lat::cy[i], Lat::cz[i]>(x, y, z, dd, l);
-                    F[brick_base(tbl[wave][dd]) + slt[i * BC + l]] = (T)f[i];
-                } else {
-                    F[base + slt[io * BC + L]] = (T)f[i];
-                }
-            });
-        }
-    }
-    if constexpr (STAGED) {
-        // even: P_i(n) -> own slot opp(i), through the stage
-        wave_sync();
-        if (live)
-            static_for<0, Q>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                constexpr int io = Lat::opp[i];
-                stage[slt[io * BC + L]] = (T)f[i];
-            });
-        wave_sync();
-        stage_copy<PAR, true>(F, stage, tbl[wave], base, L);
-    }
-}
+constexpr Variant kVariant[] = {{4, 1}, {4, 0}, {2, 1}, {2, 0}, {1, 1}, {1, 0}};
+constexpr int kVariants = (int)(sizeof(kVariant) / sizeof(kVariant[0]));
 
 // ---- conversions between the AA storage and the reference (Q, N) layout --------------------------------------
 __device__ __forceinline__ uint32_t cell_mask(const cs_bricks_convert_args& a, int b, int L, int& bb) {
@@ -567,27 +63,6 @@ __global__ __launch_bounds__(256) void k_convert(cs_bricks_convert_args a) {
     });
 }
 
-template <int MODEL, bool GAL, bool BND>
-hipError_t launch_par(const cs_bricks_step_args& a, hipStream_t s) {
-    const int64_t groups = (a.n_list + WAVES - 1) / WAVES;
-    const unsigned grid = (unsigned)(((groups + 7) / 8) * 8);
-    const bool h = a.storage == CS_STORAGE_FP16;
-    if (a.parity == 0) {
-        if (h) k_bricks<MODEL, GAL, BND, 0, true><<<grid, WG, 0, s>>>(a);
-        else k_bricks<MODEL, GAL, BND, 0, false><<<grid, WG, 0, s>>>(a);
-    } else {
-        if (h) k_bricks<MODEL, GAL, BND, 1, true><<<grid, WG, 0, s>>>(a);
-        else k_bricks<MODEL, GAL, BND, 1, false><<<grid, WG, 0, s>>>(a);
-    }
-    return hipGetLastError();
-}
-
-template <int MODEL>
-hipError_t launch_model(const cs_bricks_step_args& a, hipStream_t s) {
-    if (a.galilean) return a.boundary ? launch_par<MODEL, true, true>(a, s) : launch_par<MODEL, true, false>(a, s);
-    return a.boundary ? launch_par<MODEL, false, true>(a, s) : launch_par<MODEL, false, false>(a, s);
-}
-
 }  // namespace
 }  // namespace cs
 
@@ -610,8 +85,12 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
     if (a->n_list == 0) return CS_OK;
     CS_REQUIRE(a->n_list > 0 && a->n_list <= a->n_slots / cs::BC, "cs_bricks_step: n_list %lld",
                (long long)a->n_list);
-    hipError_t e = a->model == CS_MODEL_REGULARISED ? cs::launch_model<1>(*a, cs::as_stream(stream))
-                                                     : cs::launch_model<0>(*a, cs::as_stream(stream));
+    CS_REQUIRE(a->variant >= 0 && a->variant < cs::kVariants, "cs_bricks_step: variant %d not in [0, %d)", a->variant,
+               cs::kVariants);
+    const cs::Variant v = cs::kVariant[a->variant];
+    hipStream_t st = cs::as_stream(stream);
+    hipError_t e = v.waves == 4 ? cs::bricks_launch_w4(*a, st, v.xcd)
+                 : v.waves == 2 ? cs::bricks_launch_w2(*a, st, v.xcd) : cs::bricks_launch_w1(*a, st, v.xcd);
     if (e != hipSuccess) {
         cs::set_error("cs_bricks_step launch failed: %s", hipGetErrorString(e));
         return CS_E_HIP;
@@ -659,3 +138,11 @@ extern "C" int cs_bricks_stage_table(uint16_t* out) {
 
 extern "C" int cs_bricks_get_f(const cs_bricks_convert_args* a, void* stream) { return convert(a, stream, false); }
 extern "C" int cs_bricks_set_f(const cs_bricks_convert_args* a, void* stream) { return convert(a, stream, true); }
+
+extern "C" int cs_bricks_variant_info(int32_t variant, int32_t* waves_per_group, int32_t* xcd_order) {
+    if (variant < 0 || variant >= cs::kVariants) return cs::kVariants;
+    CS_REQUIRE(waves_per_group && xcd_order, "cs_bricks_variant_info: null output");
+    *waves_per_group = cs::kVariant[variant].waves;
+    *xcd_order = cs::kVariant[variant].xcd;
+    return CS_OK;
+}

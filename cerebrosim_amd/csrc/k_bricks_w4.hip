@@ -1,0 +1,8 @@
+// Instantiation of the brick step for 4 bricks (wavefronts) per workgroup.
+#include "k_bricks.hpp"
+
+namespace cs {
+hipError_t bricks_launch_w4(const cs_bricks_step_args& a, hipStream_t s, int xcd) {
+    return launch_bricks<4>(a, s, xcd);
+}
+}  // namespace cs

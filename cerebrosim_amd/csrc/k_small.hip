@@ -10,16 +10,28 @@ namespace cs {
 namespace {
 
 // ---- k_clock ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void clock_tick(int32_t* tick, const float* coef, int nh, int period, int ramp,
+                                           float* drive) {
+    const int t = tick[0];
+    const float pi = 3.14159265358979323846f;
+    const float ph = 2.0f * pi * (float)(t % period) / (float)period;
+    float d = coef[0];
+    for (int k = 1; k <= nh; ++k) d += coef[2 * k - 1] * cosf((float)k * ph) + coef[2 * k] * sinf((float)k * ph);
+    if (t < ramp) d *= 0.5f - 0.5f * cosf(pi * (float)t / (float)ramp);
+    drive[0] = d;
+    tick[0] = t + 1;
+}
+
 __global__ void k_clock(cs_clock_args a) {
     if (threadIdx.x != 0) return;
-    const int t = a.tick[0];
-    const float pi = 3.14159265358979323846f;
-    const float ph = 2.0f * pi * (float)(t % a.period) / (float)a.period;
-    float d = a.coef[0];
-    for (int k = 1; k <= a.nh; ++k) d += a.coef[2 * k - 1] * cosf((float)k * ph) + a.coef[2 * k] * sinf((float)k * ph);
-    if (t < a.ramp) d *= 0.5f - 0.5f * cosf(pi * (float)t / (float)a.ramp);
-    a.drive[0] = d;
-    a.tick[0] = t + 1;
+    clock_tick(a.tick, a.coef, a.nh, a.period, a.ramp, a.drive);
+}
+
+// one thread per case of a packed set of lattices
+__global__ void k_clock_cases(cs_clock_cases_args a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n_cases || a.period[c] <= 0) return;
+    clock_tick(a.tick + c, a.coef + (int64_t)c * a.coef_stride, a.nh[c], a.period[c], a.ramp[c], a.drive + c);
 }
 
 // ---- block-wide sum in a fixed order (1024 threads = 16 waves of 64) ---------------------------------------
@@ -54,27 +66,49 @@ __device__ float block_max(float v, float* lds) {
 // ---- k_flux + k_bc (one workgroup) ----------------------------------------------------------------------------
 constexpr int kMaxOpen = 64;
 
-__global__ __launch_bounds__(kRedThreads) void k_flux_bc(cs_flux_args a) {
-    __shared__ float lds[kRedThreads / 64];
-    __shared__ float qs[kMaxOpen];
-    for (int k = 0; k < a.n_open; ++k) {
+// Flow of each opening's slab over cells [0, m) and the opening-pressure update, for one lattice (workgroup).
+__device__ __forceinline__ void flux_bc(const int32_t* idx, const float* w, const int32_t* slot, int64_t m,
+                                        const float* vel, int64_t vel_elem, int64_t vel_comp, const float* R,
+                                        float drive, float* p_out, float* q_last, int n_open, float pref, float alpha,
+                                        float* lds, float* qs) {
+    for (int k = 0; k < n_open; ++k) {
         float acc = 0.0f;
-        for (int64_t t = threadIdx.x; t < a.m; t += kRedThreads) {
-            if (a.slot[t] != k) continue;
-            const int64_t s = a.idx[t];
-            const float* v = a.vel + s * a.vel_elem;
-            acc += v[0] * a.w[3 * t] + v[a.vel_comp] * a.w[3 * t + 1] + v[2 * a.vel_comp] * a.w[3 * t + 2];
+        for (int64_t t = threadIdx.x; t < m; t += kRedThreads) {
+            if (slot[t] != k) continue;
+            const int64_t s = idx[t];
+            const float* v = vel + s * vel_elem;
+            acc += v[0] * w[3 * t] + v[vel_comp] * w[3 * t + 1] + v[2 * vel_comp] * w[3 * t + 2];
         }
         const float tot = block_sum(acc, lds);
         if (threadIdx.x == 0) qs[k] = tot;
     }
     if (threadIdx.x == 0) {
-        a.p_out[0] = a.drive[0];
-        for (int k = 0; k < a.n_open; ++k) {
-            a.q_last[k] = qs[k];
-            if (k > 0) a.p_out[k] = a.p_out[k] + a.alpha * (a.R[k] * qs[k] - a.pref - a.p_out[k]);
+        p_out[0] = drive;
+        for (int k = 0; k < n_open; ++k) {
+            q_last[k] = qs[k];
+            if (k > 0) p_out[k] = p_out[k] + alpha * (R[k] * qs[k] - pref - p_out[k]);
         }
     }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_flux_bc(cs_flux_args a) {
+    __shared__ float lds[kRedThreads / 64];
+    __shared__ float qs[kMaxOpen];
+    flux_bc(a.idx, a.w, a.slot, a.m, a.vel, a.vel_elem, a.vel_comp, a.R, a.drive[0], a.p_out, a.q_last, a.n_open,
+            a.pref, a.alpha, lds, qs);
+}
+
+// one workgroup per case of a packed set of lattices
+__global__ __launch_bounds__(kRedThreads) void k_flux_cases(cs_flux_cases_args a) {
+    __shared__ float lds[kRedThreads / 64];
+    __shared__ float qs[kMaxOpen];
+    const int c = blockIdx.x;
+    const int n_open = a.n_open[c];
+    if (n_open <= 0) return;  // workgroup-uniform
+    const int64_t s0 = a.start[c], m = a.start[c + 1] - s0;
+    const int64_t o = (int64_t)c * a.open_stride;
+    flux_bc(a.idx + s0, a.w + 3 * s0, a.slot + s0, m, a.vel, a.vel_elem, a.vel_comp, a.R + o, a.drive[c],
+            a.p_out + o, a.q_last + o, n_open, a.pref[c], a.alpha[c], lds, qs);
 }
 
 // ---- k_gather (one workgroup) ---------------------------------------------------------------------------------
@@ -186,6 +220,29 @@ extern "C" int cs_flux_bc(const cs_flux_args* a, void* stream) {
     CS_REQUIRE(a->m == 0 || (a->idx && a->w && a->slot && a->vel), "cs_flux_bc: null cell list");
     cs::k_flux_bc<<<1, cs::kRedThreads, 0, cs::as_stream(stream)>>>(*a);
     CS_LAUNCHED("k_flux_bc");
+    return CS_OK;
+}
+
+extern "C" int cs_clock_cases(const cs_clock_cases_args* a, void* stream) {
+    CS_REQUIRE(a && a->tick && a->coef && a->nh && a->period && a->ramp && a->drive, "cs_clock_cases: null argument");
+    CS_REQUIRE(a->n_cases >= 0 && a->n_cases <= CS_MAX_CASES && a->coef_stride >= 1,
+               "cs_clock_cases: n_cases %d / coef_stride %d", a->n_cases, a->coef_stride);
+    if (a->n_cases == 0) return CS_OK;
+    cs::k_clock_cases<<<(a->n_cases + 63) / 64, 64, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_clock_cases");
+    return CS_OK;
+}
+
+extern "C" int cs_flux_cases(const cs_flux_cases_args* a, void* stream) {
+    CS_REQUIRE(a && a->start && a->R && a->drive && a->p_out && a->q_last && a->n_open && a->pref && a->alpha &&
+                   a->vel,
+               "cs_flux_cases: null argument");
+    CS_REQUIRE(a->n_cases >= 0 && a->n_cases <= CS_MAX_CASES && a->open_stride >= 1 &&
+                   a->open_stride <= cs::kMaxOpen,
+               "cs_flux_cases: n_cases %d / open_stride %d", a->n_cases, a->open_stride);
+    if (a->n_cases == 0) return CS_OK;
+    cs::k_flux_cases<<<a->n_cases, cs::kRedThreads, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_flux_cases");
     return CS_OK;
 }
 

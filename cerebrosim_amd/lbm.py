@@ -244,6 +244,25 @@ class _BricksBackend:
             self.set_f(s, cur, f)
 
 
+def sponge_tau(lat, tau0, length, tau_max, kinds=("inlet", "outlet")):
+    """Per-cell tau with sponge layers at the openings of the given kinds (lbm.py:398-416)."""
+    P = lat["origin"] + lat["coords"] * lat["dx"]
+    tau = np.full(lat["nbr"].shape[1], tau0)
+    for o in lat["openings"]:
+        if o["kind"] not in kinds:
+            continue
+        centre = np.asarray(o["centre"], float)
+        normal = np.asarray(o["normal"], float)
+        rel = P - centre
+        along = rel @ normal
+        depth = -along / lat["dx"]
+        side = np.linalg.norm(rel - along[:, None] * normal, axis=1) < 1.5 * o["radius"]
+        w = np.clip(1.0 - depth / length, 0.0, 1.0)
+        w = w * w * (3.0 - 2.0 * w)
+        tau = np.maximum(tau, np.where(side, tau0 + (tau_max - tau0) * w, tau0))
+    return tau
+
+
 class Solver:
     """Lattice Boltzmann on a sparse lattice (lbm.py:346-455). ``model`` "trt" (magic = Lambda) or "regularised";
     ``sigma`` < 1 blends the regularised stress with the finite-difference one. ``layout`` selects the device
@@ -357,21 +376,7 @@ class Solver:
     def set_sponge(self, length, tau_max, kinds=("inlet", "outlet")):
         """Raise tau smoothly towards the openings of the given kinds: tau_max at the opening plane, back to the
         fluid's tau `length` cells inside, within 1.5 radii of the opening axis (lbm.py:398-416). Written in place."""
-        lat = self.lat
-        P = lat["origin"] + lat["coords"] * lat["dx"]
-        tau = np.full(self.N, self.tau)
-        for o in lat["openings"]:
-            if o["kind"] not in kinds:
-                continue
-            centre = np.asarray(o["centre"], float)
-            normal = np.asarray(o["normal"], float)
-            rel = P - centre
-            along = rel @ normal
-            depth = -along / lat["dx"]
-            side = np.linalg.norm(rel - along[:, None] * normal, axis=1) < 1.5 * o["radius"]
-            w = np.clip(1.0 - depth / length, 0.0, 1.0)
-            w = w * w * (3.0 - 2.0 * w)
-            tau = np.maximum(tau, np.where(side, self.tau + (tau_max - self.tau) * w, self.tau))
+        tau = sponge_tau(self.lat, self.tau, length, tau_max, kinds)
         self.tau_cell = tau
         self.synchronize()
         with _torch().cuda.stream(self._tstream):

@@ -68,6 +68,20 @@ typedef struct cs_clock_args {
 } cs_clock_args;
 int cs_clock(const cs_clock_args* a, void* stream);
 
+/* The clocks of packed lattices (cases): k_clock for each case c < n_cases with period[c] > 0, one thread each;
+ * coefficients of case c at coef + c * coef_stride. */
+typedef struct cs_clock_cases_args {
+    int32_t* tick;              /* [n_cases] */
+    const float* coef;          /* [n_cases][coef_stride] */
+    const int32_t* nh;          /* [n_cases] */
+    const int32_t* period;      /* [n_cases]; 0 = unused case */
+    const int32_t* ramp;        /* [n_cases] */
+    float* drive;               /* [n_cases] */
+    int32_t n_cases;            /* at most CS_MAX_CASES */
+    int32_t coef_stride;
+} cs_clock_cases_args;
+int cs_clock_cases(const cs_clock_cases_args* a, void* stream);
+
 /* ---------------------------------------------------------------------------------------------------------------
  * Solver step, indexed layout -- replaces k_step (lbm.py:140-257) launched at lbm.py:425 with dim = N.
  * Same arrays as the reference: f_in/f_out (Q, N) row-major, nbr/qw (Q, N), uin/vel_old/vel (N, 3) float AoS,
@@ -150,13 +164,16 @@ int cs_bricks_stage_table(uint16_t* out);
 /* One AA step over a list of bricks (two launches per step: interior bricks, then boundary bricks).
  * parity 0 = even step (reads and writes the cell's own slots), 1 = odd step (pulls from and pushes to the
  * neighbours). Velocity is stored as three component planes of n_slots floats; rho as one plane. */
+#define CS_CASE_SHIFT 22        /* brick-list entry = brick | case << CS_CASE_SHIFT (bricks < 2^22, cases < 1024) */
+#define CS_MAX_CASES 1024
+#define CS_LIST_SKIP 0xFFFFFFFFu /* list entry of an unused record: the wavefront does nothing */
 #define CS_STORAGE_FP32 0       /* populations stored as float32 deviations f - w */
 #define CS_STORAGE_FP16 1       /* populations stored as float16 of (f - feq2(writer's moments)) / w * 2^10 */
 typedef struct cs_bricks_step_args {
     void* f;                    /* [n_bricks][1728] float or half, slot order of cs_bricks_slot_table */
     const int32_t* nbr;         /* [n_bricks][27] */
-    const int32_t* list;        /* brick ids to process; for boundary bricks the "boundary" list of the builder
-                                   (its position li is the brick's boundary index) */
+    const uint32_t* list;       /* bricks to process (entries brick | case << CS_CASE_SHIFT, or CS_LIST_SKIP); for
+                                   boundary bricks, position li in the list is the brick's boundary record */
     int64_t n_list;
     const int32_t* bidx;        /* [n_bricks] */
     const uint32_t* lmask;      /* [n_boundary][64] */
@@ -182,8 +199,18 @@ typedef struct cs_bricks_step_args {
     int32_t parity;
     int32_t boundary;           /* 1 = the list holds boundary bricks */
     int32_t storage;            /* CS_STORAGE_* */
+    /* Several lattices (cases) packed in one brick array: case c of a list entry takes drive[c], the opening
+     * pressures p_out + c * p_out_stride, and, when given, case_omega[c] / case_force[3c..3c+2] in place of
+     * omega / force. A single lattice: case 0, NULL tables. */
+    const float* case_omega;
+    const float* case_force;
+    int32_t p_out_stride;
+    int32_t variant;            /* launch variant, 0 .. cs_bricks_variant_info(-1) - 1 (0 = default) */
 } cs_bricks_step_args;
 int cs_bricks_step(const cs_bricks_step_args* a, void* stream);
+/* Launch variant `variant`: wavefronts (bricks) per workgroup and XCD-aware workgroup order. Returns CS_OK, or the
+ * number of variants when `variant` is out of range (no outputs written). */
+int cs_bricks_variant_info(int32_t variant, int32_t* waves_per_group, int32_t* xcd_order);
 
 /* Conversions between the AA brick storage and the reference (Q, N) layout of post-collision populations, for the
  * state that the next step of the given parity reads. set_f also refreshes the side copies. */
@@ -230,6 +257,29 @@ typedef struct cs_flux_args {
     float alpha;
 } cs_flux_args;
 int cs_flux_bc(const cs_flux_args* a, void* stream);
+
+/* k_flux + k_bc for packed lattices: one workgroup per case c with n_open[c] > 0, over the slab cells
+ * start[c] .. start[c+1]-1 of the concatenated lists; per-case arrays at c * open_stride. Same arithmetic and
+ * summation order as cs_flux_bc on the case alone. */
+typedef struct cs_flux_cases_args {
+    const int32_t* idx;
+    const float* w;
+    const int32_t* slot;
+    const int64_t* start;       /* [n_cases + 1] */
+    const float* vel;
+    int64_t vel_elem;
+    int64_t vel_comp;
+    const float* R;             /* [n_cases][open_stride] */
+    const float* drive;         /* [n_cases] */
+    float* p_out;               /* [n_cases][open_stride] */
+    float* q_last;              /* [n_cases][open_stride] */
+    const int32_t* n_open;      /* [n_cases]; 0 = no flow measurement */
+    const float* pref;          /* [n_cases] */
+    const float* alpha;         /* [n_cases] */
+    int32_t n_cases;
+    int32_t open_stride;        /* at most 64 */
+} cs_flux_cases_args;
+int cs_flux_cases(const cs_flux_cases_args* a, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Probe reduction -- replaces k_gather (simulate.py:85-90) launched at simulate.py:309: out[slot] = sum rho / 3
