@@ -1,0 +1,59 @@
+"""Serving form on the GPU: requests arriving over time are admitted into the running batch (continuous batching)
+under an HBM budget; each request's result is the same bits as the request solved alone."""
+import numpy as np
+import pytest
+
+from cerebrosim_amd import serving, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def small_requests():
+    specs = [dict(R=5.0, L=40.0, area_frac=0.5, tau=0.6), dict(R=7.0, L=24.0, area_frac=0.0, tau=0.8),
+             dict(R=4.0, L=64.0, area_frac=0.8, tau=0.55), dict(R=6.0, L=32.0, area_frac=0.3, tau=0.7)]
+    out = []
+    for k, sp in enumerate(specs):
+        sp = dict(sp, name=f"r{k}", arrival=0.04 * k, steps=200 + 100 * (k % 2),
+                  u_mean=workloads.inlet_speed(sp["R"], sp["area_frac"], sp["tau"]))
+        out.append(sp)
+    return out
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_served_results_equal_sequential(storage):
+    reqs = small_requests()
+    srv = serving.Server(storage=storage, chunk=100, arena_bricks=300, arena_records=300, keep="fields")
+    rep = srv.serve(reqs, workloads.tube_case)
+    seq = serving.run_sequential(reqs, workloads.tube_case, storage=storage, chunk=100, keep="fields")
+    assert rep["cases"] == len(reqs) == seq["cases"]
+    got = {r["name"]: r for r in rep["records"]}
+    for s in seq["records"]:
+        g = got[s["name"]]
+        assert g["steps"] == s["steps"]
+        assert np.array_equal(g["fields"][1], s["fields"][1]) and np.array_equal(g["fields"][0], s["fields"][0]), \
+            s["name"]
+        assert g["result"]["ok"] and g["result"]["u_max"] > 1e-4
+    assert 0 < rep["p50_s"] <= rep["p99_s"] and rep["cases_per_hour"] > 0
+    print(f"{storage}: served {rep['cases_per_hour']:.0f} cases/h p50 {rep['p50_s']:.3f} s p99 {rep['p99_s']:.3f} s; "
+          f"sequential {seq['cases_per_hour']:.0f} cases/h", flush=True)
+
+
+def test_budget_queues_requests():
+    """A budget that holds one request at a time: the others wait, all finish, results unchanged."""
+    reqs = small_requests()[:3]
+    from cerebrosim_amd import batch as B
+    from cerebrosim_amd import bricks
+    lays = [bricks.build(workloads.tube_case(r)[0]) for r in reqs]
+    big = max(B.arena_bytes(l["n_bricks"], l["n_boundary"], "fp32") for l in lays)
+    srv = serving.Server(storage="fp32", chunk=100, arena_bricks=1, arena_records=1, budget_bytes=int(1.2 * big),
+                         keep="fields")
+    for r in reqs:
+        r["arrival"] = 0.0
+    rep = srv.serve(reqs, workloads.tube_case)
+    recs = sorted(rep["records"], key=lambda r: r["admitted"])
+    for a, b in zip(recs, recs[1:]):
+        assert b["admitted"] >= a["done"] - 1e-6          # one at a time
+    seq = serving.run_sequential(reqs, workloads.tube_case, storage="fp32", chunk=100, keep="fields")
+    got = {r["name"]: r for r in rep["records"]}
+    for s in seq["records"]:
+        assert np.array_equal(got[s["name"]]["fields"][1], s["fields"][1])
