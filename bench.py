@@ -6,6 +6,17 @@ correction, Bouzidi walls, FP32 arithmetic. A "step" is one solver step over all
 ``--storage fp16`` stores the populations as float16 equilibrium deviations (140 B per fluid cell: 27 populations
 read + written x 2 B, previous velocity and density read 16 B, velocity + density written 16 B).
 
+The chosen forms are measured after the headline (``--forms``, default all three, on reduced samples unless
+``--full``):
+
+* precision: FP16 population storage on the same pipe (MLUPS and roofline of its dominant launch);
+* batching: a seeded set of stenosed tubes (radius U[8, 48], length U[128, 960], area reduction U[0, 0.95],
+  tau U[0.51, 0.9]; 32 tubes at half size by default, the full 96 with ``--full``) stepped together in brick arenas
+  (autotuned launch variant), FP32 and FP16, against the same tubes solved one after the other by the plain solver;
+* serving: Poisson-arriving stenosis requests (severity, pressure, grid spacing) served with continuous batching in
+  FP16 under the HBM budget -- cases/hour with p50/p99 time-to-solution -- against the same requests solved one
+  after the other by the plain FP32 solver (both include building each lattice on the host).
+
 Prints one JSON line. ``roofline.achieved`` = algorithmic bytes of the step kernel per launch (244 B per fluid cell:
 27 populations read + 27 written x 4 B, previous velocity read 12 B, velocity + density written 16 B) / the kernel's
 mean duration measured with HIP events on the solver's stream. ``cpu_baseline`` times the oracle (the C restatement
@@ -83,6 +94,119 @@ def cpu_baseline(lat, steps):
                       f"{threads} threads; the reference's Warp CPU path cannot run here)"}
 
 
+def form_precision(lat, steps, warmup):
+    """FP16 storage on the headline pipe: MLUPS and the dominant launch's algorithmic bandwidth."""
+    import torch
+    s = make_solver(lat, "bricks", "fp16")
+    s.run(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(steps)
+    s.synchronize()
+    secs = time.perf_counter() - t0
+    ok = s.ok()
+    kt = kernel_times_ms(s)
+    name, (k_ms, k_cells) = max(kt.items(), key=lambda kv: kv[1][0])
+    ach = BYTES_PER_LUP["fp16"] * k_cells / (k_ms * 1e-3) / 1e9
+    del s
+    torch.cuda.empty_cache()
+    return {"storage": "fp16", "mlups": round(lat["nbr"].shape[1] * steps / secs / 1e6, 1), "ok": ok,
+            "kernel": name, "kernel_ms": round(k_ms, 5), "achieved_gbs": round(ach, 1),
+            "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_cell": BYTES_PER_LUP["fp16"],
+            "error_vs_fp64": "tests/test_gpu_fp16.py: relative L2 of u within 1e-3 of the FP64 oracle"}
+
+
+def form_batching(full, steps=400):
+    """Tubes stepped together (FP32 and FP16 arenas) vs solved one after the other (plain FP32 solver)."""
+    import torch
+    from cerebrosim_amd import batch as B
+    from cerebrosim_amd import bricks, lbm, workloads
+    specs = workloads.tube_specs(96, seed=1)
+    if not full:
+        specs = [dict(sp, R=sp["R"] / 2, L=sp["L"] / 2, u_mean=workloads.inlet_speed(sp["R"] / 2, sp["area_frac"],
+                                                                                      sp["tau"]))
+                 for sp in specs[:32]]
+    t0 = time.perf_counter()
+    cases = []
+    for sp in specs:
+        lat, nu, setup = workloads.tube_case(sp)
+        cases.append((lat, nu, setup, bricks.build(lat)))
+    host_s = time.perf_counter() - t0
+    cells = sum(c[0]["nbr"].shape[1] for c in cases)
+    print(f"bench: batching sample {len(cases)} tubes, {cells} cells, built in {host_s:.1f} s", file=sys.stderr,
+          flush=True)
+    out = {"workload": f"tubes_seed1_{'n96' if full else 'n32_half_size'}", "cases": len(cases), "cells": cells,
+           "steps": steps, "host_build_s": round(host_s, 1)}
+    # sequential plain solves (same host layouts; set-up and stepping timed apart)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step_s = 0.0
+    for lat, nu, setup, lay in cases:
+        s = lbm.Solver(lat, nu, model="regularised", layout="bricks", brick_layout=lay)
+        setup(s)
+        s.synchronize()
+        t1 = time.perf_counter()
+        s.run(steps)
+        s.synchronize()
+        step_s += time.perf_counter() - t1
+        del s
+    seq = time.perf_counter() - t0
+    torch.cuda.empty_cache()
+    out["sequential_fp32"] = {"s": round(seq, 3), "setup_s": round(seq - step_s, 3), "step_s": round(step_s, 3),
+                              "mlups": round(cells * steps / step_s / 1e6, 1)}
+    for storage in ("fp32", "fp16"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bt = B.Batch(storage=storage, arena_bricks=min(B.MAX_BRICKS, 1 << 21), arena_records=1 << 19,
+                     max_cases=max(128, len(cases)))
+        hs = [bt.add(lat, nu, layout=lay) for lat, nu, setup, lay in cases]
+        for h, (lat, nu, setup, lay) in zip(hs, cases):
+            setup(h)
+        bt.synchronize()
+        t_add = time.perf_counter() - t0
+        tune = bt.autotune(steps=10, repeats=1)
+        t1 = time.perf_counter()
+        bt.run(steps)
+        bt.synchronize()
+        run_s = time.perf_counter() - t1
+        ok = all(h.ok() for h in hs)
+        tot = t_add + run_s
+        out[f"batched_{storage}"] = {"s": round(tot, 3), "admit_s": round(t_add, 3), "step_s": round(run_s, 3),
+                                     "mlups": round(cells * steps / run_s / 1e6, 1), "arenas": len(bt.arenas),
+                                     "variant": bt.variant, "autotune_ms_per_step": {str(k): round(v, 4)
+                                                                                      for k, v in tune.items()},
+                                     "ok": ok}
+        out[f"speedup_{storage}_vs_sequential"] = round(seq / tot, 2)
+        out[f"step_speedup_{storage}_vs_sequential"] = round(step_s / run_s, 2)
+        del hs, bt
+        torch.cuda.empty_cache()
+    return out
+
+
+def form_serving(full, rate=None):
+    """Stenosis requests served with continuous batching (FP16) vs solved one after the other (FP32)."""
+    import torch
+    from cerebrosim_amd import serving, workloads
+    n = 64 if full else 32
+    grids = (0.2e-3, 0.1e-3, 0.05e-3) if full else (0.2e-3, 0.1e-3)
+    beat = 2000 if full else 500
+    rate = rate or (8.0 if full else 32.0)          # arrivals faster than service: the queue builds up
+    reqs = workloads.request_specs(n, rate_per_s=rate, seed=2, beats=4, beat_steps=beat, grids=grids)
+    srv = serving.Server(storage="fp16", chunk=200)
+    rep = srv.serve(reqs, workloads.request_case)
+    del srv
+    torch.cuda.empty_cache()
+    seq = serving.run_sequential(reqs, workloads.request_case, storage="fp32", chunk=200)
+    ok = all(r["result"]["ok"] for r in rep["records"])
+    return {"workload": f"stenosis_requests_seed2_n{n}_{'full' if full else 'grids_0.2_0.1mm'}_beat{beat}",
+            "requests": n, "rate_per_s": rate, "storage": "fp16", "chunk": rep["chunk"],
+            "cases_per_hour": round(rep["cases_per_hour"], 1), "p50_s": round(rep["p50_s"], 3),
+            "p99_s": round(rep["p99_s"], 3), "wall_s": round(rep["wall_s"], 2), "mlups": round(rep["mlups"], 1),
+            "ok": ok, "sequential_fp32_cases_per_hour": round(seq["cases_per_hour"], 1),
+            "sequential_fp32_wall_s": round(seq["wall_s"], 2),
+            "speedup_vs_sequential": round(rep["cases_per_hour"] / seq["cases_per_hour"], 2)}
+
+
 def load_traffic(layout, storage="fp32"):
     tag = layout if storage == "fp32" else f"{layout}_{storage}"
     path = os.path.join(ROOT, "profiles", f"traffic_step_{tag}_womersley.json")
@@ -118,6 +242,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
+    ap.add_argument("--forms", default="precision,batching,serving",
+                    help="comma-separated chosen forms to measure after the headline ('none' to skip)")
+    ap.add_argument("--full", action="store_true", help="full-size batching / serving workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (the step of one lattice "
@@ -162,6 +289,20 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(lat, args.cpu_steps)
+    forms = [f for f in args.forms.split(",") if f and f != "none"]
+    if forms:
+        del s
+        torch.cuda.empty_cache()
+        out["forms"] = {}
+        if "precision" in forms:
+            out["forms"]["precision"] = form_precision(lat, args.steps, args.warmup)
+            print(f"bench: precision form {out['forms']['precision']['mlups']} MLUPS", file=sys.stderr, flush=True)
+        if "batching" in forms:
+            out["forms"]["batching"] = form_batching(args.full)
+            print(f"bench: batching form done", file=sys.stderr, flush=True)
+        if "serving" in forms:
+            out["forms"]["serving"] = form_serving(args.full)
+            print(f"bench: serving form done", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -268,10 +268,11 @@ class Solver:
     ``sigma`` < 1 blends the regularised stress with the finite-difference one. ``layout`` selects the device
     storage: "bricks" (4^3 bricks, AA in-place streaming), "indexed" (reference (Q, N) layout, any neighbour table)
     or "auto" (bricks when the lattice and options allow, else indexed). ``storage`` "fp16" keeps the populations
-    in float16 (bricks layout only; float32 arithmetic and reductions)."""
+    in float16 (bricks layout only; float32 arithmetic and reductions). ``brick_layout``: the host layout of
+    ``bricks.build(lat)`` when the caller already has it."""
 
     def __init__(self, lat, nu_lu, magic=3.0 / 16.0, bouzidi=True, model="trt", sigma=1.0, galilean=None,
-                 device=None, layout="auto", storage="fp32"):
+                 device=None, layout="auto", storage="fp32", brick_layout=None):
         torch = _torch()
         self.dev = _require_gpu(device)
         self.lat = lat
@@ -298,7 +299,7 @@ class Solver:
             if storage not in ("fp32", "fp16"):
                 raise ValueError(f"unknown storage {storage!r}")
             self.storage = storage
-            self.backend = self._make_backend(lat, layout)
+            self.backend = self._make_backend(lat, layout, brick_layout)
         self._tstream.synchronize()
         self.layout = self.backend.name
         self.force = (0.0, 0.0, 0.0)
@@ -307,7 +308,7 @@ class Solver:
         self.cur = 0
         self.steps = 0
 
-    def _make_backend(self, lat, layout):
+    def _make_backend(self, lat, layout, brick_layout=None):
         from . import bricks
         if layout not in ("auto", "bricks", "indexed"):
             raise ValueError(f"unknown layout {layout!r}")
@@ -320,7 +321,8 @@ class Solver:
             reason = "the hybrid blend (sigma < 1) runs on the indexed layout"
         else:
             try:
-                return _BricksBackend(self, lat, bricks.build(lat), self.storage)
+                lay = brick_layout if brick_layout is not None else bricks.build(lat)
+                return _BricksBackend(self, lat, lay, self.storage)
             except bricks.LayoutError as e:
                 reason = str(e)
         if layout == "bricks" or self.storage != "fp32":
