@@ -84,29 +84,36 @@ class _Ranges:
         return self.free[-1][0]
 
 
-def arena_bytes(bricks, records, storage):
-    """Device bytes of an arena with the given capacities."""
+LINKS_PER_RECORD = 320        # default capacity of the compact link pool per boundary record (a sphere-like
+SIDES_PER_RECORD = 160        # wall has ~125 missing links and ~60 side copies per boundary brick)
+
+
+def arena_bytes(bricks, records, storage, links=None, sides=None):
+    """Device bytes of an arena with the given capacities (link / side pools default to the per-record figures)."""
+    links = LINKS_PER_RECORD * records if links is None else links
+    sides = SIDES_PER_RECORD * records if sides is None else sides
     elem = 2 if storage == "fp16" else 4
     per_brick = BLOCK * elem + 27 * 4 + 4 + 1 + BC * 4 + 2 * 3 * BC * 4 + BC * 4 * (2 if storage == "fp16" else 1) + 4
-    per_record = BC * (4 + 4 + 27 * 4 + 3 * 4 + 27 * 4) + 4
-    return int(bricks) * per_brick + int(records) * per_record
+    per_record = BC * (4 + 4 + 3 * 4) + 2 * 4
+    return int(bricks) * per_brick + int(records) * per_record + 4 * (int(links) + int(sides))
 
 
 def case_bytes(layout, storage):
-    """Device bytes a case occupies inside an arena (its bricks and boundary records)."""
-    return arena_bytes(layout["n_bricks"], layout["n_boundary"], storage)
+    """Device bytes a case occupies inside an arena (its bricks, boundary records and link / side entries)."""
+    return arena_bytes(layout["n_bricks"], layout["n_boundary"], storage, layout["n_links"], layout["n_side"])
 
 
 class Arena:
     """Packed brick storage for the cases of one storage type."""
 
-    def __init__(self, batch, bricks, records):
+    def __init__(self, batch, bricks, records, links, sides):
         torch = _torch()
         if not 0 < bricks <= MAX_BRICKS:
             raise ValueError(f"arena of {bricks} bricks: 1 .. {MAX_BRICKS} (32-bit population offsets)")
         dev = batch.dev
         self.batch = batch
         self.cap_b, self.cap_r = int(bricks), max(1, int(records))
+        self.cap_l, self.cap_s = max(1, int(links)), max(1, int(sides))
         self.S = self.cap_b * BC
         dt = torch.float16 if batch.storage == "fp16" else torch.float32
         z = lambda n, d=torch.float32: torch.zeros(n, dtype=d, device=dev)
@@ -120,30 +127,43 @@ class Arena:
         self.rhos = [r, z(self.S)] if batch.storage == "fp16" else [r, r]
         self.lmask = z(self.cap_r * BC, torch.int32)
         self.smask = z(self.cap_r * BC, torch.int32)
-        self.qc = z(self.cap_r * 27 * BC)
+        self.qlink = z(self.cap_l)
+        self.qstart = z(self.cap_r, torch.int32)
         self.uin = z(self.cap_r * 3 * BC)
-        self.side = z(self.cap_r * 27 * BC)
+        self.side = z(self.cap_s)
+        self.sstart = z(self.cap_r, torch.int32)
         self.n_open = z(batch.max_cases, torch.int32)       # flow-measuring cases of this arena
         self.bricks = _Ranges(self.cap_b)
         self.records = _Ranges(self.cap_r if records > 0 else 0)
+        self.links = _Ranges(self.cap_l)
+        self.sides = _Ranges(self.cap_s)
         self.cases = []
         self.interior = self.boundary = None
         self.n_interior = self.n_boundary = 0
         self.flux = None
 
     def nbytes(self):
-        return arena_bytes(self.cap_b, self.cap_r, self.batch.storage)
+        return arena_bytes(self.cap_b, self.cap_r, self.batch.storage, self.cap_l, self.cap_s)
+
+    def _pools(self, lay):
+        return ((self.bricks, int(lay["n_bricks"])), (self.records, int(lay["n_boundary"])),
+                (self.links, int(lay["n_links"])), (self.sides, int(lay["n_side"])))
 
     def place(self, lay):
-        """(first brick, first record) for the layout, or None."""
-        b0 = self.bricks.alloc(int(lay["n_bricks"]))
-        if b0 is None:
-            return None
-        r0 = self.records.alloc(int(lay["n_boundary"])) if lay["n_boundary"] else 0
-        if r0 is None:
-            self.bricks.release(b0, int(lay["n_bricks"]))
-            return None
-        return b0, r0
+        """(first brick, first record, first link, first side entry) for the layout, or None."""
+        got = []
+        for pool, n in self._pools(lay):
+            s = pool.alloc(n)
+            if s is None:
+                for (p2, n2), s2 in zip(self._pools(lay), got):
+                    p2.release(s2, n2)
+                return None
+            got.append(s)
+        return tuple(got)
+
+    def release(self, lay, spot):
+        for (pool, n), s in zip(self._pools(lay), spot):
+            pool.release(s, n)
 
     def rebuild_lists(self):
         """Device lists from the cases now in the arena (host arrays, one upload each)."""
@@ -248,16 +268,26 @@ class Batch:
     def nbytes(self):
         return sum(a.nbytes() for a in self.arenas)
 
-    def _new_arena(self, lay):
-        nb = max(self.arena_bricks, int(lay["n_bricks"]))
+    def _arena_shape(self, lay):
+        nb = min(max(self.arena_bricks, int(lay["n_bricks"])), MAX_BRICKS)
         nr = max(self.arena_records, int(lay["n_boundary"]))
-        nb = min(nb, MAX_BRICKS)
+        nl = max(LINKS_PER_RECORD * self.arena_records, int(lay["n_links"]))
+        ns = max(SIDES_PER_RECORD * self.arena_records, int(lay["n_side"]))
+        return nb, nr, nl, ns
+
+    def arena_bytes_for(self, lay):
+        """Bytes of the arena this batch would open for a case of this layout."""
+        nb, nr, nl, ns = self._arena_shape(lay)
+        return arena_bytes(nb, nr, self.storage, nl, ns)
+
+    def _new_arena(self, lay):
+        nb, nr, nl, ns = self._arena_shape(lay)
         if lay["n_bricks"] > nb:
             raise ValueError(f"case of {lay['n_bricks']} bricks exceeds one arena ({MAX_BRICKS} bricks)")
-        if self.nbytes() + arena_bytes(nb, nr, self.storage) > self.budget:
+        if self.nbytes() + arena_bytes(nb, nr, self.storage, nl, ns) > self.budget:
             return None
         with _torch().cuda.stream(self._tstream):
-            a = Arena(self, nb, nr)
+            a = Arena(self, nb, nr, nl, ns)
         self.arenas.append(a)
         return a
 
@@ -266,22 +296,12 @@ class Batch:
         if len(self.case_by_id) >= self.max_cases:
             return False
         for a in self.arenas:
-            b0 = a.bricks.alloc(int(lay["n_bricks"]))
-            if b0 is None:
-                continue
-            ok = True
-            r0 = None
-            if lay["n_boundary"]:
-                r0 = a.records.alloc(int(lay["n_boundary"]))
-                ok = r0 is not None
-                if ok:
-                    a.records.release(r0, int(lay["n_boundary"]))
-            a.bricks.release(b0, int(lay["n_bricks"]))
-            if ok:
+            spot = a.place(lay)
+            if spot is not None:
+                a.release(lay, spot)
                 return True
-        nb = max(self.arena_bricks, int(lay["n_bricks"]))
-        nr = max(self.arena_records, int(lay["n_boundary"]))
-        return self.nbytes() + arena_bytes(min(nb, MAX_BRICKS), nr, self.storage) <= self.budget
+        nb, nr, nl, ns = self._arena_shape(lay)
+        return self.nbytes() + arena_bytes(nb, nr, self.storage, nl, ns) <= self.budget
 
     # --- cases ----------------------------------------------------------------------------------------------
     def add(self, lat, nu_lu, layout=None):
@@ -306,7 +326,7 @@ class Batch:
                 raise BatchFull("case does not fit a fresh arena")
             spot = (a, p)
         self.synchronize()
-        c = Case(self, cid, lat, nu_lu, lay, spot[0], *spot[1])
+        c = Case(self, cid, lat, nu_lu, lay, spot[0], spot[1])
         self.case_by_id[cid] = c
         spot[0].cases.append(c)
         self._topology_changed(spot[0])
@@ -318,9 +338,7 @@ class Batch:
         self.synchronize()
         a = case.arena
         a.cases.remove(case)
-        a.bricks.release(case.b0, int(case.lay["n_bricks"]))
-        if case.lay["n_boundary"]:
-            a.records.release(case.r0, int(case.lay["n_boundary"]))
+        a.release(case.lay, case.spot)
         with torch.cuda.stream(self._tstream):
             self.period[case.cid] = 0
         del self.case_by_id[case.cid]
@@ -352,7 +370,8 @@ class Batch:
         A.f, A.nbr, A.list = arena.f.data_ptr(), arena.nbr.data_ptr(), lst.data_ptr()
         A.n_list = arena.n_boundary if boundary else arena.n_interior
         A.bidx, A.lmask, A.smask = arena.bidx.data_ptr(), arena.lmask.data_ptr(), arena.smask.data_ptr()
-        A.qc, A.uin, A.side = arena.qc.data_ptr(), arena.uin.data_ptr(), arena.side.data_ptr()
+        A.qlink, A.qstart, A.uin = arena.qlink.data_ptr(), arena.qstart.data_ptr(), arena.uin.data_ptr()
+        A.side, A.sstart = arena.side.data_ptr(), arena.sstart.data_ptr()
         A.om_cell, A.om_brick, A.omega = arena.om.data_ptr(), arena.om_brick.data_ptr(), 1.0
         A.p_out, A.drive = self.p_out.data_ptr(), self.drive.data_ptr()
         A.vel_old, A.vel = arena.vels[a].data_ptr(), arena.vels[1 - a].data_ptr()
@@ -498,9 +517,11 @@ class Case:
     """One lattice inside a Batch, with the lbm.Solver surface (lbm.py:346-455) and the resistance-outlet flow
     measurement of simulate.py (ArterySolver)."""
 
-    def __init__(self, batch, cid, lat, nu_lu, lay, arena, b0, r0):
+    def __init__(self, batch, cid, lat, nu_lu, lay, arena, spot):
         torch = _torch()
-        self.batch, self.cid, self.lat, self.lay, self.arena, self.b0, self.r0 = batch, cid, lat, lay, arena, b0, r0
+        b0, r0, q0, s0 = spot
+        self.batch, self.cid, self.lat, self.lay, self.arena = batch, cid, lat, lay, arena
+        self.spot, self.b0, self.r0 = spot, b0, r0
         self.N = int(lat["nbr"].shape[1])
         self.tau = 3.0 * nu_lu + 0.5
         self.steps = 0
@@ -528,9 +549,13 @@ class Case:
             if nr:
                 arena.lmask[r0 * BC:(r0 + nr) * BC].copy_(up(lay["lmask"].view(np.int32).reshape(-1)))
                 arena.smask[r0 * BC:(r0 + nr) * BC].copy_(up(lay["smask"].view(np.int32).reshape(-1)))
-                arena.qc[r0 * 27 * BC:(r0 + nr) * 27 * BC].copy_(up(lay["qc"].reshape(-1)))
                 arena.uin[r0 * 3 * BC:(r0 + nr) * 3 * BC].copy_(up(lay["uin"].reshape(-1)))
-                arena.side[r0 * 27 * BC:(r0 + nr) * 27 * BC].zero_()
+                arena.qstart[r0:r0 + nr].copy_(up((lay["qstart"][:-1].astype(np.int64) + q0).astype(np.int32)))
+                arena.sstart[r0:r0 + nr].copy_(up((lay["sstart"][:-1].astype(np.int64) + s0).astype(np.int32)))
+                if lay["n_links"]:
+                    arena.qlink[q0:q0 + int(lay["n_links"])].copy_(up(lay["qlink"]))
+                if lay["n_side"]:
+                    arena.side[s0:s0 + int(lay["n_side"])].zero_()
             c = cid
             batch.tick[c] = 0
             batch.drive[c] = 0.0
@@ -637,6 +662,7 @@ class Case:
         A = _native.BricksConvertArgs()
         A.f, A.nbr, A.bidx = ar.f.data_ptr(), ar.nbr.data_ptr(), ar.bidx.data_ptr()
         A.lmask, A.smask, A.side = ar.lmask.data_ptr(), ar.smask.data_ptr(), ar.side.data_ptr()
+        A.sstart = ar.sstart.data_ptr()
         A.slot_of_cell, A.n_cells = self.slot32.data_ptr(), self.N
         A.f_ref, A.parity = f_ref.data_ptr(), cur
         A.storage, A.vel, A.rho, A.n_slots = STORAGES[self.batch.storage], ar.vels[cur].data_ptr(), \

@@ -79,7 +79,38 @@ def build(lat):
             out[name] = arr
     finally:
         lib.cs_bricks_free(h)
+    out.update(compact_links(out))
     return out
+
+
+def compact_links(lay):
+    """The per-step boundary data in compact form (only links that exist):
+
+    * ``qlink``: qc of every missing link of every boundary brick -- brick by brick, direction 1..26, then lane --
+      with ``qstart`` (n_boundary + 1) the start of each brick's run;
+    * the Bouzidi side copies (links with a bit in smask) are stored in the same order, starting at ``sstart``
+      (n_boundary + 1); ``n_side`` entries in all.
+
+    A wavefront finds its lane's entry for direction i from the brick's start, the counts of the directions before
+    i and the lanes before it with the same link (ballot prefix counts)."""
+    nb = int(lay["n_boundary"])
+    lm = lay["lmask"].reshape(nb, 64)
+    dirs = np.arange(27, dtype=np.uint32)
+    missing = ((lm[:, None, :] >> dirs[None, :, None]) & 1) == 0
+    missing &= (lm != 0)[:, None, :]
+    missing[:, 0, :] = False
+    qlink = np.ascontiguousarray(lay["qc"].reshape(nb, 27, 64)[missing], np.float32)
+    qstart = np.zeros(nb + 1, np.int64)
+    np.cumsum(missing.sum((1, 2)), out=qstart[1:])
+    sm = lay["smask"].reshape(nb, 64)
+    sbits = ((sm[:, None, :] >> dirs[None, :, None]) & 1) == 1
+    sbits[:, 0, :] = False
+    sstart = np.zeros(nb + 1, np.int64)
+    np.cumsum(sbits.sum((1, 2)), out=sstart[1:])
+    if qstart[-1] >= 2 ** 31 or sstart[-1] >= 2 ** 31:
+        raise LayoutError("more than 2^31 boundary links")
+    return {"qlink": qlink, "qstart": qstart.astype(np.int32), "sstart": sstart.astype(np.int32),
+            "n_links": int(qstart[-1]), "n_side": int(sstart[-1])}
 
 
 def slot_table():
@@ -94,6 +125,13 @@ def stage_table():
     odd step (own group first, then the neighbour records in offset order 0..12, 14..26)."""
     out = np.empty((27, 64), np.uint16)
     _native.call("cs_bricks_stage_table", ctypes.c_void_p(out.ctypes.data))
+    return out
+
+
+def record_table():
+    """(728,) uint16: neighbour offset o << 11 | block offset of staged odd-step positions 1000 .. 1727."""
+    out = np.empty(27 * 64 - 1000, np.uint16)
+    _native.call("cs_bricks_record_table", ctypes.c_void_p(out.ctypes.data))
     return out
 
 

@@ -31,6 +31,18 @@ __device__ __forceinline__ uint32_t location(const cs_bricks_convert_args& a, in
     return brick_base(nbk) + kSlots.p[I][l];
 }
 
+// Position of the side copy of link i of lane L of boundary brick bb in the compact side array: after the copies
+// of the directions before i and of the lanes before L with link i (the order the step kernels count with ballots).
+__device__ uint32_t side_index(const cs_bricks_convert_args& a, int bb, int L, int i) {
+    uint32_t n = (uint32_t)a.sstart[bb];
+    const uint32_t below = (1u << i) - 2u;  // directions 1 .. i-1
+    for (int l = 0; l < BC; ++l) {
+        const uint32_t s = a.smask[(uint32_t)bb * BC + l];
+        n += (uint32_t)__popc(s & below) + ((l < L) ? ((s >> i) & 1u) : 0u);
+    }
+    return n;
+}
+
 // P_i(n) is written by n itself, so the FP16 form is encoded and decoded with n's own newest moments.
 template <bool SET, bool H16>
 __global__ __launch_bounds__(256) void k_convert(cs_bricks_convert_args a) {
@@ -55,7 +67,7 @@ __global__ __launch_bounds__(256) void k_convert(cs_bricks_convert_args a) {
         if constexpr (SET) {
             if constexpr (H16) F[loc] = h16_encode<i>(*ref, m.x, m.y, m.z, m.w);
             else F[loc] = *ref;
-            if ((sm >> i) & 1u) a.side[((size_t)bb * Q + i) * BC + L] = *ref;
+            if ((sm >> i) & 1u) a.side[side_index(a, bb, L, i)] = *ref;
         } else {
             if constexpr (H16) *ref = h16_decode<i>(F[loc], m);
             else *ref = F[loc];
@@ -70,8 +82,8 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
     CS_REQUIRE(a && a->f && a->nbr && a->list && a->bidx && a->om_cell && a->om_brick && a->p_out && a->drive &&
                    a->vel_old && a->vel && a->rho,
                "cs_bricks_step: null argument");
-    CS_REQUIRE(!a->boundary || (a->lmask && a->smask && a->qc && a->uin && a->side),
-               "cs_bricks_step: boundary bricks need lmask/smask/qc/uin/side");
+    CS_REQUIRE(!a->boundary || (a->lmask && a->smask && a->qlink && a->qstart && a->uin && a->side && a->sstart),
+               "cs_bricks_step: boundary bricks need lmask/smask/qlink/qstart/uin/side/sstart");
     CS_REQUIRE(a->model == CS_MODEL_TRT || a->model == CS_MODEL_REGULARISED, "cs_bricks_step: unknown model %d",
                a->model);
     CS_REQUIRE(a->parity == 0 || a->parity == 1, "cs_bricks_step: parity %d", a->parity);
@@ -99,7 +111,8 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
 }
 
 static int convert(const cs_bricks_convert_args* a, void* stream, bool set) {
-    CS_REQUIRE(a && a->f && a->nbr && a->bidx && a->slot_of_cell && a->f_ref && a->lmask && a->smask && a->side,
+    CS_REQUIRE(a && a->f && a->nbr && a->bidx && a->slot_of_cell && a->f_ref && a->lmask && a->smask && a->side &&
+                   a->sstart,
                "cs_bricks convert: null argument");
     CS_REQUIRE(a->parity == 0 || a->parity == 1, "cs_bricks convert: parity %d", a->parity);
     CS_REQUIRE(a->storage == CS_STORAGE_FP32 || a->storage == CS_STORAGE_FP16, "cs_bricks convert: storage %d",
@@ -133,6 +146,13 @@ extern "C" int cs_bricks_stage_table(uint16_t* out) {
     constexpr cs::SlotTable t = cs::make_oddpos();
     for (int k = 0; k < cs::Q; ++k)
         for (int l = 0; l < cs::BC; ++l) out[k * cs::BC + l] = t.p[k][l];
+    return CS_OK;
+}
+
+extern "C" int cs_bricks_record_table(uint16_t* out) {
+    CS_REQUIRE(out != nullptr, "cs_bricks_record_table: null output");
+    constexpr cs::RecordTable t = cs::make_records();
+    for (int k = 0; k < cs::kRecs; ++k) out[k] = t.e[k];
     return CS_OK;
 }
 

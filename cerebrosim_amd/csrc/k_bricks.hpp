@@ -118,6 +118,36 @@ constexpr SlotTable make_oddpos() {
     return t;
 }
 
+// The odd-step footprint of an interior brick in staged order: the own group (block offsets [0, kOwn) of the own
+// block), then the 26 neighbour records. kRecords.e[p] = (o << 11) | block offset, in brick tbl[o], of staged
+// position kOwn + p.
+constexpr int kOwn = 1000;
+constexpr int kRecs = Q * BC - kOwn;
+struct RecordTable {
+    uint16_t e[kRecs];
+};
+
+constexpr RecordTable make_records() {
+    RecordTable t{};
+    int p = 0;
+    for (int oi = 0; oi < 27; ++oi) {
+        const int h = oi == 0 ? 13 : (oi <= 13 ? oi - 1 : oi);
+        if (h == 13) continue;
+        const int g = 26 - h;
+        for (int k = 0; k < group_size(g); ++k) t.e[p++] = (uint16_t)((h << 11) | (group_start(g) + k));
+    }
+    return t;
+}
+
+static_assert(group_start(13) == 0 && group_size(13) == kOwn, "own group first");
+static_assert(stage_start(0) == kOwn, "records after the own group");
+__constant__ RecordTable kRecords = make_records();
+__constant__ SlotTable kOddPos = make_oddpos();
+
+// Number of lanes below this one with their bit set in the wave mask m.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 
 // Cross-lane LDS hand-over inside one wavefront.
 __device__ __forceinline__ void wave_sync() {
@@ -135,9 +165,11 @@ __device__ __forceinline__ void load_table(uint32_t* lds, const SlotTable& src, 
 }
 
 // Move the footprint of brick b between global memory and the wave's LDS stage with full-row accesses.
-// Even steps: the whole own block. Odd steps: own group + the 26 records in the neighbour blocks.
+// Even steps: the whole own block. Odd steps (interior bricks): the own group as 16-byte vectors, then the 26
+// neighbour records packed lane by lane (rec: the record table in LDS).
 template <int PAR, bool STORE, typename T>
-__device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int* tbl, uint32_t base, int L) {
+__device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int* tbl, const uint16_t* rec,
+                                           uint32_t base, int L) {
     if constexpr (PAR == 0) {
         // the whole block as 16-byte vectors (a block is Q * BC * sizeof(T) bytes, a multiple of 16)
         constexpr int NV = Q * BC * (int)sizeof(T) / 16;
@@ -152,19 +184,27 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
             }
         });
     } else {
-        static_for<0, 27>([&](auto oc) {
-            constexpr int o = decltype(oc)::value;
-            constexpr int g = o == 13 ? 13 : 26 - o;
-            constexpr int size = group_size(g), start = group_start(g), st = stage_start(o);
-            const uint32_t nb = o == 13 ? base : brick_base(__builtin_amdgcn_readfirstlane(tbl[o]));
-            static_for<0, (size + BC - 1) / BC>([&](auto jc) {
-                constexpr int j = decltype(jc)::value;
-                const int idx = j * BC + L;
-                if (size % BC == 0 || j * BC + BC <= size || idx < size) {
-                    if constexpr (STORE) F[nb + start + idx] = stage[st + idx];
-                    else stage[st + idx] = F[nb + start + idx];
-                }
-            });
+        static_assert(kOwn * sizeof(T) % 16 == 0, "own group in 16-byte vectors");
+        constexpr int NV = kOwn * (int)sizeof(T) / 16;
+        uint4* g = reinterpret_cast<uint4*>(F + base);
+        uint4* l = reinterpret_cast<uint4*>(stage);
+        static_for<0, (NV + BC - 1) / BC>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const int k = j * BC + L;
+            if (NV % BC == 0 || (j + 1) * BC <= NV || k < NV) {
+                if constexpr (STORE) g[k] = l[k];
+                else l[k] = g[k];
+            }
+        });
+        static_for<0, (kRecs + BC - 1) / BC>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const int p = j * BC + L;
+            if ((j + 1) * BC <= kRecs || p < kRecs) {
+                const uint32_t e = rec[p];
+                const uint32_t at = brick_base(tbl[e >> 11]) + (e & 2047u);
+                if constexpr (STORE) F[at] = stage[kOwn + p];
+                else stage[kOwn + p] = F[at];
+            }
         });
     }
 }
@@ -268,16 +308,24 @@ template <int W, int MODEL, bool GAL, bool BND, int PAR, bool H16>
 __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xcd) {
     constexpr int WAVES = W;
     using T = std::conditional_t<H16, _Float16, float>;
-    // Even steps stage the brick's own block in LDS: 27 full-row loads and stores instead of 27 gathers that each
-    // touch many lines of the block (in an even step no other wavefront touches the block). Odd steps access
-    // global memory directly (staging the 26 small neighbour records of an odd step costs more than the gathers).
-    constexpr bool STAGED = PAR == 0;
+    // Even steps stage the brick's own block in LDS: vector loads and stores instead of 27 gathers that each touch
+    // many lines of the block (in an even step no other wavefront touches the block). Odd steps of interior bricks
+    // stage their footprint (own group + one record in each neighbour block: slots that no other wavefront
+    // touches, since every cell and link around an interior brick is fluid). Odd steps of boundary bricks access
+    // global memory directly: their records hold slots that neighbouring wavefronts write (redirected writes).
+    // slt: in staged odd steps, the staged position of the value lane l reads for direction i (which is also where
+    // it writes P_opp(i)); otherwise the block offset of population i of cell l.
+    constexpr bool STAGED = PAR == 0 || !BND;
+    constexpr bool ODD_STAGED = PAR == 1 && !BND;
     __shared__ int tbl[WAVES][32];
     __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
+    __shared__ __align__(16) uint16_t rec_lds[ODD_STAGED ? kRecs : 1];
     __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
     __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
     const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
-    load_table(slots32, kSlots, 64 * W);
+    load_table(slots32, ODD_STAGED ? kOddPos : kSlots, 64 * W);
+    if constexpr (ODD_STAGED)
+        for (int k = threadIdx.x; k < kRecs; k += 64 * W) rec_lds[k] = kRecords.e[k];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int L = threadIdx.x & 63;
     // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
@@ -328,7 +376,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         }
     };
     if constexpr (STAGED) {
-        stage_copy<PAR, false>(F, stage, tbl[wave], base, L);
+        stage_copy<PAR, false>(F, stage, tbl[wave], rec_lds, base, L);
         wave_sync();
         if (live)
             static_for<0, Q>([&](auto ic) {
@@ -359,6 +407,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         const float* p_out = a.p_out + (int64_t)cs * a.p_out_stride;
         // ---- boundary links and the previous velocity --------------------------------------------------------
         float up[3] = {0.0f, 0.0f, 0.0f};
+        float q16[6];  // BND + GAL: qc of links 1..6 (read by the gradient stencil), loaded with the link batch
         if constexpr (BND) {
             up[0] = a.vel_old[slot];
             up[1] = a.vel_old[S + slot];
@@ -372,27 +421,50 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                     uind[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
                 }
                 const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
-                const float* qc = a.qc + (uint32_t)bb * Q * BC + L;
-                const float* side = a.side + (uint32_t)bb * Q * BC + L;
-                static_for<1, Q>([&](auto ic) {
-                    constexpr int i = decltype(ic)::value;
-                    if constexpr (pair_head<i>()) {
-                        constexpr int j = i + 1;
+                // compact link data: the lanes with missing link i (a ballot over the lanes active here, which are
+                // all lanes with any missing link) follow those of the directions before i
+                const float* ql = a.qlink + a.qstart[bb];
+                const float* sd = a.side + a.sstart[bb];
+                uint32_t qoff = 0, soff = 0;
+                // links in batches of 8 (4 opposite pairs): first issue the batch's loads, then evaluate
+                static_for<0, 4>([&](auto bc) {
+                    constexpr int h0 = 1 + 8 * decltype(bc)::value;
+                    constexpr int nl = h0 + 8 <= Q ? 8 : Q - h0;
+                    float qv[nl], pv[nl];
+                    static_for<0, nl>([&](auto kc) {
+                        constexpr int k = decltype(kc)::value;
+                        const bool miss = !((mask >> (h0 + k)) & 1u);
+                        const uint64_t bm = __ballot(miss);
+                        qv[k] = miss ? ql[qoff + lanes_below(bm)] : 0.0f;
+                        qoff += (uint32_t)__popcll(bm);
+                        const bool sb = (sm >> (h0 + k)) & 1u;
+                        const uint64_t bs = __ballot(sb);
+                        pv[k] = sb ? sd[soff + lanes_below(bs)] : 0.0f;
+                        soff += (uint32_t)__popcll(bs);
+                    });
+                    if constexpr (GAL && h0 == 1)
+                        static_for<0, 6>([&](auto kc) { q16[decltype(kc)::value] = qv[decltype(kc)::value]; });
+                    static_for<0, nl / 2>([&](auto pc) {
+                        constexpr int k = 2 * decltype(pc)::value;
+                        constexpr int i = h0 + k, j = i + 1;
+                        static_assert(pair_head<i>(), "batches start at pair heads");
                         const bool fi = (mask >> i) & 1u, fj = (mask >> j) & 1u;
                         const float ri = f[i], rj = f[j];
                         float vi = ri, vj = rj;
                         if (!fi) {
-                            const float ps = ((sm >> i) & 1u) ? side[i * BC] : rj;
-                            vi = link_value<i>(qc[i * BC], ri, fj, rj, ps, a.bouzidi, uind, up, upsq, p_out);
+                            const float ps = ((sm >> i) & 1u) ? pv[k] : rj;
+                            vi = link_value<i>(qv[k], ri, fj, rj, ps, a.bouzidi, uind, up, upsq, p_out);
                         }
                         if (!fj) {
-                            const float ps = ((sm >> j) & 1u) ? side[j * BC] : ri;
-                            vj = link_value<j>(qc[j * BC], rj, fi, ri, ps, a.bouzidi, uind, up, upsq, p_out);
+                            const float ps = ((sm >> j) & 1u) ? pv[k + 1] : ri;
+                            vj = link_value<j>(qv[k + 1], rj, fi, ri, ps, a.bouzidi, uind, up, upsq, p_out);
                         }
                         f[i] = vi;
                         f[j] = vj;
-                    }
+                    });
                 });
+            } else if constexpr (GAL) {
+                static_for<0, 6>([&](auto kc) { q16[decltype(kc)::value] = 0.0f; });
             }
         }
 
@@ -435,7 +507,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                     if ((mask >> i1) & 1u) {
                         v1 = nbv(true);
                     } else {
-                        const float c = a.qc[((uint32_t)bb * Q + i1) * BC + L];
+                        const float c = q16[i1 - 1];
                         if (c >= 0.0f) {
                             h1 = fmaxf(c, 0.1f);
                             v1 = 0.0f;
@@ -444,7 +516,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                     if ((mask >> i2) & 1u) {
                         v2 = nbv(false);
                     } else {
-                        const float c = a.qc[((uint32_t)bb * Q + i2) * BC + L];
+                        const float c = q16[i2 - 1];
                         if (c >= 0.0f) {
                             h2 = fmaxf(c, 0.1f);
                             v2 = 0.0f;
@@ -477,10 +549,14 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         if constexpr (BND) {
             const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
             if (sm) {
-                float* side = a.side + (uint32_t)bb * Q * BC + L;
+                float* sd = a.side + a.sstart[bb];
+                uint32_t soff = 0;
                 static_for<1, Q>([&](auto ic) {
                     constexpr int i = decltype(ic)::value;
-                    if ((sm >> i) & 1u) side[i * BC] = f[i];
+                    const bool sb = (sm >> i) & 1u;
+                    const uint64_t bs = __ballot(sb);
+                    if (sb) sd[soff + lanes_below(bs)] = f[i];
+                    soff += (uint32_t)__popcll(bs);
                 });
             }
         }
@@ -511,7 +587,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         }
     }
     if constexpr (STAGED) {
-        // even: P_i(n) -> own slot opp(i), through the stage
+        // even: P_i(n) -> own slot opp(i); staged odd: P_i(n) -> S[n + c_i][i], the slot n reads for direction
+        // opp(i) (the AA step reads and writes the same footprint); both at slt[opp(i)][l]
         wave_sync();
         if (live)
             static_for<0, Q>([&](auto ic) {
@@ -520,7 +597,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                 stage[slt[io * BC + L]] = (T)f[i];
             });
         wave_sync();
-        stage_copy<PAR, true>(F, stage, tbl[wave], base, L);
+        stage_copy<PAR, true>(F, stage, tbl[wave], rec_lds, base, L);
     }
 }
 

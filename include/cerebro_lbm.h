@@ -160,6 +160,9 @@ int cs_bricks_slot_table(uint16_t* out);
 /* Staging order of an interior brick's odd-step footprint (its own group, then the record of each neighbour in the
  * order o = 0..12, 14..26): position of the population lane l reads for direction i is out[i * 64 + l]. */
 int cs_bricks_stage_table(uint16_t* out);
+/* The 728 neighbour records of an interior brick's staged odd-step footprint, after its own 1000-slot group: entry
+ * p = (neighbour offset o << 11) | block offset, for staged position 1000 + p. */
+int cs_bricks_record_table(uint16_t* out);
 
 /* One AA step over a list of bricks (two launches per step: interior bricks, then boundary bricks).
  * parity 0 = even step (reads and writes the cell's own slots), 1 = odd step (pulls from and pushes to the
@@ -178,9 +181,12 @@ typedef struct cs_bricks_step_args {
     const int32_t* bidx;        /* [n_bricks] */
     const uint32_t* lmask;      /* [n_boundary][64] */
     const uint32_t* smask;      /* [n_boundary][64] */
-    const float* qc;            /* [n_boundary][27][64] */
+    const float* qlink;         /* codes (q >= 0 wall, -1 inlet, -2-k outlet k) of the missing links only: per
+                                   boundary brick, direction 1..26, then lane */
+    const int32_t* qstart;      /* [n_boundary + 1] start of each boundary brick's run in qlink */
     const float* uin;           /* [n_boundary][3][64] */
-    float* side;                /* [n_boundary][27][64] copies of q >= 1/2 partners */
+    float* side;                /* copies of the Bouzidi q >= 1/2 partners (links with a bit in smask), same order */
+    const int32_t* sstart;      /* [n_boundary + 1] start of each boundary brick's run in side */
     const float* om_cell;       /* [n_slots] per-cell 1 / tau, read only where om_brick != 0 */
     const uint8_t* om_brick;    /* [n_bricks] 1 = the brick's cells have their own omega */
     float omega;                /* 1 / tau of every other cell */
@@ -220,7 +226,8 @@ typedef struct cs_bricks_convert_args {
     const int32_t* bidx;
     const uint32_t* lmask;
     const uint32_t* smask;
-    float* side;
+    float* side;                /* compact side copies (cs_bricks_step_args.side) */
+    const int32_t* sstart;
     const int32_t* slot_of_cell; /* [N] */
     int64_t n_cells;
     float* f_ref;               /* device (Q, N) */

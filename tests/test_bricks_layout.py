@@ -169,6 +169,37 @@ def test_odd_step_staging_maps_to_direct_slots():
         assert staged == direct, i
     # a bijection: the 27 x 64 reads cover the footprint once
     assert sorted(stage.ravel().tolist()) == list(range(27 * 64))
+    # the packed record table the kernels load the footprint with
+    rec = bricks.record_table().astype(np.int64)
+    assert foot[:1000] == [(13, j) for j in range(1000)]
+    assert [(int(e >> 11), int(e & 2047)) for e in rec] == foot[1000:]
+
+
+def test_compact_link_arrays():
+    """qlink / side order: brick by brick, direction 1..26, lane; qstart / sstart the runs. Checked against the dense
+    qc, lmask and smask."""
+    lat = tilted_pipe()
+    lay = bricks.build(lat)
+    nb = lay["n_boundary"]
+    qc = lay["qc"].reshape(nb, 27, 64)
+    expect, sides = [], []
+    starts, sstarts = [0], [0]
+    for bb in range(nb):
+        for i in range(1, 27):
+            for l in range(64):
+                m = int(lay["lmask"][bb, l])
+                if m and not (m >> i) & 1:
+                    expect.append(qc[bb, i, l])
+                if (int(lay["smask"][bb, l]) >> i) & 1:
+                    sides.append((bb, i, l))
+        starts.append(len(expect))
+        sstarts.append(len(sides))
+    assert np.array_equal(lay["qlink"], np.asarray(expect, np.float32))
+    assert lay["qstart"].tolist() == starts and lay["sstart"].tolist() == sstarts
+    assert lay["n_links"] == len(expect) and lay["n_side"] == len(sides)
+    # every side link is a missing wall link with q >= 1/2
+    for bb, i, l in sides:
+        assert not (int(lay["lmask"][bb, l]) >> i) & 1 and qc[bb, i, l] >= 0.5
 
 
 def run_fp16_model(steps, bounced_dir=True):

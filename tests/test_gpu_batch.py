@@ -144,9 +144,9 @@ def test_admission_respects_budget():
     lat = pipe((0, 0, 1), 16, 4.0)
     from cerebrosim_amd import bricks
     lay = bricks.build(lat)
-    need = B.arena_bytes(lay["n_bricks"], lay["n_boundary"], "fp16")
-    bt = B.Batch(storage="fp16", arena_bricks=lay["n_bricks"], arena_records=lay["n_boundary"],
-                 budget_bytes=int(2.5 * need))
+    bt = B.Batch(storage="fp16", arena_bricks=lay["n_bricks"], arena_records=lay["n_boundary"])
+    need = bt.arena_bytes_for(lay)
+    bt.budget = int(2.5 * need)
     c1 = bt.add(lat, 0.05, layout=lay)
     c2 = bt.add(lat, 0.05, layout=lay)
     assert len(bt.arenas) == 2 and not bt.can_admit(lay)

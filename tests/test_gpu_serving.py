@@ -44,7 +44,10 @@ def test_budget_queues_requests():
     from cerebrosim_amd import batch as B
     from cerebrosim_amd import bricks
     lays = [bricks.build(workloads.tube_case(r)[0]) for r in reqs]
-    big = max(B.arena_bytes(l["n_bricks"], l["n_boundary"], "fp32") for l in lays)
+    probe = B.Batch(storage="fp32", arena_bricks=1, arena_records=1)
+    big = max(probe.arena_bytes_for(l) for l in lays)
+    small = min(probe.arena_bytes_for(l) for l in lays)
+    assert 1.2 * big < 2 * small
     srv = serving.Server(storage="fp32", chunk=100, arena_bricks=1, arena_records=1, budget_bytes=int(1.2 * big),
                          keep="fields")
     for r in reqs:
