@@ -53,7 +53,7 @@ class FluxArgs(ctypes.Structure):
 class GatherArgs(ctypes.Structure):
     _fields_ = [("idx", c_void_p), ("slot", c_void_p), ("m", c_i64), ("rho", c_void_p), ("vel", c_void_p),
                 ("vel_elem", c_i64), ("vel_comp", c_i64), ("out", c_void_p), ("umax", c_void_p),
-                ("n_probe", c_i32)]
+                ("n_probe", c_i32), ("rho_elem", c_i64)]
 
 
 class WssArgs(ctypes.Structure):

@@ -153,6 +153,7 @@ class ArterySolver(Solver):
         A.idx, A.slot, A.m = gi.data_ptr(), gs.data_ptr(), gi.numel()
         A.rho, A.vel, A.vel_elem, A.vel_comp = self.backend.density(self.cur).data_ptr(), \
             self.backend.velocity(self.cur).data_ptr(), elem, comp
+        A.rho_elem = self.backend.density_stride()
         A.out, A.umax, A.n_probe = out.data_ptr(), umax.data_ptr(), n_probe
         _native.call("cs_gather", ctypes.byref(A), self.stream_handle)
         self.synchronize()

@@ -27,7 +27,7 @@ import ctypes
 import numpy as np
 
 from . import _native
-from .lbm import _require_gpu, sponge_tau
+from .lbm import BrickMoments, _require_gpu, sponge_tau
 
 BC = 64
 BLOCK = 27 * BC
@@ -122,9 +122,7 @@ class Arena:
         self.bidx = torch.full((self.cap_b,), -1, dtype=torch.int32, device=dev)
         self.om_brick = z(self.cap_b, torch.uint8)
         self.om = z(self.S)
-        self.vels = [z((3, self.S)) for _ in range(2)]
-        r = z(self.S)
-        self.rhos = [r, z(self.S)] if batch.storage == "fp16" else [r, r]
+        self.mom = BrickMoments(self.S, batch.storage == "fp16", dev)
         self.lmask = z(self.cap_r * BC, torch.int32)
         self.smask = z(self.cap_r * BC, torch.int32)
         self.qlink = z(self.cap_l)
@@ -374,8 +372,7 @@ class Batch:
         A.side, A.sstart = arena.side.data_ptr(), arena.sstart.data_ptr()
         A.om_cell, A.om_brick, A.omega = arena.om.data_ptr(), arena.om_brick.data_ptr(), 1.0
         A.p_out, A.drive = self.p_out.data_ptr(), self.drive.data_ptr()
-        A.vel_old, A.vel = arena.vels[a].data_ptr(), arena.vels[1 - a].data_ptr()
-        A.rho, A.rho_old = arena.rhos[1 - a].data_ptr(), arena.rhos[a].data_ptr()
+        A.vel_old, A.vel, A.rho = arena.mom.step_ptrs(a)
         A.n_slots, A.storage = arena.S, STORAGES[self.storage]
         A.magic, A.bouzidi, A.model, A.galilean = self.magic, self.bouzidi, self.model, self.gal
         A.parity, A.boundary = a, int(boundary)
@@ -388,7 +385,8 @@ class Batch:
         A = _native.FluxCasesArgs()
         A.idx, A.w, A.slot, A.start = fx["idx"].data_ptr(), fx["w"].data_ptr(), fx["slot"].data_ptr(), \
             fx["start"].data_ptr()
-        A.vel, A.vel_elem, A.vel_comp = arena.vels[1 - a].data_ptr(), 1, arena.S
+        A.vel = arena.mom.velocity(1 - a).data_ptr()
+        A.vel_elem, A.vel_comp = arena.mom.velocity_addressing()
         A.R, A.drive, A.p_out, A.q_last = self.R.data_ptr(), self.drive.data_ptr(), self.p_out.data_ptr(), \
             self.q_last.data_ptr()
         A.n_open, A.pref, A.alpha = arena.n_open.data_ptr(), self.pref.data_ptr(), self.alpha.data_ptr()
@@ -493,8 +491,8 @@ class Batch:
         self.synchronize()
         with _torch().cuda.stream(self._tstream):
             s = {"tables": [t.clone() for t in (self.tick, self.drive, self.p_out, self.q_last)],
-                 "arenas": [[ar.f.clone(), ar.side.clone()] + [v.clone() for v in ar.vels] +
-                            [r.clone() for r in ar.rhos] for ar in self.arenas]}
+                 "arenas": [[ar.f.clone(), ar.side.clone()] + [t.clone() for t in ar.mom.tensors()]
+                            for ar in self.arenas]}
         self.synchronize()
         return s
 
@@ -504,12 +502,8 @@ class Batch:
             for dst, src in zip((self.tick, self.drive, self.p_out, self.q_last), s["tables"]):
                 dst.copy_(src)
             for ar, saved in zip(self.arenas, s["arenas"]):
-                ar.f.copy_(saved[0])
-                ar.side.copy_(saved[1])
-                ar.vels[0].copy_(saved[2])
-                ar.vels[1].copy_(saved[3])
-                ar.rhos[0].copy_(saved[4])
-                ar.rhos[1].copy_(saved[5])
+                for dst, src in zip([ar.f, ar.side] + ar.mom.tensors(), saved):
+                    dst.copy_(src)
         self.synchronize()
 
 
@@ -542,10 +536,7 @@ class Case:
             arena.om_brick[b0:b0 + nb].zero_()
             arena.om[b0 * BC:(b0 + nb) * BC].zero_()
             arena.f[b0 * BLOCK:(b0 + nb) * BLOCK].zero_()
-            for v in arena.vels:
-                v[:, b0 * BC:(b0 + nb) * BC].zero_()
-            for r in arena.rhos:
-                r[b0 * BC:(b0 + nb) * BC].zero_()
+            arena.mom.zero(b0 * BC, (b0 + nb) * BC)
             if nr:
                 arena.lmask[r0 * BC:(r0 + nr) * BC].copy_(up(lay["lmask"].view(np.int32).reshape(-1)))
                 arena.smask[r0 * BC:(r0 + nr) * BC].copy_(up(lay["smask"].view(np.int32).reshape(-1)))
@@ -652,8 +643,7 @@ class Case:
         self.batch.synchronize()
         with _torch().cuda.stream(self.batch._tstream):
             ar, cur = self.arena, self.batch.cur
-            rho = ar.rhos[cur].index_select(0, self.slot_dev)
-            u = ar.vels[cur].index_select(1, self.slot_dev).t().contiguous()
+            rho, u = ar.mom.gather(cur, self.slot_dev)
             out = rho.cpu().numpy() / 3.0, u.cpu().numpy()
         return out
 
@@ -665,8 +655,8 @@ class Case:
         A.sstart = ar.sstart.data_ptr()
         A.slot_of_cell, A.n_cells = self.slot32.data_ptr(), self.N
         A.f_ref, A.parity = f_ref.data_ptr(), cur
-        A.storage, A.vel, A.rho, A.n_slots = STORAGES[self.batch.storage], ar.vels[cur].data_ptr(), \
-            ar.rhos[cur].data_ptr(), ar.S
+        A.vel, A.rho = ar.mom.convert_ptrs(cur)
+        A.storage, A.n_slots = STORAGES[self.batch.storage], ar.S
         _native.call(name, ctypes.byref(A), self.batch.stream_handle)
 
     def get_f(self):
@@ -692,7 +682,7 @@ class Case:
         self.batch.synchronize()
         with torch.cuda.stream(self.batch._tstream):
             uu = torch.from_numpy(np.ascontiguousarray(u, np.float32)).to(self.batch.dev)
-            self.arena.vels[self.batch.cur][:, self.slot_dev] = uu.t()
+            self.arena.mom.set_velocity(self.batch.cur, self.slot_dev, uu)
         self.batch.synchronize()
         if f is not None:
             self.set_f(f)
@@ -700,7 +690,7 @@ class Case:
     def ok(self):
         """All densities of the case finite (device reduction)."""
         ar, b0, nb = self.arena, self.b0, int(self.lay["n_bricks"])
-        x = ar.rhos[self.batch.cur][b0 * BC:(b0 + nb) * BC]
+        x = ar.mom.flat(self.batch.cur, b0 * BC, (b0 + nb) * BC)
         _native.call("cs_reduce_finite", ctypes.c_void_p(x.data_ptr()), x.numel(),
                      ctypes.c_void_p(self.batch._work.data_ptr()), ctypes.c_void_p(self.batch._red.data_ptr()),
                      self.batch.stream_handle)

@@ -57,8 +57,8 @@ __global__ __launch_bounds__(256) void k_convert(cs_bricks_convert_args a) {
     T* F = reinterpret_cast<T*>(a.f);
     float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if constexpr (H16) {
-        const float u0 = a.vel[s], u1 = a.vel[a.n_slots + s], u2 = a.vel[2 * a.n_slots + s];
-        m = make_float4(a.rho[s] - 1.5f * (u0 * u0 + u1 * u1 + u2 * u2), u0, u1, u2);
+        m = reinterpret_cast<const float4*>(a.vel)[s];
+        m.x = m.x - 1.5f * (m.y * m.y + m.z * m.z + m.w * m.w);
     }
     static_for<0, Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void k_convert(cs_bricks_convert_args a) {
 
 extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
     CS_REQUIRE(a && a->f && a->nbr && a->list && a->bidx && a->om_cell && a->om_brick && a->p_out && a->drive &&
-                   a->vel_old && a->vel && a->rho,
+                   a->vel_old && a->vel,
                "cs_bricks_step: null argument");
     CS_REQUIRE(!a->boundary || (a->lmask && a->smask && a->qlink && a->qstart && a->uin && a->side && a->sstart),
                "cs_bricks_step: boundary bricks need lmask/smask/qlink/qstart/uin/side/sstart");
@@ -89,8 +89,8 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
     CS_REQUIRE(a->parity == 0 || a->parity == 1, "cs_bricks_step: parity %d", a->parity);
     CS_REQUIRE(a->storage == CS_STORAGE_FP32 || a->storage == CS_STORAGE_FP16, "cs_bricks_step: storage %d",
                a->storage);
-    CS_REQUIRE(a->storage == CS_STORAGE_FP32 || (a->rho_old && a->rho_old != a->rho),
-               "cs_bricks_step: FP16 storage needs the previous density plane rho_old (distinct from rho)");
+    CS_REQUIRE(a->vel_old != a->vel, "cs_bricks_step: vel_old and vel must be distinct buffers");
+    CS_REQUIRE(a->storage == CS_STORAGE_FP16 || a->rho, "cs_bricks_step: FP32 storage needs rho");
     CS_REQUIRE(a->n_slots > 0 && a->n_slots % cs::BC == 0, "cs_bricks_step: n_slots %lld", (long long)a->n_slots);
     CS_REQUIRE(a->n_slots * cs::Q < ((int64_t)1 << 32), "cs_bricks_step: %lld slots exceed 32-bit population offsets",
                (long long)a->n_slots);
@@ -117,8 +117,8 @@ static int convert(const cs_bricks_convert_args* a, void* stream, bool set) {
     CS_REQUIRE(a->parity == 0 || a->parity == 1, "cs_bricks convert: parity %d", a->parity);
     CS_REQUIRE(a->storage == CS_STORAGE_FP32 || a->storage == CS_STORAGE_FP16, "cs_bricks convert: storage %d",
                a->storage);
-    CS_REQUIRE(a->storage == CS_STORAGE_FP32 || (a->vel && a->rho && a->n_slots > 0),
-               "cs_bricks convert: FP16 storage needs the newest vel/rho planes and n_slots");
+    CS_REQUIRE(a->storage == CS_STORAGE_FP32 || (a->vel && a->n_slots > 0),
+               "cs_bricks convert: FP16 storage needs the newest moments (vel) and n_slots");
     if (a->n_cells == 0) return CS_OK;
     const unsigned grid = (unsigned)((a->n_cells + 255) / 256);
     hipStream_t st = cs::as_stream(stream);

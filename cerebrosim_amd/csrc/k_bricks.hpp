@@ -254,23 +254,24 @@ __device__ __forceinline__ float h16_decode(_Float16 h, const float4& m) {
     return Lat::w[I] * ((float)h * (1.0f / kH16Scale) + m.x + cu * (3.0f + 4.5f * cu));
 }
 
-// Stage the previous-step moments {drho - 1.5 |u|^2, ux, uy, uz} of the brick and its one-cell shell in LDS.
-__device__ __forceinline__ void load_halo(float4* halo, const int* tbl, const float* rho_old, const float* vel_old,
-                                          uint32_t S, int L) {
-    for (int e = L; e < kHalo; e += BC) {
-        const int hx = e % 6, hy = (e / 6) % 6, hz = e / 36;
-        const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
-                      9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
-        const int l = ((hx - 1) & 3) + 4 * ((hy - 1) & 3) + 16 * ((hz - 1) & 3);
-        const int nb = tbl[o];
-        float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (nb >= 0) {
-            const uint32_t sl = (uint32_t)nb * BC + l;
-            const float u0 = vel_old[sl], u1 = vel_old[S + sl], u2 = vel_old[2 * S + sl];
-            m = make_float4(rho_old[sl] - 1.5f * (u0 * u0 + u1 * u1 + u2 * u2), u0, u1, u2);
+// Stage the previous-step moments {drho - 1.5 |u|^2, ux, uy, uz} of the brick and its one-cell shell in LDS
+// (mom: float4 {drho, ux, uy, uz} per slot). Slots of absent neighbour bricks read as zero.
+__device__ __forceinline__ void load_halo(float4* halo, const int* tbl, const float4* __restrict__ mom, int L) {
+    static_for<0, (kHalo + BC - 1) / BC>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int e = k * BC + L;
+        if ((k + 1) * BC <= kHalo || e < kHalo) {
+            const int hx = e % 6, hy = (e / 6) % 6, hz = e / 36;
+            const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
+                          9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
+            const int l = ((hx - 1) & 3) + 4 * ((hy - 1) & 3) + 16 * ((hz - 1) & 3);
+            const int nb = tbl[o];
+            float4 m = mom[(uint32_t)(nb < 0 ? 0 : nb) * BC + l];
+            m.x = m.x - 1.5f * (m.y * m.y + m.z * m.z + m.w * m.w);
+            if (nb < 0) m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            halo[e] = m;
         }
-        halo[e] = m;
-    }
+    });
 }
 
 
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     T* stage = stage_all + (STAGED ? wave * Q * BC : 0);
     float4* halo = halo_all + (H16 ? wave * kHalo : 0);
     if constexpr (H16) {
-        load_halo(halo, tbl[wave], a.rho_old, a.vel_old, S, L);
+        load_halo(halo, tbl[wave], reinterpret_cast<const float4*>(a.vel_old), L);
         wave_sync();
     }
 
@@ -409,9 +410,16 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         float up[3] = {0.0f, 0.0f, 0.0f};
         float q16[6];  // BND + GAL: qc of links 1..6 (read by the gradient stencil), loaded with the link batch
         if constexpr (BND) {
-            up[0] = a.vel_old[slot];
-            up[1] = a.vel_old[S + slot];
-            up[2] = a.vel_old[2 * S + slot];
+            if constexpr (H16) {
+                const float4 mo = reinterpret_cast<const float4*>(a.vel_old)[slot];
+                up[0] = mo.y;
+                up[1] = mo.z;
+                up[2] = mo.w;
+            } else {
+                up[0] = a.vel_old[slot];
+                up[1] = a.vel_old[S + slot];
+                up[2] = a.vel_old[2 * S + slot];
+            }
             if ((mask & kAllLinks) != kAllLinks) {
                 const float upsq = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
                 float uind[3] = {0.0f, 0.0f, 0.0f};
@@ -542,10 +550,14 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         }
 
         // ---- outputs -------------------------------------------------------------------------------------------
-        a.vel[slot] = u[0];
-        a.vel[S + slot] = u[1];
-        a.vel[2 * S + slot] = u[2];
-        a.rho[slot] = dr;
+        if constexpr (H16) {
+            reinterpret_cast<float4*>(a.vel)[slot] = make_float4(dr, u[0], u[1], u[2]);
+        } else {
+            a.vel[slot] = u[0];
+            a.vel[S + slot] = u[1];
+            a.vel[2 * S + slot] = u[2];
+            a.rho[slot] = dr;
+        }
         if constexpr (BND) {
             const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
             if (sm) {

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kRedThreads) void k_gather(cs_gather_args a) {
     for (int k = 0; k < a.n_probe; ++k) {
         float acc = 0.0f;
         for (int64_t t = threadIdx.x; t < a.m; t += kRedThreads)
-            if (a.slot[t] == k) acc += a.rho[a.idx[t]] / 3.0f;
+            if (a.slot[t] == k) acc += a.rho[(int64_t)a.idx[t] * (a.rho_elem > 1 ? a.rho_elem : 1)] / 3.0f;
         const float tot = block_sum(acc, lds);
         if (threadIdx.x == 0) a.out[k] = tot;
     }

@@ -70,6 +70,12 @@ class _IndexedBackend:
     def density(self, cur):
         return self.rho
 
+    def density_stride(self):
+        return 1
+
+    def finite_view(self, cur):
+        return self.rho
+
     def fields(self, cur):
         return self.rho.cpu().numpy(), self.vels[cur].cpu().numpy()
 
@@ -112,11 +118,80 @@ class _IndexedBackend:
         self.vels[cur].copy_(_torch().from_numpy(np.ascontiguousarray(u, np.float32)))
 
 
+class BrickMoments:
+    """Per-slot moments of a brick storage (S slots), double-buffered by step parity.
+
+    FP32 storage: velocity as three planes (3, S) per parity and one density plane (the reference's separate vel /
+    rho arrays). FP16 storage: one float4 {drho, ux, uy, uz} per slot (S, 4) per parity -- the decoding of a
+    population needs the writer's previous density and velocity together, so one 16-byte load fetches them."""
+
+    def __init__(self, S, fp16, dev):
+        torch = _torch()
+        self.S, self.fp16 = int(S), bool(fp16)
+        if self.fp16:
+            self.m = [torch.zeros((self.S, 4), dtype=torch.float32, device=dev) for _ in range(2)]
+        else:
+            self.v = [torch.zeros((3, self.S), dtype=torch.float32, device=dev) for _ in range(2)]
+            self.r = torch.zeros(self.S, dtype=torch.float32, device=dev)
+
+    def tensors(self):
+        return list(self.m) if self.fp16 else list(self.v) + [self.r]
+
+    def step_ptrs(self, a):
+        """(vel_old, vel, rho) of a step of parity a."""
+        if self.fp16:
+            return self.m[a].data_ptr(), self.m[1 - a].data_ptr(), 0
+        return self.v[a].data_ptr(), self.v[1 - a].data_ptr(), self.r.data_ptr()
+
+    def convert_ptrs(self, cur):
+        """(vel, rho) of the newest moments, for the conversions."""
+        if self.fp16:
+            return self.m[cur].data_ptr(), 0
+        return self.v[cur].data_ptr(), self.r.data_ptr()
+
+    def velocity(self, cur):
+        return self.m[cur][:, 1:] if self.fp16 else self.v[cur]
+
+    def velocity_addressing(self):
+        """(element stride, component stride) in floats from velocity(cur).data_ptr()."""
+        return (4, 1) if self.fp16 else (1, self.S)
+
+    def density(self, cur):
+        return self.m[cur][:, 0] if self.fp16 else self.r
+
+    def density_stride(self):
+        return 4 if self.fp16 else 1
+
+    def flat(self, cur, lo=0, hi=None):
+        """Contiguous float view of the newest moments of slots [lo, hi) whose finiteness ok() checks."""
+        hi = self.S if hi is None else hi
+        return self.m[cur][lo:hi].reshape(-1) if self.fp16 else self.r[lo:hi]
+
+    def gather(self, cur, idx):
+        """(drho (n,), u (n, 3)) at slots idx (device int64)."""
+        if self.fp16:
+            m = self.m[cur].index_select(0, idx)
+            return m[:, 0].contiguous(), m[:, 1:].contiguous()
+        return self.r.index_select(0, idx), self.v[cur].index_select(1, idx).t().contiguous()
+
+    def set_velocity(self, cur, idx, u):
+        if self.fp16:
+            self.m[cur][idx, 1:] = u
+        else:
+            self.v[cur][:, idx] = u.t()
+
+    def zero(self, lo, hi):
+        for t in self.tensors():
+            if t.dim() == 2 and not self.fp16:
+                t[:, lo:hi].zero_()
+            else:
+                t[lo:hi].zero_()
+
+
 class _BricksBackend:
     """4^3 brick storage with AA in-place streaming (see csrc/k_bricks.hip): populations [bricks][27][64] once,
-    velocity as three planes of n_slots floats (double-buffered), density one plane (two with FP16 storage, whose
-    decoding reads the previous step's density of the neighbours). Two launches per step: the interior bricks (no
-    boundary data) and the boundary bricks.
+    moments per slot (BrickMoments). Two launches per step: the interior bricks (no boundary data) and the boundary
+    bricks.
 
     storage "fp32": populations stored as float. "fp16": stored as float16 deviations from the writing cell's
     second-order equilibrium (scaled by 1024 / w_i); the arithmetic stays float32, the side buffer float32."""
@@ -147,9 +222,7 @@ class _BricksBackend:
         self.side = torch.zeros(max(1, layout["n_side"]), dtype=torch.float32, device=dev)
         self.slot_of_cell = up(layout["slot_of_cell"].astype(np.int64))
         self.slot_of_cell32 = up(layout["slot_of_cell"])
-        self.vels = [torch.zeros((3, S), dtype=torch.float32, device=dev) for _ in range(2)]
-        r = torch.zeros(S, dtype=torch.float32, device=dev)
-        self.rhos = [r, torch.zeros_like(r)] if self.storage == _native.CS_STORAGE_FP16 else [r, r]
+        self.mom = BrickMoments(S, self.storage == _native.CS_STORAGE_FP16, dev)
         self.omega = float(np.float32(1.0 / s.tau))
         self.om = torch.full((S,), self.omega, dtype=torch.float32, device=dev)
         self.om_brick = torch.zeros(self.nb, dtype=torch.uint8, device=dev)
@@ -172,8 +245,7 @@ class _BricksBackend:
         A.side, A.sstart = self.side.data_ptr(), self.sstart.data_ptr()
         A.om_cell, A.om_brick, A.omega = self.om.data_ptr(), self.om_brick.data_ptr(), self.omega
         A.p_out, A.drive = s.p_out.data_ptr(), s.drv.data_ptr()
-        A.vel_old, A.vel = self.vels[a].data_ptr(), self.vels[1 - a].data_ptr()
-        A.rho, A.rho_old = self.rhos[1 - a].data_ptr(), self.rhos[a].data_ptr()
+        A.vel_old, A.vel, A.rho = self.mom.step_ptrs(a)
         A.n_slots, A.storage = self.S, self.storage
         A.force = (ctypes.c_float * 3)(*s.force)
         A.magic = s.magic
@@ -204,8 +276,8 @@ class _BricksBackend:
         A.slot_of_cell, A.n_cells = self.slot_of_cell32.data_ptr(), self.n_cells
         A.f_ref, A.parity = f_ref.data_ptr(), parity
         # the newest moments: those of the step that left this state (parity = the next step's parity)
-        A.storage, A.vel, A.rho, A.n_slots = self.storage, self.vels[parity].data_ptr(), \
-            self.rhos[parity].data_ptr(), self.S
+        A.vel, A.rho = self.mom.convert_ptrs(parity)
+        A.storage, A.n_slots = self.storage, self.S
         _native.call(name, ctypes.byref(A), s.stream_handle)
 
     def get_f(self, s, cur):
@@ -220,29 +292,33 @@ class _BricksBackend:
         s.synchronize()
 
     def velocity(self, cur):
-        return self.vels[cur]
+        return self.mom.velocity(cur)
 
     def velocity_addressing(self):
-        return 1, self.S
+        return self.mom.velocity_addressing()
 
     def slots(self, cells):
         return self.L["slot_of_cell"][np.asarray(cells)].astype(np.int32)
 
     def density(self, cur):
-        return self.rhos[cur]
+        return self.mom.density(cur)
+
+    def density_stride(self):
+        return self.mom.density_stride()
+
+    def finite_view(self, cur):
+        return self.mom.flat(cur)
 
     def fields(self, cur):
-        idx = self.slot_of_cell
-        rho = self.rhos[cur].index_select(0, idx)
-        u = self.vels[cur].index_select(1, idx).t()
-        return rho.cpu().numpy(), u.contiguous().cpu().numpy()
+        rho, u = self.mom.gather(cur, self.slot_of_cell)
+        return rho.cpu().numpy(), u.cpu().numpy()
 
     def set_velocity(self, cur, u, s=None):
         torch = _torch()
         # FP16: the stored deviations are relative to these moments -- re-encode the populations around the new ones
         f = self.get_f(s, cur) if self.storage == _native.CS_STORAGE_FP16 else None
-        u = torch.from_numpy(np.ascontiguousarray(u, np.float32)).to(self.vels[cur].device)
-        self.vels[cur][:, self.slot_of_cell] = u.t()
+        u = torch.from_numpy(np.ascontiguousarray(u, np.float32)).to(self.f.device)
+        self.mom.set_velocity(cur, self.slot_of_cell, u)
         if f is not None:
             self.set_f(s, cur, f)
 
@@ -435,7 +511,7 @@ class Solver:
 
     def ok(self):
         """All densities finite (device reduction; reads back 16 bytes)."""
-        x = self.backend.density(self.cur)
+        x = self.backend.finite_view(self.cur)
         _native.call("cs_reduce_finite", ctypes.c_void_p(x.data_ptr()), x.numel(),
                      ctypes.c_void_p(self._work.data_ptr()), ctypes.c_void_p(self._red.data_ptr()),
                      self.stream_handle)

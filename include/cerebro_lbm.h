@@ -192,10 +192,11 @@ typedef struct cs_bricks_step_args {
     float omega;                /* 1 / tau of every other cell */
     const float* p_out;
     const float* drive;
-    const float* vel_old;       /* [3][n_slots] */
-    float* vel;                 /* [3][n_slots] */
-    float* rho;                 /* [n_slots] */
-    const float* rho_old;       /* [n_slots] density of the previous step (FP16 storage only, else NULL) */
+    const float* vel_old;       /* FP32 storage: [3][n_slots] velocity planes of the previous step; FP16 storage:
+                                   [n_slots][4] float4 {drho, ux, uy, uz} of the previous step */
+    float* vel;                 /* the same for this step's output */
+    float* rho;                 /* FP32 storage: [n_slots] density deviation; FP16 storage: unused (in vel) */
+    const float* rho_old;       /* unused (NULL) */
     int64_t n_slots;
     float force[3];
     float magic;
@@ -233,8 +234,8 @@ typedef struct cs_bricks_convert_args {
     float* f_ref;               /* device (Q, N) */
     int32_t parity;             /* parity of the next step */
     int32_t storage;            /* CS_STORAGE_* */
-    const float* vel;           /* FP16 storage: newest velocity planes [3][n_slots] (the cells' own moments) */
-    const float* rho;           /* FP16 storage: newest density plane [n_slots] */
+    const float* vel;           /* FP16 storage: newest moments [n_slots][4] {drho, ux, uy, uz} (the cells' own) */
+    const float* rho;           /* unused */
     int64_t n_slots;
 } cs_bricks_convert_args;
 int cs_bricks_get_f(const cs_bricks_convert_args* a, void* stream);
@@ -303,6 +304,7 @@ typedef struct cs_gather_args {
     float* out;              /* device, n_probe sums */
     float* umax;             /* device, 1 float */
     int32_t n_probe;         /* at most 64 */
+    int64_t rho_elem;        /* density of slot s at rho[s * rho_elem] (0 or 1: a dense plane) */
 } cs_gather_args;
 int cs_gather(const cs_gather_args* a, void* stream);
 
