@@ -145,11 +145,16 @@ struct CollideParams {
 };
 
 // Regularised BGK (lbm.py:217-240) or TRT (lbm.py:241-255) of one cell, in place. Returns dr and u (= j + F/2).
+// uset: use the velocity ut instead of j + F/2. With om = 1, F = 0 and g = 0 the regularised collision is then
+// exactly the inlet equilibrium of lbm.py:205-216 (the same expressions, each added term an exact zero), so inlet
+// cells share this code with their neighbours instead of diverging into a second path.
 template <int MODEL, bool HYB>
-__device__ __forceinline__ void collide(float (&f)[Q], const CollideParams& p, float& dr_out, float (&u)[3]) {
+__device__ __forceinline__ void collide(float (&f)[Q], const CollideParams& p, float& dr_out, float (&u)[3],
+                                        bool uset = false, float ut0 = 0.0f, float ut1 = 0.0f, float ut2 = 0.0f) {
     RawMoments m = raw_moments(f);
     const float F0 = p.F[0], F1 = p.F[1], F2 = p.F[2];
-    const float u0 = m.jx + F0 * 0.5f, u1 = m.jy + F1 * 0.5f, u2 = m.jz + F2 * 0.5f;
+    const float u0 = uset ? ut0 : m.jx + F0 * 0.5f, u1 = uset ? ut1 : m.jy + F1 * 0.5f,
+                u2 = uset ? ut2 : m.jz + F2 * 0.5f;
     const float usq = u0 * u0 + u1 * u1 + u2 * u2;
     const float uF = u0 * F0 + u1 * F1 + u2 * F2;
     const float om = p.om;

@@ -539,7 +539,19 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         float dr, u[3];
         bool inlet_cell = false;
         if constexpr (BND) inlet_cell = (mask & kInletBit) != 0u;
-        if (inlet_cell) {
+        if constexpr (MODEL == 1 && BND) {
+            // inlet cells: the regularised collision at the prescribed velocity with om = 1, F = 0, g = 0
+            float ut[3] = {0.0f, 0.0f, 0.0f};
+            if (inlet_cell) {
+                ut[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
+                ut[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
+                ut[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
+                p.om = 1.0f;
+                p.F[0] = p.F[1] = p.F[2] = 0.0f;
+                p.g[0] = p.g[1] = p.g[2] = 0.0f;
+            }
+            collide<MODEL, false>(f, p, dr, u, inlet_cell, ut[0], ut[1], ut[2]);
+        } else if (inlet_cell) {
             dr = density_deviation(f);
             u[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
             u[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
