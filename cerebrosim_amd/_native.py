@@ -37,6 +37,12 @@ class ClockCasesArgs(ctypes.Structure):
                 ("drive", c_void_p), ("n_cases", c_i32), ("coef_stride", c_i32)]
 
 
+class ClockTableArgs(ctypes.Structure):
+    _fields_ = [("tick", c_void_p), ("coef", c_void_p), ("nh", c_void_p), ("period", c_void_p), ("ramp", c_void_p),
+                ("drive", c_void_p), ("n_cases", c_i32), ("coef_stride", c_i32), ("n_steps", c_i32),
+                ("drive_stride", c_i32)]
+
+
 class FluxCasesArgs(ctypes.Structure):
     _fields_ = [("idx", c_void_p), ("w", c_void_p), ("slot", c_void_p), ("start", c_void_p), ("vel", c_void_p),
                 ("vel_elem", c_i64), ("vel_comp", c_i64), ("R", c_void_p), ("drive", c_void_p), ("p_out", c_void_p),
@@ -106,6 +112,7 @@ SIGNATURES = {
     "cs_step_indexed": [ctypes.POINTER(StepArgs), c_void_p],
     "cs_flux_bc": [ctypes.POINTER(FluxArgs), c_void_p],
     "cs_clock_cases": [ctypes.POINTER(ClockCasesArgs), c_void_p],
+    "cs_clock_table": [ctypes.POINTER(ClockTableArgs), c_void_p],
     "cs_flux_cases": [ctypes.POINTER(FluxCasesArgs), c_void_p],
     "cs_bricks_variant_info": [c_i32, c_void_p, c_void_p],
     "cs_gather": [ctypes.POINTER(GatherArgs), c_void_p],

@@ -135,7 +135,7 @@ class ArterySolver(Solver):
         A = _native.FluxArgs()
         A.idx, A.w, A.slot, A.m = self.f_idx.data_ptr(), self.f_w.data_ptr(), self.f_slot.data_ptr(), self.f_idx.numel()
         A.vel, A.vel_elem, A.vel_comp = self.backend.velocity(1 - a).data_ptr(), elem, comp
-        A.R, A.drive, A.p_out, A.q_last = self.R.data_ptr(), self.drv.data_ptr(), self.p_out.data_ptr(), \
+        A.R, A.drive, A.p_out, A.q_last = self.R.data_ptr(), self._drive_ptr(), self.p_out.data_ptr(), \
             self.Qlast.data_ptr()
         A.n_open, A.pref, A.alpha = self.n_open, self.pref, self.alpha
         _native.call("cs_flux_bc", ctypes.byref(A), self.stream_handle)

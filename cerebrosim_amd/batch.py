@@ -245,7 +245,8 @@ class Batch:
             self.nh = z(M, torch.int32)
             self.period = z(M, torch.int32)
             self.ramp = z(M, torch.int32)
-            self.drive = z(M)
+            self.drive = z((256, M))           # drives of the steps of a chunk, per case
+            self._k = 0
             self.omega = z(M)
             self.force = z((M, 3))
             self.p_out = z((M, K))
@@ -355,12 +356,32 @@ class Batch:
         self.graphs = {}
 
     # --- stepping -------------------------------------------------------------------------------------------
+    def _drive_ptr(self):
+        """Device address of the per-case drives of the step being launched (table row self._k)."""
+        return self.drive.data_ptr() + 4 * self.max_cases * self._k
+
+    def _clock_table(self, n):
+        """Every case's clock for the next n steps at once; rows 0..n-1 of the drive table."""
+        A = _native.ClockTableArgs()
+        A.tick, A.coef, A.nh = self.tick.data_ptr(), self.coef.data_ptr(), self.nh.data_ptr()
+        A.period, A.ramp, A.drive = self.period.data_ptr(), self.ramp.data_ptr(), self.drive.data_ptr()
+        A.n_cases, A.coef_stride, A.n_steps, A.drive_stride = self.max_cases, self.coef.shape[1], int(n), \
+            self.max_cases
+        _native.call("cs_clock_table", ctypes.byref(A), self.stream_handle)
+
     def _clock(self):
-        a = _native.ClockCasesArgs()
-        a.tick, a.coef, a.nh = self.tick.data_ptr(), self.coef.data_ptr(), self.nh.data_ptr()
-        a.period, a.ramp, a.drive = self.period.data_ptr(), self.ramp.data_ptr(), self.drive.data_ptr()
-        a.n_cases, a.coef_stride = self.max_cases, self.coef.shape[1]
-        _native.call("cs_clock_cases", ctypes.byref(a), self.stream_handle)
+        """The clocks of one step (eager launches)."""
+        self._clock_table(1)
+        self._k = 0
+
+    def _grow_table(self, n):
+        if n > self.drive.shape[0]:
+            torch = _torch()
+            self.synchronize()
+            self.graphs = {}
+            with torch.cuda.stream(self._tstream):
+                self.drive = torch.zeros((n, self.max_cases), dtype=torch.float32, device=self.dev)
+            self.synchronize()
 
     def step_args(self, arena, a, boundary):
         A = _native.BricksStepArgs()
@@ -371,7 +392,7 @@ class Batch:
         A.qlink, A.qstart, A.uin = arena.qlink.data_ptr(), arena.qstart.data_ptr(), arena.uin.data_ptr()
         A.side, A.sstart = arena.side.data_ptr(), arena.sstart.data_ptr()
         A.om_cell, A.om_brick, A.omega = arena.om.data_ptr(), arena.om_brick.data_ptr(), 1.0
-        A.p_out, A.drive = self.p_out.data_ptr(), self.drive.data_ptr()
+        A.p_out, A.drive = self.p_out.data_ptr(), self._drive_ptr()
         A.vel_old, A.vel, A.rho = arena.mom.step_ptrs(a)
         A.n_slots, A.storage = arena.S, STORAGES[self.storage]
         A.magic, A.bouzidi, A.model, A.galilean = self.magic, self.bouzidi, self.model, self.gal
@@ -387,7 +408,7 @@ class Batch:
             fx["start"].data_ptr()
         A.vel = arena.mom.velocity(1 - a).data_ptr()
         A.vel_elem, A.vel_comp = arena.mom.velocity_addressing()
-        A.R, A.drive, A.p_out, A.q_last = self.R.data_ptr(), self.drive.data_ptr(), self.p_out.data_ptr(), \
+        A.R, A.drive, A.p_out, A.q_last = self.R.data_ptr(), self._drive_ptr(), self.p_out.data_ptr(), \
             self.q_last.data_ptr()
         A.n_open, A.pref, A.alpha = arena.n_open.data_ptr(), self.pref.data_ptr(), self.alpha.data_ptr()
         A.n_cases, A.open_stride = self.max_cases, self.max_open
@@ -409,7 +430,7 @@ class Batch:
             _native.call("cs_bricks_step", ctypes.byref(A), self.stream_handle)
 
     def _launch(self, a):
-        self._clock()
+        """The launches of one step of parity a (drives at table row self._k)."""
         for ar in self.arenas:
             for boundary in (False, True):
                 A = self.step_args(ar, a, boundary)
@@ -425,18 +446,23 @@ class Batch:
         chunk += chunk % 2
         full, rest = divmod(int(steps), chunk)
         if full:
+            self._grow_table(chunk)
             key = (self.cur, chunk, self.variant)
             if key not in self.graphs:
                 with _native.capture(self.stream_handle) as cap:
+                    self._clock_table(chunk)
                     a = self.cur
-                    for _ in range(chunk):
+                    for k in range(chunk):
+                        self._k = k
                         self._launch(a)
                         a = 1 - a
                 self.graphs[key] = cap.graph
             g = self.graphs[key]
             for _ in range(full):
                 g.launch(self.stream_handle)
+            self._k = chunk - 1
         for _ in range(rest):
+            self._clock()
             self._launch(self.cur)
             self.cur = 1 - self.cur
         self.steps += int(steps)
@@ -474,6 +500,7 @@ class Batch:
                     e0.record(self._tstream)
                     a = self.cur
                     for _ in range(steps):
+                        self._clock()
                         self._launch(a)
                         a = 1 - a
                     e1.record(self._tstream)
@@ -549,7 +576,7 @@ class Case:
                     arena.side[s0:s0 + int(lay["n_side"])].zero_()
             c = cid
             batch.tick[c] = 0
-            batch.drive[c] = 0.0
+            batch.drive[:, c] = 0.0
             batch.omega[c] = float(np.float32(1.0 / self.tau))
             batch.force[c].zero_()
             batch.p_out[c].zero_()

@@ -10,16 +10,33 @@ namespace cs {
 namespace {
 
 // ---- k_clock ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void clock_tick(int32_t* tick, const float* coef, int nh, int period, int ramp,
-                                           float* drive) {
-    const int t = tick[0];
+__device__ __forceinline__ float clock_value(int t, const float* coef, int nh, int period, int ramp) {
     const float pi = 3.14159265358979323846f;
     const float ph = 2.0f * pi * (float)(t % period) / (float)period;
     float d = coef[0];
     for (int k = 1; k <= nh; ++k) d += coef[2 * k - 1] * cosf((float)k * ph) + coef[2 * k] * sinf((float)k * ph);
     if (t < ramp) d *= 0.5f - 0.5f * cosf(pi * (float)t / (float)ramp);
-    drive[0] = d;
+    return d;
+}
+
+__device__ __forceinline__ void clock_tick(int32_t* tick, const float* coef, int nh, int period, int ramp,
+                                           float* drive) {
+    const int t = tick[0];
+    drive[0] = clock_value(t, coef, nh, period, ramp);
     tick[0] = t + 1;
+}
+
+// The drives of n_steps consecutive steps of one case per workgroup: drive[k * stride + c] = k_clock at tick + k,
+// then tick += n_steps (read by every thread before the barrier, advanced after it).
+__global__ __launch_bounds__(256) void k_clock_table(cs_clock_table_args a) {
+    const int c = blockIdx.x;
+    if (a.period[c] <= 0) return;  // workgroup-uniform
+    const int t0 = a.tick[c];
+    __syncthreads();
+    const float* coef = a.coef + (int64_t)c * a.coef_stride;
+    for (int k = threadIdx.x; k < a.n_steps; k += 256)
+        a.drive[(int64_t)k * a.drive_stride + c] = clock_value(t0 + k, coef, a.nh[c], a.period[c], a.ramp[c]);
+    if (threadIdx.x == 0) a.tick[c] = t0 + a.n_steps;
 }
 
 __global__ void k_clock(cs_clock_args a) {
@@ -220,6 +237,18 @@ extern "C" int cs_flux_bc(const cs_flux_args* a, void* stream) {
     CS_REQUIRE(a->m == 0 || (a->idx && a->w && a->slot && a->vel), "cs_flux_bc: null cell list");
     cs::k_flux_bc<<<1, cs::kRedThreads, 0, cs::as_stream(stream)>>>(*a);
     CS_LAUNCHED("k_flux_bc");
+    return CS_OK;
+}
+
+extern "C" int cs_clock_table(const cs_clock_table_args* a, void* stream) {
+    CS_REQUIRE(a && a->tick && a->coef && a->nh && a->period && a->ramp && a->drive, "cs_clock_table: null argument");
+    CS_REQUIRE(a->n_cases >= 0 && a->n_cases <= CS_MAX_CASES && a->coef_stride >= 1 && a->n_steps >= 0 &&
+                   a->drive_stride >= a->n_cases,
+               "cs_clock_table: n_cases %d / coef_stride %d / n_steps %d / drive_stride %d", a->n_cases,
+               a->coef_stride, a->n_steps, a->drive_stride);
+    if (a->n_cases == 0 || a->n_steps == 0) return CS_OK;
+    cs::k_clock_table<<<a->n_cases, 256, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_clock_table");
     return CS_OK;
 }
 

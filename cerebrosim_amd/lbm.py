@@ -90,7 +90,7 @@ class _IndexedBackend:
         args.f_in, args.f_out = self.f[a].data_ptr(), self.f[1 - a].data_ptr()
         args.nbr, args.qw, args.uin, args.inl = (self.nbr.data_ptr(), self.q.data_ptr(), self.uin.data_ptr(),
                                                  self.inl.data_ptr())
-        args.p_out, args.drive, args.om_cell = s.p_out.data_ptr(), s.drv.data_ptr(), self.om.data_ptr()
+        args.p_out, args.drive, args.om_cell = s.p_out.data_ptr(), s._drive_ptr(), self.om.data_ptr()
         args.vel_old, args.vel, args.rho = self.vels[a].data_ptr(), self.vels[1 - a].data_ptr(), self.rho.data_ptr()
         args.n_cells = s.N
         args.force = (ctypes.c_float * 3)(*s.force)
@@ -244,7 +244,7 @@ class _BricksBackend:
         A.qlink, A.qstart, A.uin = self.qlink.data_ptr(), self.qstart.data_ptr(), self.uin.data_ptr()
         A.side, A.sstart = self.side.data_ptr(), self.sstart.data_ptr()
         A.om_cell, A.om_brick, A.omega = self.om.data_ptr(), self.om_brick.data_ptr(), self.omega
-        A.p_out, A.drive = s.p_out.data_ptr(), s.drv.data_ptr()
+        A.p_out, A.drive = s.p_out.data_ptr(), s._drive_ptr()
         A.vel_old, A.vel, A.rho = self.mom.step_ptrs(a)
         A.n_slots, A.storage = self.S, self.storage
         A.force = (ctypes.c_float * 3)(*s.force)
@@ -372,7 +372,9 @@ class Solver:
         with torch.cuda.stream(self._tstream):
             self.p_out = torch.zeros(max(1, lat["n_outlets"]), dtype=torch.float32, device=dev)
             self.tick = torch.zeros(1, dtype=torch.int32, device=dev)
-            self.drv = torch.zeros(1, dtype=torch.float32, device=dev)
+            self._clk = torch.zeros(3, dtype=torch.int32, device=dev)          # nh, period, ramp
+            self._dtab = torch.zeros(256, dtype=torch.float32, device=dev)     # drives of the steps of a chunk
+            self._k = 0
             self._work = torch.zeros(_native.CS_REDUCE_WORK, dtype=torch.float64, device=dev)
             self._red = torch.zeros(2, dtype=torch.float64, device=dev)
             if storage not in ("fp32", "fp16"):
@@ -444,6 +446,9 @@ class Solver:
         self._invalidate()
         self.coef = torch.from_numpy(coef).to(self.dev)
         self.nh, self.period, self.ramp = (len(coef) - 1) // 2, int(period), int(ramp)
+        with torch.cuda.stream(self._tstream):
+            self._clk.copy_(torch.tensor([self.nh, self.period, self.ramp], dtype=torch.int32))
+        self._tstream.synchronize()
 
     def set_force(self, F):
         self.force = tuple(float(np.float32(x)) for x in F)
@@ -468,13 +473,37 @@ class Solver:
         self._host_to_device(self.p_out, np.asarray(p_lu, np.float32).reshape(-1))
 
     # --- stepping (lbm.py:422-448)
+    @property
+    def drv(self):
+        """Drive of the most recent step (1-element device tensor; lbm.py:379)."""
+        return self._dtab[self._k:self._k + 1]
+
+    def _drive_ptr(self):
+        """Device address of the drive the step being launched reads."""
+        return self._dtab.data_ptr() + 4 * self._k
+
+    def _clock_table(self, n):
+        """k_clock for the next n steps at once (lbm.py:90-102): drives into the table, tick += n."""
+        if n > self._dtab.numel():
+            torch = _torch()
+            self._invalidate()
+            with torch.cuda.stream(self._tstream):
+                self._dtab = torch.zeros(n, dtype=torch.float32, device=self.dev)
+        A = _native.ClockTableArgs()
+        A.tick, A.coef, A.drive = self.tick.data_ptr(), self.coef.data_ptr(), self._dtab.data_ptr()
+        base = self._clk.data_ptr()
+        A.nh, A.period, A.ramp = base, base + 4, base + 8
+        A.n_cases, A.coef_stride, A.n_steps, A.drive_stride = 1, self.coef.numel(), int(n), 1
+        _native.call("cs_clock_table", ctypes.byref(A), self.stream_handle)
+
     def _clock(self):
-        a = _native.ClockArgs(self.tick.data_ptr(), self.coef.data_ptr(), self.nh, self.period, self.ramp,
-                              self.drv.data_ptr())
-        _native.call("cs_clock", ctypes.byref(a), self.stream_handle)
+        """The clock of one step (eager launches)."""
+        self._clock_table(1)
+        self._k = 0
 
     def _launch(self, a):
-        self._clock()
+        """The launches of one step of parity a, reading the drive at table slot self._k (the clock runs once per
+        chunk, before the chunk's steps)."""
         self.backend.step(self, a)
 
     def run(self, steps, chunk=200):
@@ -482,18 +511,27 @@ class Solver:
         chunk += chunk % 2
         full, rest = divmod(int(steps), chunk)
         if full:
+            if chunk > self._dtab.numel():          # grow the table outside the capture
+                self._invalidate()
+                torch = _torch()
+                with torch.cuda.stream(self._tstream):
+                    self._dtab = torch.zeros(chunk, dtype=torch.float32, device=self.dev)
             key = (self.cur, chunk)
             if key not in self.graphs:
                 with _native.capture(self.stream_handle) as cap:
+                    self._clock_table(chunk)
                     a = self.cur
-                    for _ in range(chunk):
+                    for k in range(chunk):
+                        self._k = k
                         self._launch(a)
                         a = 1 - a
                 self.graphs[key] = cap.graph
             g = self.graphs[key]
             for _ in range(full):
                 g.launch(self.stream_handle)
+            self._k = chunk - 1
         for _ in range(rest):
+            self._clock()
             self._launch(self.cur)
             self.cur = 1 - self.cur
         self.steps += int(steps)

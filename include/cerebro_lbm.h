@@ -82,6 +82,23 @@ typedef struct cs_clock_cases_args {
 } cs_clock_cases_args;
 int cs_clock_cases(const cs_clock_cases_args* a, void* stream);
 
+/* The clocks of n_steps consecutive steps at once (graphs launch it once per chunk instead of a clock per step):
+ * for each case c < n_cases with period[c] > 0, drive[k * drive_stride + c] = k_clock's value at tick[c] + k for
+ * k < n_steps, then tick[c] += n_steps. Same arithmetic as cs_clock. */
+typedef struct cs_clock_table_args {
+    int32_t* tick;              /* [n_cases] */
+    const float* coef;          /* [n_cases][coef_stride] */
+    const int32_t* nh;          /* [n_cases] */
+    const int32_t* period;      /* [n_cases]; 0 = unused case */
+    const int32_t* ramp;        /* [n_cases] */
+    float* drive;               /* [n_steps][drive_stride] */
+    int32_t n_cases;            /* at most CS_MAX_CASES */
+    int32_t coef_stride;
+    int32_t n_steps;
+    int32_t drive_stride;       /* >= n_cases */
+} cs_clock_table_args;
+int cs_clock_table(const cs_clock_table_args* a, void* stream);
+
 /* ---------------------------------------------------------------------------------------------------------------
  * Solver step, indexed layout -- replaces k_step (lbm.py:140-257) launched at lbm.py:425 with dim = N.
  * Same arrays as the reference: f_in/f_out (Q, N) row-major, nbr/qw (Q, N), uin/vel_old/vel (N, 3) float AoS,
