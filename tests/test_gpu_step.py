@@ -162,6 +162,38 @@ def test_poiseuille_d20_published(layout, nu_healthy):
     assert abs(fl - -0.03803448455849523) < 2e-4
 
 
+@pytest.mark.parametrize("layout,storage", [("indexed", "fp32"), ("bricks", "fp32"), ("bricks", "fp16")])
+def test_womersley_wo56_published(layout, storage, nu_healthy):
+    """Oscillating body force in the D = 20 periodic pipe at Wo = 5.6 (validate.py:92-117): relative L2 against the
+    analytic Bessel profile over the third cycle (12 phases), published 0.060357 (geometry/solver.json). Measured on
+    MI355X: 0.0603572 in FP32 (both layouts), 0.0603620 with FP16 storage."""
+    import math
+    from cerebrosim_amd.validation import womersley_exact
+    nu = nu_healthy
+    lat, r, R = periodic_pipe(20, nz=4)
+    Wo, phases = 5.6, 12
+    T = int(round(2 * math.pi * R * R / (Wo * Wo * nu) / phases)) * phases
+    w = 2 * math.pi / T
+    Wo = R * math.sqrt(w / nu)
+    rr = np.linspace(0, R, 50)
+    peak = max(np.abs(womersley_exact(rr, R, Wo, 1.0, w, t)).max() for t in np.linspace(0, T, 24))
+    F0 = 0.02 / peak
+    s = lbm.Solver(lat, nu, model="regularised", layout=layout, storage=storage)
+    s.set_force([0, 0, F0])
+    s.set_drive([0.0, 1.0, 0.0], period=T)
+    s.run(2 * T)
+    num = den = 0.0
+    for _ in range(phases):
+        s.run(T // phases)
+        _, u = s.fields()
+        ua = womersley_exact(r, R, Wo, F0, w, s.steps - 0.5)
+        num += ((u[:, 2] - ua) ** 2).sum()
+        den += (ua ** 2).sum()
+    e = math.sqrt(num / den)
+    print(f"{layout} {storage} womersley Wo 5.6: l2 {e}", flush=True)
+    assert abs(e / 0.060357378395948014 - 1) < (1e-4 if storage == "fp32" else 1e-3), e
+
+
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("name,axis,ref", [
     ("straight", (0, 0, 1), {"grad_err": 0.08768536633211488, "profile_l2": 0.01247263825422971,
