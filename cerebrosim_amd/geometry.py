@@ -101,3 +101,89 @@ def stenosis_pipe(dx, sev=0.7, D0_mm=2.0, Ls_mm=4.0, up_d=4, down_d=18):
 
     lat, sc, rc, _ = open_pipe((0, 0, 1), L, radius)
     return lat, sc, rc, D
+
+
+def _segment_sdf(P, A, axis, L, radius):
+    """Signed distance (approximate, exact on the lateral wall of a straight tube) of points P (n, 3) to a tube from A
+    along unit `axis`, length L, radius(s), flat caps."""
+    rel = P - A
+    s = rel @ axis
+    rad = np.linalg.norm(rel - s[:, None] * axis, axis=1)
+    return np.maximum(np.maximum(rad - radius(s), -s), s - L)
+
+
+def bifurcation(scale=1.0, pad=3):
+    """Stenosed bifurcation: parent tube along z of radius 60 s narrowed to 30 % area at z = 600 s (Gaussian,
+    sigma 40 s), splitting at z = 1100 s into branches of radius 45 s and 35 s at +-30 degrees in the y-z plane, in
+    a 240 s x 400 s x 1800 s box (s = scale). The branches end inside the box (length 300 s: at the full 1800 s they
+    would leave its 400 s width). Velocity inlet (parabolic profile) at the parent's start, pressure outlets at the
+    branch ends. The signed distance is evaluated slab by slab in z, so the full-size grid fits in host memory.
+
+    Returns (lattice, info) with info = {"segments", "sdf" (points -> signed distance), "R", "throat"}."""
+    s = float(scale)
+    n = np.array([int(round(240 * s)), int(round(400 * s)), int(round(1800 * s))])
+    c = np.array([n[0] / 2 - 0.5, n[1] / 2 - 0.5, 0.0]) + OFFSET * np.array([1, 1, 0])
+    z_in, z_j = float(pad), 1100.0 * s
+    R0, d = 60.0 * s, 1.0 - math.sqrt(0.3)
+    zs, sig = 600.0 * s, 40.0 * s
+
+    def parent_r(t):
+        z = np.asarray(t, float) + z_in
+        return R0 * (1.0 - d * np.exp(-0.5 * ((z - zs) / sig) ** 2))
+
+    ang = math.radians(30.0)
+    Lb = 300.0 * s
+    A0 = c + np.array([0.0, 0.0, z_in])
+    J = c + np.array([0.0, 0.0, z_j])
+    segs = [(A0, np.array([0.0, 0.0, 1.0]), z_j - z_in + 0.5 * R0, parent_r, R0)]
+    for sgn, Rb in ((1.0, 45.0 * s), (-1.0, 35.0 * s)):
+        ax = np.array([0.0, sgn * math.sin(ang), math.cos(ang)])
+        segs.append((J, ax, Lb, (lambda Rb: lambda t: np.full_like(np.asarray(t, float), Rb))(Rb), Rb))
+
+    def sdf(P):
+        P = np.asarray(P, float)
+        out = np.full(len(P), np.inf)
+        for A, ax, L, rf, _ in segs:
+            out = np.minimum(out, _segment_sdf(P, A, ax, L, rf))
+        return out
+
+    phi = np.empty(tuple(n), np.float32)
+    I, Jj = np.meshgrid(np.arange(n[0]), np.arange(n[1]), indexing="ij")
+    plane = np.stack([I.ravel(), Jj.ravel()], 1).astype(float)
+    for k in range(n[2]):
+        P = np.concatenate([plane, np.full((len(plane), 1), float(k))], 1)
+        phi[:, :, k] = sdf(P).reshape(n[0], n[1])
+    openings = [{"kind": "inlet", "centre": A0, "normal": np.array([0.0, 0.0, -1.0]), "radius": float(parent_r(0.0))}]
+    for A, ax, L, rf, Rb in segs[1:]:
+        openings.append({"kind": "outlet", "centre": A + ax * L, "normal": ax, "radius": Rb})
+    lat = build_lattice(phi, [0, 0, 0], 1.0, openings=openings)
+    info = {"segments": segs, "sdf": sdf, "R": R0, "throat": R0 * math.sqrt(0.3), "shape": n}
+    return lat, info
+
+
+def wall_points(info, spacing=2.0, end_margin=6.0):
+    """Points on the bifurcation's wall (spacing in cells) with unit normals into the lumen, away from the openings
+    and from the junction (points inside another segment are dropped). Returns (points (V, 3), normals (V, 3))."""
+    sdf = info["sdf"]
+    pts, nrm = [], []
+    for A, ax, L, rf, Rmax in info["segments"]:
+        e1 = np.cross(ax, [1.0, 0.0, 0.0])
+        if np.linalg.norm(e1) < 1e-6:
+            e1 = np.cross(ax, [0.0, 1.0, 0.0])
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(ax, e1)
+        t = np.arange(end_margin, L - end_margin, spacing)
+        for tt in t:
+            r = float(rf(np.array([tt]))[0])
+            dr = (float(rf(np.array([tt + 0.5]))[0]) - float(rf(np.array([tt - 0.5]))[0]))
+            th = np.arange(0.0, 2 * math.pi, spacing / max(r, 1.0))
+            radial = np.cos(th)[:, None] * e1 + np.sin(th)[:, None] * e2
+            P = A + tt * ax + r * radial
+            nvec = -radial + dr * ax
+            nvec /= np.linalg.norm(nvec, axis=1, keepdims=True)
+            pts.append(P)
+            nrm.append(nvec)
+    P = np.concatenate(pts)
+    Nn = np.concatenate(nrm)
+    keep = np.abs(sdf(P)) < 0.25                     # on the union's wall, not inside another segment
+    return P[keep], Nn[keep]

@@ -1,4 +1,7 @@
-"""Synthetic case sets for the batching and serving forms.
+"""Synthetic case sets for the precision, batching and serving forms.
+
+* ``bifurcation_case``: the precision-form workload -- the stenosed bifurcation of geometry.bifurcation with a
+  seeded 8-harmonic pulsatile inlet (mean 0.03, peak 0.08), tau 0.52; ``wall_shear_weights`` gives its WSS fit.
 
 * ``stenosed_tube``: straight tube along z with a Gaussian narrowing, a velocity inlet (the reference's parabolic
   profile, lbm.py:322-328) at z = 0 and a pressure outlet at z = L.
@@ -113,3 +116,46 @@ def request_case(spec, tau=0.55):
         s.set_sponge(D, 0.7, kinds=("outlet",))
 
     return lat, nu, setup
+
+
+def bifurcation_waveform(seed=0, harmonics=8, mean=0.03, peak=0.08):
+    """Fourier coefficients [a0, a1, b1, ...] of the precision-form inlet: harmonics drawn from default_rng(seed)
+    with a 1/k fall-off, scaled so that the waveform's mean is `mean` and its maximum `peak`."""
+    rng = np.random.default_rng(seed)
+    ab = rng.standard_normal((harmonics, 2)) / np.arange(1, harmonics + 1)[:, None]
+    t = np.linspace(0.0, 1.0, 2048, endpoint=False)
+    g = sum(ab[k, 0] * np.cos(2 * np.pi * (k + 1) * t) + ab[k, 1] * np.sin(2 * np.pi * (k + 1) * t)
+            for k in range(harmonics))
+    amp = (peak - mean) / g.max()
+    coef = [mean] + [float(v) for k in range(harmonics) for v in amp * ab[k]]
+    return np.asarray(coef, np.float32)
+
+
+def bifurcation_case(scale=1.0, tau=0.52, period=None, ramp=None, mean=0.03, peak=0.08):
+    """(lattice, nu, setup, info) of the precision-form workload: the stenosed bifurcation (geometry.bifurcation)
+    with the seeded 8-harmonic pulsatile inlet, pressure outlets at 0 and sponges at the outlets."""
+    from .geometry import bifurcation
+    lat, info = bifurcation(scale)
+    nu = (tau - 0.5) / 3.0
+    period = int(period or round(4000 * scale))
+    ramp = int(ramp if ramp is not None else period // 4)
+    coef = bifurcation_waveform(0, 8, mean, peak)
+
+    def setup(s):
+        s.set_drive(coef, period=period, ramp=ramp)
+        s.set_outlet_pressure([0.0, 0.0])
+        s.set_sponge(max(4.0, 20.0 * scale), 0.6, kinds=("outlet",))
+
+    return lat, nu, setup, info
+
+
+def wall_shear_weights(lat, info, dx_mm=0.1, spacing=2.0):
+    """Wall points of a bifurcation and their WSS fit weights (wss.weights) on a grid of spacing dx_mm:
+    (idx (V, K), c (V, K), normals (V, 3), points (V, 3))."""
+    from .geometry import wall_points
+    from .wss import weights
+    P, Nn = wall_points(info, spacing=spacing)
+    cells = lat["coords"].astype(float)
+    depth = -info["sdf"](cells) * dx_mm
+    idx, c, _, _ = weights(P * dx_mm, Nn, cells * dx_mm, depth)
+    return idx, c, Nn.astype(np.float32), P

@@ -131,3 +131,32 @@ def test_fp16_needs_bricks():
         lbm.Solver(lat, 0.05, storage="bf16")
     s = lbm.Solver(lat, 0.05, storage="fp16")
     assert s.layout == "bricks" and s.f.dtype.itemsize == 2
+
+
+def test_precision_bifurcation_u_p_wss_vs_fp64():
+    """The precision-form workload at 0.15 scale (48,593 cells): stenosed bifurcation, pulsatile 8-harmonic inlet
+    (peak 0.08, throat speed ~0.22), tau 0.52, 800 steps. FP16 and FP32 storage against the FP64 oracle on velocity,
+    pressure and wall shear stress (device cs_wss vs the numpy restatement on the FP64 field)."""
+    from oracle.wss import shear
+    from cerebrosim_amd import wss, workloads
+    lat, nu, setup, info = workloads.bifurcation_case(0.15, period=800)
+    idx, c, normals, _ = workloads.wall_shear_weights(lat, info)
+    steps = 800
+    o = oracle.OracleSolver(lat, nu, model="regularised", dtype=np.float64)
+    setup(o)
+    o.run(steps)
+    po, uo = o.fields()
+    to = shear(uo, normals, idx, c)
+    errs = {}
+    for st in ("fp32", "fp16"):
+        s = lbm.Solver(lat, nu, model="regularised", layout="bricks", storage=st)
+        setup(s)
+        s.run(steps)
+        assert s.ok()
+        p, u = s.fields()
+        ws = wss.WallShear(s, idx, c, normals)
+        tau = ws.frame(accumulate=False).cpu().numpy()
+        errs[st] = {"u": rel(u, uo), "p": rel(p, po), "wss": rel(tau, to)}
+    print(f"bifurcation vs FP64: {errs}; |u|max {np.abs(uo).max():.3f}", flush=True)
+    assert errs["fp32"]["u"] <= 1e-5 and errs["fp32"]["p"] <= 1e-5 and errs["fp32"]["wss"] <= 1e-5
+    assert errs["fp16"]["u"] <= TOL_U and errs["fp16"]["p"] <= TOL_U and errs["fp16"]["wss"] <= TOL_U
