@@ -9,7 +9,9 @@ read + written x 2 B, previous velocity and density read 16 B, velocity + densit
 The chosen forms are measured after the headline (``--forms``, default all three, on reduced samples unless
 ``--full``):
 
-* precision: FP16 population storage on the same pipe (MLUPS and roofline of its dominant launch);
+* precision: FP16 population storage on the same pipe (MLUPS and roofline of its dominant launch), and the stenosed
+  bifurcation (half size by default, ~1.8 M cells; full size with ``--full``) in FP32 and FP16 with their difference
+  on u, p and wall shear stress;
 * batching: a seeded set of stenosed tubes (radius U[8, 48], length U[128, 960], area reduction U[0, 0.95],
   tau U[0.51, 0.9]; 32 tubes at half size by default, the full 96 with ``--full``) stepped together in brick arenas
   (autotuned launch variant), FP32 and FP16, against the same tubes solved one after the other by the plain solver;
@@ -114,6 +116,44 @@ def form_precision(lat, steps, warmup):
             "kernel": name, "kernel_ms": round(k_ms, 5), "achieved_gbs": round(ach, 1),
             "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_cell": BYTES_PER_LUP["fp16"],
             "error_vs_fp64": "tests/test_gpu_fp16.py: relative L2 of u within 1e-3 of the FP64 oracle"}
+
+
+def form_precision_bifurcation(full, steps=None):
+    """The stenosed bifurcation (precision-form workload) with FP32 and FP16 storage on the GPU: MLUPS of each and the
+    FP16-vs-FP32 relative L2 of u, p and wall shear stress after the same steps. (The FP64 comparison needs the
+    oracle, which only the tests run: tests/test_gpu_fp16.py::test_precision_bifurcation_u_p_wss_vs_fp64.)"""
+    import torch
+    from cerebrosim_amd import lbm, wss, workloads
+    scale = 1.0 if full else 0.5
+    t0 = time.perf_counter()
+    lat, nu, setup, info = workloads.bifurcation_case(scale)
+    idx, c, normals, _ = workloads.wall_shear_weights(lat, info)
+    N = lat["nbr"].shape[1]
+    print(f"bench: bifurcation scale {scale}: {N} cells, built in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    steps = steps or int(round(4000 * scale))          # one inlet period
+    res, out = {}, {}
+    for st in ("fp32", "fp16"):
+        s = lbm.Solver(lat, nu, model="regularised", layout="bricks", storage=st)
+        setup(s)
+        s.run(200)
+        s.synchronize()
+        t1 = time.perf_counter()
+        s.run(steps - 200)
+        s.synchronize()
+        secs = time.perf_counter() - t1
+        p, u = s.fields()
+        tau = wss.WallShear(s, idx, c, normals).frame(accumulate=False).cpu().numpy()
+        res[st] = (p, u, tau, s.ok())
+        out[f"mlups_{st}"] = round(N * (steps - 200) / secs / 1e6, 1)
+        del s
+        torch.cuda.empty_cache()
+    rel = lambda a, b: float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+    (p32, u32, t32, ok32), (p16, u16, t16, ok16) = res["fp32"], res["fp16"]
+    out.update({"workload": f"stenosed_bifurcation_scale{scale}", "cells": N, "steps": steps, "ok": ok32 and ok16,
+                "fp16_vs_fp32_rel_l2": {"u": rel(u16, u32), "p": rel(p16, p32), "wss": rel(t16, t32)},
+                "wall_points": int(len(normals))})
+    return out
 
 
 def form_batching(full, steps=400):
@@ -296,6 +336,7 @@ def main():
         out["forms"] = {}
         if "precision" in forms:
             out["forms"]["precision"] = form_precision(lat, args.steps, args.warmup)
+            out["forms"]["precision"]["bifurcation"] = form_precision_bifurcation(args.full)
             print(f"bench: precision form {out['forms']['precision']['mlups']} MLUPS", file=sys.stderr, flush=True)
         if "batching" in forms:
             out["forms"]["batching"] = form_batching(args.full)
