@@ -31,17 +31,19 @@ class LayoutError(ValueError):
 
 
 def infer_periodic(lat):
-    """Axes along which some fluid link wraps around the grid (build_lattice's `periodic`, not stored in the dict)."""
-    nbr, coords = lat["nbr"], lat["coords"].astype(np.int64)
+    """Axes along which some fluid link wraps around the grid (build_lattice's `periodic`, not stored in the dict).
+    Only cells on a grid face can have a wrapping link: at coordinate 0 along an axis, a link pulling from -1 has a
+    fluid source only if the axis wraps (and likewise at shape - 1)."""
+    nbr, coords = lat["nbr"], lat["coords"]
+    shape = np.asarray(lat["shape"])
     periodic = [0, 0, 0]
-    for i in range(1, len(C)):
-        src = nbr[i]
-        m = src >= 0
-        if not m.any():
-            continue
-        delta = coords[src[m]] - coords[m] + C[i]
-        for a in range(3):
-            if np.any(delta[:, a] != 0):
+    for a in range(3):
+        for edge, sign in ((0, 1), (int(shape[a]) - 1, -1)):
+            cells = np.flatnonzero(coords[:, a] == edge)
+            if not len(cells):
+                continue
+            dirs = np.flatnonzero(C[:, a] == sign)            # sources at coordinate edge - sign: off the grid
+            if (nbr[np.ix_(dirs, cells)] >= 0).any():
                 periodic[a] = 1
     return periodic
 

@@ -48,7 +48,7 @@ class Server:
     lattices; keep: "summary" or "fields" (also keep the final (p, u) of each request)."""
 
     def __init__(self, storage="fp16", budget_bytes=None, chunk=200, arena_bricks=1 << 20, arena_records=1 << 18,
-                 workers=4, keep="summary", max_cases=256, variant=0, device=None, **solver_kw):
+                 workers=12, keep="summary", max_cases=256, variant=0, device=None, **solver_kw):
         import torch
         free = None
         if torch.cuda.is_available():

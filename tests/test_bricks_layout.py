@@ -59,6 +59,25 @@ def test_periodic_box_wraps():
     assert lay["n_cells"] == 20160
 
 
+def test_periodic_inference_matches_brute_force():
+    """The face-cell test of infer_periodic equals the definition: some fluid link whose source coordinate differs
+    from coords - c by a wrap."""
+    def brute(lat):
+        nbr, coords = lat["nbr"], lat["coords"].astype(np.int64)
+        per = [0, 0, 0]
+        for i in range(1, 27):
+            m = nbr[i] >= 0
+            d = coords[nbr[i][m]] - coords[m] + L.C[i]
+            for a in range(3):
+                per[a] |= int(np.any(d[:, a] != 0))
+        return per
+    lats = [periodic_pipe(8, nz=4)[0], poiseuille_box()[0], tilted_pipe(),
+            open_pipe((0, 0, 1), 12, lambda s: np.full_like(np.asarray(s, float), 3.0))[0]]
+    for lat in lats:
+        assert bricks.infer_periodic(lat) == brute(lat)
+    assert bricks.infer_periodic(lats[0]) == [0, 0, 1] and bricks.infer_periodic(lats[2]) == [0, 0, 0]
+
+
 def test_short_periodic_axis_is_rejected():
     lat, _, _ = periodic_pipe(10)           # 2 nodes along the periodic axis: not a multiple of the brick edge
     with pytest.raises(bricks.LayoutError):
