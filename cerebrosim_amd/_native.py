@@ -32,6 +32,17 @@ class StepArgs(ctypes.Structure):
                 ("model", c_i32), ("galilean", c_i32)]
 
 
+class Opening(ctypes.Structure):
+    _fields_ = [("kind", c_i32), ("centre", ctypes.c_double * 3), ("normal", ctypes.c_double * 3),
+                ("radius", ctypes.c_double)]
+
+
+class GridIn(ctypes.Structure):
+    _fields_ = [("phi", c_void_p), ("shape", c_i32 * 3), ("periodic", c_i32 * 3), ("origin", ctypes.c_double * 3),
+                ("dx", ctypes.c_double), ("openings", c_void_p), ("n_openings", c_i32), ("slack", ctypes.c_double),
+                ("threads", c_i32)]
+
+
 class ClockCasesArgs(ctypes.Structure):
     _fields_ = [("tick", c_void_p), ("coef", c_void_p), ("nh", c_void_p), ("period", c_void_p), ("ramp", c_void_p),
                 ("drive", c_void_p), ("n_cases", c_i32), ("coef_stride", c_i32)]
@@ -118,6 +129,10 @@ SIGNATURES = {
     "cs_gather": [ctypes.POINTER(GatherArgs), c_void_p],
     "cs_reduce_finite": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
     "cs_wss": [ctypes.POINTER(WssArgs), c_void_p],
+    "cs_lattice_build": [ctypes.POINTER(GridIn), ctypes.POINTER(c_void_p)],
+    "cs_lattice_sizes": [c_void_p, ctypes.POINTER(c_i64), c_void_p, c_i32],
+    "cs_lattice_copy": [c_void_p, ctypes.c_char_p, c_void_p],
+    "cs_lattice_free": [c_void_p],
     "cs_bricks_build": [ctypes.POINTER(LatticeIn), ctypes.POINTER(c_void_p)],
     "cs_bricks_sizes_of": [c_void_p, ctypes.POINTER(BricksSizes)],
     "cs_bricks_copy": [c_void_p, ctypes.c_char_p, c_void_p],

@@ -87,7 +87,54 @@ def _classify_openings(points, code, inlets, outlets, dx, slack):
     return code
 
 
-def build_lattice(phi, origin, dx, openings=(), periodic=(), opening_slack=1.0):
+def build_lattice(phi, origin, dx, openings=(), periodic=(), opening_slack=1.0, native=True, threads=0):
+    """Sparse fluid-cell lattice from a signed distance sampled on grid nodes (negative = fluid), lbm.py:264-330.
+
+    native=True (default) builds with the threaded C++ builder (cs_lattice_build); native=False with the numpy
+    restatement below (build_lattice_numpy), which gives identical arrays (tested). Arguments and result as
+    build_lattice_numpy."""
+    if not native:
+        return build_lattice_numpy(phi, origin, dx, openings, periodic, opening_slack)
+    import ctypes
+    from . import _native
+    lib = _native.load()
+    phi = np.ascontiguousarray(phi, np.float32)
+    origin = np.asarray(origin, float)
+    outlets = [o for o in openings if o["kind"] == "outlet"]
+    inlets = [o for o in openings if o["kind"] == "inlet"]
+    ops = (_native.Opening * max(1, len(openings)))()
+    for k, o in enumerate(inlets + outlets):
+        ops[k].kind = 0 if o["kind"] == "inlet" else 1
+        ops[k].centre = (ctypes.c_double * 3)(*[float(v) for v in o["centre"]])
+        ops[k].normal = (ctypes.c_double * 3)(*[float(v) for v in o["normal"]])
+        ops[k].radius = float(o["radius"])
+    g = _native.GridIn()
+    g.phi = phi.ctypes.data
+    g.shape = (ctypes.c_int32 * 3)(*phi.shape)
+    g.periodic = (ctypes.c_int32 * 3)(*[int(a in tuple(periodic)) for a in range(3)])
+    g.origin = (ctypes.c_double * 3)(*origin)
+    g.dx, g.openings, g.n_openings = float(dx), ctypes.addressof(ops), len(openings)
+    g.slack, g.threads = float(opening_slack), int(threads)
+    h = ctypes.c_void_p()
+    _native.call("cs_lattice_build", ctypes.byref(g), ctypes.byref(h))
+    try:
+        n = ctypes.c_int64()
+        cnt = np.zeros(2 + len(outlets), np.int64)
+        lib.cs_lattice_sizes(h, ctypes.byref(n), ctypes.c_void_p(cnt.ctypes.data), len(cnt))
+        N = int(n.value)
+        out = {"coords": np.empty((N, 3), np.int32), "nbr": np.empty((Q, N), np.int32),
+               "q": np.empty((Q, N), np.float32), "uin": np.empty((N, 3), np.float32)}
+        for name, arr in out.items():
+            if arr.size:
+                _native.call("cs_lattice_copy", h, name.encode(), ctypes.c_void_p(arr.ctypes.data))
+    finally:
+        lib.cs_lattice_free(h)
+    counts = {"wall": int(cnt[0]), "inlet": int(cnt[1]), "outlet": [int(v) for v in cnt[2:]]}
+    return {"nbr": out["nbr"], "q": out["q"], "uin": out["uin"], "coords": out["coords"], "origin": origin, "dx": dx,
+            "shape": phi.shape, "n_outlets": len(outlets), "counts": counts, "openings": inlets + outlets}
+
+
+def build_lattice_numpy(phi, origin, dx, openings=(), periodic=(), opening_slack=1.0):
     """Sparse fluid-cell lattice from a signed distance sampled on grid nodes (negative = fluid).
 
     Nodes sit at origin + (i, j, k) * dx. ``openings`` are dicts {"kind": "inlet"|"outlet", "centre", "normal"

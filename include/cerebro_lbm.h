@@ -136,6 +136,37 @@ int cs_step_indexed(const cs_step_args* a, void* stream);
  * checks that every fluid link of nbr is the implicit brick neighbour; it fails with CS_E_LAYOUT otherwise (the
  * caller then uses the indexed layout).
  * ------------------------------------------------------------------------------------------------------------- */
+/* ---------------------------------------------------------------------------------------------------------------
+ * Lattice builder -- build_lattice of lbm.py:264-330 from a signed distance on grid nodes (negative = fluid), in
+ * native code. Output arrays (cs_lattice_copy): "coords" (N, 3) int32 in C order of the grid, "nbr" (Q, N) int32
+ * source cell or link code, "q" (Q, N) float32 wall fraction (0.5 where unused), "uin" (N, 3) float32 inlet profile.
+ * ------------------------------------------------------------------------------------------------------------- */
+#define CS_WALL (-1)
+#define CS_INLET (-2)
+#define CS_OUTLET0 (-3)          /* outlet k: CS_OUTLET0 - k */
+typedef struct cs_opening {
+    int32_t kind;               /* 0 inlet, 1 outlet */
+    double centre[3];
+    double normal[3];           /* outward */
+    double radius;
+} cs_opening;
+typedef struct cs_grid_in {
+    const float* phi;           /* [shape0][shape1][shape2], C order */
+    int32_t shape[3];
+    int32_t periodic[3];
+    double origin[3];
+    double dx;
+    const cs_opening* openings; /* inlets are tried before outlets; outlet k = k-th outlet in this order */
+    int32_t n_openings;
+    double slack;               /* opening radius slack, in dx */
+    int32_t threads;            /* host threads, 0 = all */
+} cs_grid_in;
+int cs_lattice_build(const cs_grid_in* in, void** handle);
+/* N into *n_cells and up to max_counts link counts (wall, inlet, outlet 0, ...); returns the number of counts. */
+int cs_lattice_sizes(const void* handle, int64_t* n_cells, int64_t* counts, int32_t max_counts);
+int cs_lattice_copy(const void* handle, const char* name, void* dst);
+int cs_lattice_free(void* handle);
+
 typedef struct cs_lattice_in {
     int64_t n_cells;
     int32_t shape[3];

@@ -109,3 +109,29 @@ def test_scatter_roundtrip():
     g = L.scatter(lat, r)
     assert np.isnan(g).sum() == np.prod(lat["shape"]) - len(r)
     assert np.array_equal(g[tuple(lat["coords"].T)], r)
+
+
+def test_native_builder_equals_numpy(monkeypatch):
+    """The C++ builder (cs_lattice_build, the default) gives the numpy restatement's arrays and counts exactly, on
+    periodic, tilted, narrowed and branching geometries with inlets and several outlets."""
+    from cerebrosim_amd import geometry
+    calls = []
+    real = geometry.build_lattice
+
+    def spy(phi, origin, dx, openings=(), periodic=(), opening_slack=1.0, **kw):
+        calls.append((np.array(phi, np.float32), origin, dx, list(openings), tuple(periodic), opening_slack))
+        return real(phi, origin, dx, openings, periodic, opening_slack)
+
+    monkeypatch.setattr(geometry, "build_lattice", spy)
+    geometry.periodic_pipe(20, nz=4)
+    geometry.poiseuille_box()
+    geometry.open_pipe((1, 2, 2), 30, lambda s: np.full_like(np.asarray(s, float), 5.0))
+    geometry.stenosis_pipe(1.5e-4, sev=0.6)
+    geometry.bifurcation(0.12)
+    assert len(calls) == 5
+    for phi, origin, dx, ops, per, slack in calls:
+        a = L.build_lattice(phi, origin, dx, ops, per, slack, native=True, threads=3)
+        b = L.build_lattice_numpy(phi, origin, dx, ops, per, slack)
+        for k in ("coords", "nbr", "q", "uin"):
+            assert a[k].dtype == b[k].dtype and np.array_equal(a[k], b[k]), (phi.shape, k)
+        assert a["counts"] == b["counts"] and a["n_outlets"] == b["n_outlets"]

@@ -51,7 +51,8 @@ STRUCTS = {"cs_clock_args": _native.ClockArgs, "cs_step_args": _native.StepArgs,
            "cs_gather_args": _native.GatherArgs, "cs_wss_args": _native.WssArgs, "cs_lattice_in": _native.LatticeIn,
            "cs_bricks_sizes": _native.BricksSizes, "cs_bricks_step_args": _native.BricksStepArgs,
            "cs_bricks_convert_args": _native.BricksConvertArgs, "cs_clock_cases_args": _native.ClockCasesArgs,
-           "cs_flux_cases_args": _native.FluxCasesArgs, "cs_clock_table_args": _native.ClockTableArgs}
+           "cs_flux_cases_args": _native.FluxCasesArgs, "cs_clock_table_args": _native.ClockTableArgs,
+           "cs_opening": _native.Opening, "cs_grid_in": _native.GridIn}
 
 
 def test_ctypes_layout_matches_header(tmp_path):
