@@ -227,10 +227,10 @@ def form_serving(full, rate=None):
     """Stenosis requests served with continuous batching (FP16) vs solved one after the other (FP32)."""
     import torch
     from cerebrosim_amd import serving, workloads
-    n = 64 if full else 32
+    n = 192 if full else 96
     grids = (0.2e-3, 0.1e-3, 0.05e-3) if full else (0.2e-3, 0.1e-3)
     beat = 2000 if full else 500
-    rate = rate or (8.0 if full else 32.0)          # arrivals faster than service: the queue builds up
+    rate = rate or 32.0                              # arrivals faster than service: the queue builds up
     reqs = workloads.request_specs(n, rate_per_s=rate, seed=2, beats=4, beat_steps=beat, grids=grids)
     srv = serving.Server(storage="fp16", chunk=200)
     rep = srv.serve(reqs, workloads.request_case)
@@ -238,10 +238,15 @@ def form_serving(full, rate=None):
     torch.cuda.empty_cache()
     seq = serving.run_sequential(reqs, workloads.request_case, storage="fp32", chunk=200)
     ok = all(r["result"]["ok"] for r in rep["records"])
+    # steady state: completions between the 25th and 75th percentile completion times
+    done = np.sort([r["done"] for r in rep["records"]])
+    a, b = done[len(done) // 4], done[(3 * len(done)) // 4]
+    steady = ((3 * len(done)) // 4 - len(done) // 4) / max(b - a, 1e-9) * 3600.0
     return {"workload": f"stenosis_requests_seed2_n{n}_{'full' if full else 'grids_0.2_0.1mm'}_beat{beat}",
             "requests": n, "rate_per_s": rate, "storage": "fp16", "chunk": rep["chunk"],
             "cases_per_hour": round(rep["cases_per_hour"], 1), "p50_s": round(rep["p50_s"], 3),
             "p99_s": round(rep["p99_s"], 3), "wall_s": round(rep["wall_s"], 2), "mlups": round(rep["mlups"], 1),
+            "steady_cases_per_hour": round(steady, 1),
             "ok": ok, "sequential_fp32_cases_per_hour": round(seq["cases_per_hour"], 1),
             "sequential_fp32_wall_s": round(seq["wall_s"], 2),
             "speedup_vs_sequential": round(rep["cases_per_hour"] / seq["cases_per_hour"], 2)}

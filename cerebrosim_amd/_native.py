@@ -160,6 +160,10 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    # torch first: its libraries need "libamdhip64.so" by that name, ours needs "libamdhip64.so.7". Loaded in this
+    # order the process has one HIP runtime (torch's copy, whose soname satisfies our library); loaded the other way
+    # round it would get two, and the streams torch creates would be foreign to the runtime our kernels launch on.
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     for name, args in SIGNATURES.items():
         fn = getattr(L, name)

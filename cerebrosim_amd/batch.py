@@ -139,6 +139,7 @@ class Arena:
         self.interior = self.boundary = None
         self.n_interior = self.n_boundary = 0
         self.flux = None
+        self.dirty = True
 
     def nbytes(self):
         return arena_bytes(self.cap_b, self.cap_r, self.batch.storage, self.cap_l, self.cap_s)
@@ -164,47 +165,74 @@ class Arena:
             pool.release(s, n)
 
     def rebuild_lists(self):
-        """Device lists from the cases now in the arena (host arrays, one upload each)."""
+        """Device lists from the cases now in the arena. The list buffers have power-of-two capacities and hold skip
+        entries past the live ones, and the launches always cover the whole capacity, so a captured graph stays valid
+        when cases join or leave. Returns True when a buffer had to grow (the graphs must be captured again)."""
         torch = _torch()
         dev = self.batch.dev
         shift = _native.CS_CASE_SHIFT
+        skip = np.uint32(_native.CS_LIST_SKIP)
         inner = [(c.lay["interior"].astype(np.uint32) + np.uint32(c.b0)) | np.uint32(c.cid << shift)
                  for c in self.cases]
         inner = np.concatenate(inner) if inner else np.zeros(0, np.uint32)
         hi = self.records.high() if self.records.capacity else 0
-        bnd = np.full(hi, _native.CS_LIST_SKIP, np.uint32)
+        bnd = np.full(hi, skip, np.uint32)
         for c in self.cases:
             nbd = int(c.lay["n_boundary"])
             if nbd:
                 bnd[c.r0:c.r0 + nbd] = ((c.lay["boundary"].astype(np.uint32) + np.uint32(c.b0)) |
                                         np.uint32(c.cid << shift))
-        up = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).to(dev)
-        self.interior = up(inner) if len(inner) else torch.zeros(1, dtype=torch.int32, device=dev)
-        self.boundary = up(bnd) if len(bnd) else torch.zeros(1, dtype=torch.int32, device=dev)
+        grew = False
+
+        def put(buf, data, dtype, fill):
+            nonlocal grew
+            cap = _pow2(len(data))
+            if buf is None or buf.numel() < cap:
+                buf = torch.empty(cap, dtype=dtype, device=dev)
+                grew = True
+            host = np.full(buf.numel(), fill, data.dtype)
+            host[:len(data)] = data
+            self.batch._upload(buf, host.view(np.int32) if data.dtype == np.uint32 else host)
+            return buf
+
+        self.interior = put(self.interior, inner, torch.int32, skip)
+        self.boundary = put(self.boundary, bnd, torch.int32, skip)
         self.n_interior, self.n_boundary = len(inner), len(bnd)
-        # flow-measurement lists (cases with openings under measurement)
-        fl = [c for c in self.cases if c.flux is not None]
+        # flow-measurement lists (cases with openings under measurement); once present, the flow launch stays in the
+        # step (cases without measurement have n_open 0 and their workgroups return at once)
         nopen = np.zeros(self.batch.max_cases, np.int32)
-        if fl:
-            start = np.zeros(self.batch.max_cases + 1, np.int64)
-            idx, w, slot = [], [], []
-            pos = 0
-            for cid in range(self.batch.max_cases):
-                start[cid] = pos
-                c = self.batch.case_by_id.get(cid)
-                if c is not None and c.arena is self and c.flux is not None:
-                    idx.append(c.flux_slots)
-                    w.append(c.flux["w"])
-                    slot.append(c.flux["slot"])
-                    pos += len(c.flux_slots)
-                    nopen[cid] = c.n_open
-            start[self.batch.max_cases] = pos
-            cat = lambda xs, d: torch.from_numpy(np.ascontiguousarray(np.concatenate(xs), d)).to(dev)
-            self.flux = {"idx": cat(idx, np.int32), "w": cat(w, np.float32), "slot": cat(slot, np.int32),
-                         "start": torch.from_numpy(start).to(dev)}
-        else:
-            self.flux = None
-        self.n_open.copy_(torch.from_numpy(nopen))
+        start = np.zeros(self.batch.max_cases + 1, np.int64)
+        idx, w, slot = [np.zeros(0, np.int32)], [np.zeros((0, 3), np.float32)], [np.zeros(0, np.int32)]
+        pos = 0
+        for cid in range(self.batch.max_cases):
+            start[cid] = pos
+            c = self.batch.case_by_id.get(cid)
+            if c is not None and c.arena is self and c.flux is not None:
+                idx.append(c.flux_slots)
+                w.append(c.flux["w"])
+                slot.append(c.flux["slot"])
+                pos += len(c.flux_slots)
+                nopen[cid] = c.n_open
+        start[self.batch.max_cases] = pos
+        if pos or self.flux is not None:
+            fx = self.flux or {"start": torch.zeros(self.batch.max_cases + 1, dtype=torch.int64, device=dev),
+                               "idx": None, "w": None, "slot": None}
+            if self.flux is None:
+                grew = True
+            fx["idx"] = put(fx["idx"], np.concatenate(idx).astype(np.int32), torch.int32, 0)
+            fx["slot"] = put(fx["slot"], np.concatenate(slot).astype(np.int32), torch.int32, 0)
+            fx["w"] = put(fx["w"], np.concatenate(w).astype(np.float32).reshape(-1), torch.float32, 0.0)
+            self.batch._upload(fx["start"], start)
+            self.flux = fx
+        self.batch._upload(self.n_open, nopen)
+        return grew
+
+
+def _pow2(n, lo=64):
+    c = lo
+    while c < n:
+        c *= 2
+    return c
 
 
 class Batch:
@@ -257,11 +285,66 @@ class Batch:
             self._work = z(_native.CS_REDUCE_WORK, torch.float64)
             self._red = z(2, torch.float64)
         self._tstream.synchronize()
+        self._ring = torch.empty(64 << 20, dtype=torch.uint8, pin_memory=True)   # upload staging
+        self._ring_np = self._ring.numpy()
+        self._ring_pos, self._ring_busy, self._ring_pending = 0, [], []
         self.arenas = []
+        self.topo = 0                      # bumped when the launch structure changes (graphs captured again)
         self.case_by_id = {}
         self.cur = 0
         self.steps = 0
         self.graphs = {}
+
+    def _stage(self, x):
+        """A pinned host tensor holding x: a region of the batch's staging ring, reused once the copies that last
+        read it have completed (their events), or a fresh pinned copy for arrays larger than the ring."""
+        torch = _torch()
+        x = np.ascontiguousarray(x)
+        n = (x.nbytes + 255) // 256 * 256
+        cap = self._ring.numel()
+        if n > cap:
+            return torch.from_numpy(x).pin_memory()
+        if self._ring_pos + n > cap:
+            self._ring_pos = 0
+        lo, hi = self._ring_pos, self._ring_pos + n
+        keep = []
+        for a, b, ev in self._ring_busy:
+            if a < hi and lo < b:
+                ev.synchronize()
+            else:
+                keep.append((a, b, ev))
+        self._ring_busy = keep
+        self._ring_np[lo:lo + x.nbytes] = x.reshape(-1).view(np.uint8)
+        self._ring_pos = hi
+        self._ring_pending.append((lo, hi))
+        return self._ring[lo:lo + x.nbytes].view(torch.from_numpy(x[:0]).dtype).reshape(x.shape)
+
+    def _staged(self):
+        """Record one event for the ring regions staged since the last call (after their copies were queued)."""
+        if self._ring_pending:
+            ev = _torch().cuda.Event()
+            ev.record(self._tstream)
+            self._ring_busy += [(a, b, ev) for a, b in self._ring_pending]
+            self._ring_pending = []
+
+    def _upload(self, dst, x):
+        """Asynchronous host-to-device copy into dst on the batch stream: ordered after the work already queued and
+        before the next launches, without blocking the host."""
+        torch = _torch()
+        x = np.asarray(x).astype(torch.empty(0, dtype=dst.dtype).numpy().dtype, copy=False).reshape(dst.shape)
+        src = self._stage(x)
+        with torch.cuda.stream(self._tstream):
+            dst.copy_(src, non_blocking=True)
+        self._staged()
+
+    def _to_device(self, x):
+        """A new device tensor holding x, copied asynchronously on the batch stream."""
+        torch = _torch()
+        src = self._stage(x)
+        with torch.cuda.stream(self._tstream):
+            out = src.to(self.dev, non_blocking=True)
+        self._staged()
+        return out
 
     # --- capacity -------------------------------------------------------------------------------------------
     def nbytes(self):
@@ -288,6 +371,7 @@ class Batch:
         with _torch().cuda.stream(self._tstream):
             a = Arena(self, nb, nr, nl, ns)
         self.arenas.append(a)
+        self.topo += 1
         return a
 
     def can_admit(self, lay):
@@ -324,7 +408,6 @@ class Batch:
             if p is None:
                 raise BatchFull("case does not fit a fresh arena")
             spot = (a, p)
-        self.synchronize()
         c = Case(self, cid, lat, nu_lu, lay, spot[0], spot[1])
         self.case_by_id[cid] = c
         spot[0].cases.append(c)
@@ -346,14 +429,22 @@ class Batch:
         if not a.cases:
             self.synchronize()
             self.arenas.remove(a)
+            self.topo += 1
         self.synchronize()
 
     def _topology_changed(self, arena):
-        self.synchronize()
+        """The arena's membership changed: its lists are rebuilt before the next launch."""
+        arena.dirty = True
+
+    def _flush(self):
+        """Rebuild the lists of the arenas whose membership changed (stream-ordered after the work in flight)."""
         with _torch().cuda.stream(self._tstream):
-            arena.rebuild_lists()
-        self.synchronize()
-        self.graphs = {}
+            for ar in self.arenas:
+                if ar.dirty:
+                    if ar.rebuild_lists():
+                        self.topo += 1
+                    ar.dirty = False
+
 
     # --- stepping -------------------------------------------------------------------------------------------
     def _drive_ptr(self):
@@ -387,7 +478,7 @@ class Batch:
         A = _native.BricksStepArgs()
         lst = arena.boundary if boundary else arena.interior
         A.f, A.nbr, A.list = arena.f.data_ptr(), arena.nbr.data_ptr(), lst.data_ptr()
-        A.n_list = arena.n_boundary if boundary else arena.n_interior
+        A.n_list = lst.numel()           # the whole capacity: entries past the live ones are skips
         A.bidx, A.lmask, A.smask = arena.bidx.data_ptr(), arena.lmask.data_ptr(), arena.smask.data_ptr()
         A.qlink, A.qstart, A.uin = arena.qlink.data_ptr(), arena.qstart.data_ptr(), arena.uin.data_ptr()
         A.side, A.sstart = arena.side.data_ptr(), arena.sstart.data_ptr()
@@ -424,6 +515,7 @@ class Batch:
         return out
 
     def step_part(self, k, boundary, a):
+        self._flush()
         ar = self.arenas[k]
         A = self.step_args(ar, a, boundary)
         if A.n_list:
@@ -443,11 +535,12 @@ class Batch:
         """Advance every case `steps` steps: graphs of `chunk` steps (made even), remainder step by step."""
         if not self.case_by_id:
             return
+        self._flush()
         chunk += chunk % 2
         full, rest = divmod(int(steps), chunk)
         if full:
             self._grow_table(chunk)
-            key = (self.cur, chunk, self.variant)
+            key = (self.cur, chunk, self.variant, self.topo)
             if key not in self.graphs:
                 with _native.capture(self.stream_handle) as cap:
                     self._clock_table(chunk)
@@ -488,6 +581,7 @@ class Batch:
         """Time every launch variant on this batch (HIP events on the batch stream, `steps` eager steps each, best of
         `repeats`), keep the fastest, and restore the state the timing runs changed. Returns {variant: ms/step}."""
         torch = _torch()
+        self._flush()
         snap = self._snapshot()
         times = {}
         try:
@@ -534,6 +628,46 @@ class Batch:
         self.synchronize()
 
 
+def sponge_slots(lat, lay, tau0, length, tau_max, kinds=("inlet", "outlet")):
+    """Host part of set_sponge: omega per slot of the layout's bricks and the flags of the bricks whose cells
+    differ from 1 / tau0."""
+    tau = sponge_tau(lat, tau0, length, tau_max, kinds)
+    omega = np.float32(1.0 / tau0)
+    nb = int(lay["n_bricks"])
+    slots = np.full(nb * BC, omega, np.float32)
+    slots[lay["slot_of_cell"]] = (1.0 / tau).astype(np.float32)
+    own = (slots.reshape(nb, BC) != omega).any(1).astype(np.uint8)
+    return slots, own
+
+
+class SetupRecorder:
+    """Stands in for a Case while a request is prepared off the serving thread: records the setter calls, and
+    computes the sponge's per-slot omega there. ``replay(case)`` applies them to the admitted case."""
+
+    def __init__(self, lat, nu_lu, lay):
+        self.lat, self.lay, self.tau = lat, lay, 3.0 * nu_lu + 0.5
+        self.calls = []
+
+    def set_drive(self, *a, **k):
+        self.calls.append(("set_drive", a, k))
+
+    def set_force(self, *a, **k):
+        self.calls.append(("set_force", a, k))
+
+    def set_outlet_pressure(self, *a, **k):
+        self.calls.append(("set_outlet_pressure", a, k))
+
+    def set_outlets(self, *a, **k):
+        self.calls.append(("set_outlets", a, k))
+
+    def set_sponge(self, length, tau_max, kinds=("inlet", "outlet")):
+        self.calls.append(("set_omega_slots", sponge_slots(self.lat, self.lay, self.tau, length, tau_max, kinds), {}))
+
+    def replay(self, case):
+        for name, a, k in self.calls:
+            getattr(case, name)(*a, **k)
+
+
 class Case:
     """One lattice inside a Batch, with the lbm.Solver surface (lbm.py:346-455) and the resistance-outlet flow
     measurement of simulate.py (ArterySolver)."""
@@ -551,27 +685,28 @@ class Case:
         nb, nr = int(lay["n_bricks"]), int(lay["n_boundary"])
         dev = batch.dev
         self.slots = (lay["slot_of_cell"].astype(np.int64) + b0 * BC)
+        # every upload below is asynchronous on the batch stream: queued after the chunks in flight (which may still
+        # step a case that left this range) and before the next launches
+        self.slot_dev = batch._to_device(self.slots)
+        self.slot32 = batch._to_device(self.slots.astype(np.int32))
+        nbr = lay["nbr"].astype(np.int64)
+        batch._upload(arena.nbr[b0 * 27:(b0 + nb) * 27], np.where(nbr >= 0, nbr + b0, -1).astype(np.int32))
+        bidx = lay["bidx"].astype(np.int64)
+        batch._upload(arena.bidx[b0:b0 + nb], np.where(bidx >= 0, bidx + r0, -1).astype(np.int32))
+        up = batch._upload
         with torch.cuda.stream(batch._tstream):
-            up = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
-            self.slot_dev = up(self.slots)
-            self.slot32 = up(self.slots.astype(np.int32))
-            nbr = lay["nbr"].astype(np.int64)
-            nbr = np.where(nbr >= 0, nbr + b0, -1).astype(np.int32)
-            arena.nbr[b0 * 27:(b0 + nb) * 27].copy_(up(nbr.reshape(-1)))
-            bidx = lay["bidx"].astype(np.int64)
-            arena.bidx[b0:b0 + nb].copy_(up(np.where(bidx >= 0, bidx + r0, -1).astype(np.int32)))
             arena.om_brick[b0:b0 + nb].zero_()
             arena.om[b0 * BC:(b0 + nb) * BC].zero_()
             arena.f[b0 * BLOCK:(b0 + nb) * BLOCK].zero_()
             arena.mom.zero(b0 * BC, (b0 + nb) * BC)
             if nr:
-                arena.lmask[r0 * BC:(r0 + nr) * BC].copy_(up(lay["lmask"].view(np.int32).reshape(-1)))
-                arena.smask[r0 * BC:(r0 + nr) * BC].copy_(up(lay["smask"].view(np.int32).reshape(-1)))
-                arena.uin[r0 * 3 * BC:(r0 + nr) * 3 * BC].copy_(up(lay["uin"].reshape(-1)))
-                arena.qstart[r0:r0 + nr].copy_(up((lay["qstart"][:-1].astype(np.int64) + q0).astype(np.int32)))
-                arena.sstart[r0:r0 + nr].copy_(up((lay["sstart"][:-1].astype(np.int64) + s0).astype(np.int32)))
+                up(arena.lmask[r0 * BC:(r0 + nr) * BC], lay["lmask"].view(np.int32).reshape(-1))
+                up(arena.smask[r0 * BC:(r0 + nr) * BC], lay["smask"].view(np.int32).reshape(-1))
+                up(arena.uin[r0 * 3 * BC:(r0 + nr) * 3 * BC], lay["uin"].reshape(-1))
+                up(arena.qstart[r0:r0 + nr], (lay["qstart"][:-1].astype(np.int64) + q0).astype(np.int32))
+                up(arena.sstart[r0:r0 + nr], (lay["sstart"][:-1].astype(np.int64) + s0).astype(np.int32))
                 if lay["n_links"]:
-                    arena.qlink[q0:q0 + int(lay["n_links"])].copy_(up(lay["qlink"]))
+                    up(arena.qlink[q0:q0 + int(lay["n_links"])], lay["qlink"])
                 if lay["n_side"]:
                     arena.side[s0:s0 + int(lay["n_side"])].zero_()
             c = cid
@@ -581,7 +716,6 @@ class Case:
             batch.force[c].zero_()
             batch.p_out[c].zero_()
             batch.q_last[c].zero_()
-        batch.synchronize()
         self.set_drive([1.0])
 
     # --- reference surface ----------------------------------------------------------------------------------
@@ -590,11 +724,7 @@ class Case:
         return self.batch.cur
 
     def _set(self, tensor, value):
-        torch = _torch()
-        self.batch.synchronize()
-        with torch.cuda.stream(self.batch._tstream):
-            tensor.copy_(torch.as_tensor(np.asarray(value), dtype=tensor.dtype).reshape(tensor.shape).to(self.batch.dev))
-        self.batch.synchronize()
+        self.batch._upload(tensor, np.asarray(value))
 
     def set_drive(self, coef, period=1, ramp=0):
         """As lbm.Solver.set_drive (lbm.py:384-392): the case's clock restarts from its current tick."""
@@ -616,21 +746,14 @@ class Case:
         self._set(self.batch.force[self.cid], [float(np.float32(x)) for x in F])
 
     def set_sponge(self, length, tau_max, kinds=("inlet", "outlet")):
-        torch = _torch()
-        tau = sponge_tau(self.lat, self.tau, length, tau_max, kinds)
-        self.tau_cell = tau
-        om = (1.0 / tau).astype(np.float32)
-        omega = np.float32(1.0 / self.tau)
-        nb = int(self.lay["n_bricks"])
-        slots = np.full(nb * BC, omega, np.float32)
-        slots[self.lay["slot_of_cell"]] = om
-        own = (slots.reshape(nb, BC) != omega).any(1).astype(np.uint8)
-        ar, b0 = self.arena, self.b0
-        self.batch.synchronize()
-        with torch.cuda.stream(self.batch._tstream):
-            ar.om[b0 * BC:(b0 + nb) * BC].copy_(torch.from_numpy(slots))
-            ar.om_brick[b0:b0 + nb].copy_(torch.from_numpy(own))
-        self.batch.synchronize()
+        """As lbm.Solver.set_sponge (lbm.py:398-416)."""
+        self.set_omega_slots(*sponge_slots(self.lat, self.lay, self.tau, length, tau_max, kinds))
+
+    def set_omega_slots(self, slots, own):
+        """Per-slot omega of the case's bricks and the bricks that need it (from sponge_slots)."""
+        nb, ar, b0 = int(self.lay["n_bricks"]), self.arena, self.b0
+        self.batch._upload(ar.om[b0 * BC:(b0 + nb) * BC], slots)
+        self.batch._upload(ar.om_brick[b0:b0 + nb], own)
 
     def set_outlet_pressure(self, p_lu):
         p = np.zeros(self.batch.max_open, np.float32)

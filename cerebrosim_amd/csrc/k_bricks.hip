@@ -95,7 +95,7 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
     CS_REQUIRE(a->n_slots * cs::Q < ((int64_t)1 << 32), "cs_bricks_step: %lld slots exceed 32-bit population offsets",
                (long long)a->n_slots);
     if (a->n_list == 0) return CS_OK;
-    CS_REQUIRE(a->n_list > 0 && a->n_list <= a->n_slots / cs::BC, "cs_bricks_step: n_list %lld",
+    CS_REQUIRE(a->n_list > 0 && a->n_list < ((int64_t)1 << 31), "cs_bricks_step: n_list %lld",
                (long long)a->n_list);
     CS_REQUIRE(a->variant >= 0 && a->variant < cs::kVariants, "cs_bricks_step: variant %d not in [0, %d)", a->variant,
                cs::kVariants);

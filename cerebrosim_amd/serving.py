@@ -8,7 +8,8 @@ Requests (lattice solves with a step count) arrive over time. The server keeps o
   request that does not fit waits, and so do the ones behind it (no starvation of large requests);
 * all admitted cases advance together in graph-captured chunks of ``chunk`` steps; a request's step count is
   rounded up to whole chunks, so it finishes at a chunk boundary, where its results are read and its memory is
-  returned to the arena allocators;
+  returned to the arena allocators. The host waits for the device only at such boundaries, and otherwise queues at
+  most two chunks ahead; joining and leaving cases do not invalidate the captured graphs (fixed-capacity lists);
 * per request the server records arrival, admission and completion times: time-to-solution = completion -
   arrival, reported as p50 / p99 with the throughput in cases per hour.
 
@@ -35,8 +36,12 @@ def _summary(case_or_solver):
 
 
 def _build(make_case, spec):
+    """Worker-thread part of a request: lattice, brick layout, and the setup with its host work done here."""
     lat, nu, setup = make_case(spec)
-    return lat, nu, setup, bricks.build(lat)
+    lay = bricks.build(lat)
+    rec = B.SetupRecorder(lat, nu, lay)
+    setup(rec)
+    return lat, nu, rec, lay
 
 
 class Server:
@@ -73,8 +78,9 @@ class Server:
         built = []                                        # (arrival, name, spec, build result)
         active = {}                                       # cid -> record
         records = []
+        import torch
         lups = 0
-        busy = 0.0
+        inflight = []
         t0 = time.perf_counter()
         with cf.ThreadPoolExecutor(self.workers) as pool:
             while pending or building or built or active:
@@ -89,7 +95,7 @@ class Server:
                 while built and bt.can_admit(built[0][3][3]):
                     _, _, r, (lat, nu, setup, lay) = built.pop(0)
                     case = bt.add(lat, nu, layout=lay)
-                    setup(case)
+                    setup.replay(case)
                     target = -(-int(r["steps"]) // chunk) * chunk
                     active[case.cid] = {"name": r["name"], "spec": r, "case": case, "target": target,
                                         "arrival": r["arrival"] * time_scale,
@@ -106,12 +112,18 @@ class Server:
                     elif pending:
                         time.sleep(max(0.0, min(0.01, pending[0]["arrival"] * time_scale - now)))
                     continue
-                t1 = time.perf_counter()
                 bt.run(chunk, chunk=chunk)
-                bt.synchronize()
-                busy += time.perf_counter() - t1
                 lups += chunk * sum(rec["cells"] for rec in active.values())
-                for cid in [c for c, rec in active.items() if rec["case"].steps >= rec["target"]]:
+                ev = torch.cuda.Event()
+                ev.record(bt._tstream)
+                inflight.append(ev)
+                finishing = [c for c, rec in active.items() if rec["case"].steps >= rec["target"]]
+                if finishing:
+                    bt.synchronize()             # results of the cases done at the end of this chunk
+                    inflight.clear()
+                elif len(inflight) > 2:
+                    inflight.pop(0).synchronize()   # run at most two chunks ahead of the device
+                for cid in finishing:
                     rec = active.pop(cid)
                     rec["done"] = time.perf_counter() - t0
                     rec["result"] = _summary(rec["case"])
@@ -125,7 +137,7 @@ class Server:
                         log(f"done {rec['name']} at {rec['done']:.2f} s (latency {rec['done'] - rec['arrival']:.2f} s)")
         wall = time.perf_counter() - t0
         lat = np.array([r["done"] - r["arrival"] for r in records])
-        return {"records": records, "cases": len(records), "wall_s": wall, "gpu_busy_s": busy,
+        return {"records": records, "cases": len(records), "wall_s": wall,
                 "cases_per_hour": len(records) / wall * 3600.0,
                 "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
                 "mlups": lups / wall / 1e6, "storage": bt.storage, "chunk": chunk}
