@@ -100,6 +100,7 @@ def form_precision(lat, steps, warmup):
     """FP16 storage on the headline pipe: MLUPS and the dominant launch's algorithmic bandwidth."""
     import torch
     s = make_solver(lat, "bricks", "fp16")
+    s.autotune()
     s.run(warmup)
     s.synchronize()
     t0 = time.perf_counter()
@@ -110,9 +111,11 @@ def form_precision(lat, steps, warmup):
     kt = kernel_times_ms(s)
     name, (k_ms, k_cells) = max(kt.items(), key=lambda kv: kv[1][0])
     ach = BYTES_PER_LUP["fp16"] * k_cells / (k_ms * 1e-3) / 1e9
+    variant = s.variant
     del s
     torch.cuda.empty_cache()
     return {"storage": "fp16", "mlups": round(lat["nbr"].shape[1] * steps / secs / 1e6, 1), "ok": ok,
+            "variant": variant,
             "kernel": name, "kernel_ms": round(k_ms, 5), "achieved_gbs": round(ach, 1),
             "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_cell": BYTES_PER_LUP["fp16"],
             "error_vs_fp64": "tests/test_gpu_fp16.py: relative L2 of u within 1e-3 of the FP64 oracle"}
@@ -290,6 +293,7 @@ def main():
     ap.add_argument("--forms", default="precision,batching,serving",
                     help="comma-separated chosen forms to measure after the headline ('none' to skip)")
     ap.add_argument("--full", action="store_true", help="full-size batching / serving workloads")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch variant")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (the step of one lattice "
@@ -304,6 +308,7 @@ def main():
     N = lat["nbr"].shape[1]
     print(f"bench: lattice {N} cells built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     s = make_solver(lat, args.layout, args.storage)
+    tune = s.autotune() if not args.no_autotune else {}
     s.run(args.warmup)
     s.synchronize()
     torch.cuda.synchronize()
@@ -325,7 +330,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(secs / args.steps * 1e3, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "womersley_pipe_R40_L480", "cells": N, "tau": 0.56, "layout": s.layout,
-                   "storage": args.storage, "model": "regularised+galilean, bouzidi"},
+                   "storage": args.storage, "model": "regularised+galilean, bouzidi", "variant": s.variant,
+                   "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune.items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": name, "kernel_ms": round(k_ms, 5), "kernel_cells": k_cells,

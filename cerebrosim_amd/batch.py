@@ -41,6 +41,19 @@ def _torch():
     return torch
 
 
+def launch_variants():
+    """The launch variants of the brick kernels: [{"variant", "waves_per_group", "xcd_order"}]."""
+    out = []
+    v = 0
+    while True:
+        w, x = ctypes.c_int32(), ctypes.c_int32()
+        if _native.load().cs_bricks_variant_info(v, ctypes.byref(w), ctypes.byref(x)) != 0:
+            break
+        out.append({"variant": v, "waves_per_group": w.value, "xcd_order": x.value})
+        v += 1
+    return out
+
+
 class BatchFull(RuntimeError):
     """No arena has room for the case (bricks, boundary records or case slots)."""
 
@@ -567,15 +580,7 @@ class Batch:
 
     # --- autotuning -----------------------------------------------------------------------------------------
     def variants(self):
-        out = []
-        v = 0
-        while True:
-            w, x = ctypes.c_int32(), ctypes.c_int32()
-            if _native.load().cs_bricks_variant_info(v, ctypes.byref(w), ctypes.byref(x)) != 0:
-                break
-            out.append({"variant": v, "waves_per_group": w.value, "xcd_order": x.value})
-            v += 1
-        return out
+        return launch_variants()
 
     def autotune(self, steps=20, repeats=2):
         """Time every launch variant on this batch (HIP events on the batch stream, `steps` eager steps each, best of

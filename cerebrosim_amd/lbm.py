@@ -251,6 +251,7 @@ class _BricksBackend:
         A.magic = s.magic
         A.bouzidi, A.model, A.galilean = s.bouzidi, s.model, s.gal
         A.parity, A.boundary = a, int(boundary)
+        A.variant = s.variant
         return A
 
     def step(self, s, a):
@@ -384,6 +385,7 @@ class Solver:
         self._tstream.synchronize()
         self.layout = self.backend.name
         self.force = (0.0, 0.0, 0.0)
+        self.variant = 0                   # launch variant of the brick kernels (cs_bricks_variant_info)
         self.graphs = {}
         self.set_drive([1.0])
         self.cur = 0
@@ -538,6 +540,42 @@ class Solver:
 
     def synchronize(self):
         self._tstream.synchronize()
+
+    def autotune(self, steps=20, repeats=2):
+        """Time each launch variant of the brick kernels (workgroup size, XCD order) on this lattice and keep the
+        fastest; the timing steps are undone (state restored). Returns {variant: ms per step}. All variants give the
+        same results."""
+        if self.layout != "bricks":
+            return {}
+        from .batch import launch_variants
+        torch = _torch()
+        b = self.backend
+        saved = [t.clone() for t in [b.f, b.side] + b.mom.tensors()] + [self.tick.clone(), self.p_out.clone()]
+        cur, nsteps = self.cur, self.steps
+        times = {}
+        for v in range(len(launch_variants())):
+            self.variant = v
+            best = None
+            for _ in range(repeats):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(self._tstream)
+                a = self.cur
+                for _ in range(steps):
+                    self._clock()
+                    self._launch(a)
+                    a = 1 - a
+                e1.record(self._tstream)
+                self.synchronize()
+                ms = e0.elapsed_time(e1) / steps
+                best = ms if best is None else min(best, ms)
+            times[v] = best
+        for dst, src in zip([b.f, b.side] + b.mom.tensors() + [self.tick, self.p_out], saved):
+            dst.copy_(src)
+        self.synchronize()
+        self.cur, self.steps = cur, nsteps
+        self.variant = min(times, key=times.get)
+        self.graphs = {}
+        return times
 
     # --- outputs (lbm.py:450-455)
     def fields(self):
