@@ -128,15 +128,6 @@ typedef struct cs_step_args {
 int cs_step_indexed(const cs_step_args* a, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
- * Brick layout -- the MI355X storage of the same lattice (no reference counterpart: the reference keeps the
- * indexed layout of lbm.py:264-330 on the device). Fluid cells are grouped into 4 x 4 x 4 bricks (one wavefront per
- * brick); populations are stored once, per brick as a block of 27 x 64 floats, and streamed in place with the AA
- * pattern.
- * The host builder converts a lattice in the reference layout (nbr/q/uin/coords as returned by build_lattice) and
- * checks that every fluid link of nbr is the implicit brick neighbour; it fails with CS_E_LAYOUT otherwise (the
- * caller then uses the indexed layout).
- * ------------------------------------------------------------------------------------------------------------- */
-/* ---------------------------------------------------------------------------------------------------------------
  * Lattice builder -- build_lattice of lbm.py:264-330 from a signed distance on grid nodes (negative = fluid), in
  * native code. Output arrays (cs_lattice_copy): "coords" (N, 3) int32 in C order of the grid, "nbr" (Q, N) int32
  * source cell or link code, "q" (Q, N) float32 wall fraction (0.5 where unused), "uin" (N, 3) float32 inlet profile.
@@ -167,6 +158,15 @@ int cs_lattice_sizes(const void* handle, int64_t* n_cells, int64_t* counts, int3
 int cs_lattice_copy(const void* handle, const char* name, void* dst);
 int cs_lattice_free(void* handle);
 
+/* ---------------------------------------------------------------------------------------------------------------
+ * Brick layout -- the MI355X storage of the same lattice (no reference counterpart: the reference keeps the
+ * indexed layout of lbm.py:264-330 on the device). Fluid cells are grouped into 4 x 4 x 4 bricks (one wavefront per
+ * brick); populations are stored once, per brick as a block of 27 x 64 floats, and streamed in place with the AA
+ * pattern.
+ * The host builder converts a lattice in the reference layout (nbr/q/uin/coords as returned by build_lattice) and
+ * checks that every fluid link of nbr is the implicit brick neighbour; it fails with CS_E_LAYOUT otherwise (the
+ * caller then uses the indexed layout).
+ * ------------------------------------------------------------------------------------------------------------- */
 typedef struct cs_lattice_in {
     int64_t n_cells;
     int32_t shape[3];
