@@ -92,9 +92,12 @@ struct RawMoments {
     float dr, jx, jy, jz, pxx, pyy, pzz, pxy, pxz, pyz;
 };
 
+// GW: the inputs are g_i = f_i / w_i, and each sum takes the weight in a fused multiply-add (as many instructions
+// as the plain sums, and the caller saves the 27 multiplications by w_i).
+template <bool GW = false>
 __device__ __forceinline__ RawMoments raw_moments(const float (&f)[Q]) {
     RawMoments m;
-    float dr = f[0], jx = -0.0f, jy = -0.0f, jz = -0.0f;
+    float dr = GW ? Lat::w[0] * f[0] : f[0], jx = -0.0f, jy = -0.0f, jz = -0.0f;
     float pxx = -0.0f, pyy = -0.0f, pzz = -0.0f, pxy = -0.0f, pxz = -0.0f, pyz = -0.0f;
     static_for<1, Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
@@ -102,22 +105,21 @@ __device__ __forceinline__ RawMoments raw_moments(const float (&f)[Q]) {
             constexpr int a = Lat::cx[i], b = Lat::cy[i], c = Lat::cz[i];
             float s = f[i] + f[i + 1];
             float d = f[i] - f[i + 1];
-            dr = dr + s;
-            if constexpr (a == 1) jx = jx + d;
-            if constexpr (a == -1) jx = jx - d;
-            if constexpr (b == 1) jy = jy + d;
-            if constexpr (b == -1) jy = jy - d;
-            if constexpr (c == 1) jz = jz + d;
-            if constexpr (c == -1) jz = jz - d;
-            if constexpr (a != 0) pxx = pxx + s;
-            if constexpr (b != 0) pyy = pyy + s;
-            if constexpr (c != 0) pzz = pzz + s;
-            if constexpr (a * b == 1) pxy = pxy + s;
-            if constexpr (a * b == -1) pxy = pxy - s;
-            if constexpr (a * c == 1) pxz = pxz + s;
-            if constexpr (a * c == -1) pxz = pxz - s;
-            if constexpr (b * c == 1) pyz = pyz + s;
-            if constexpr (b * c == -1) pyz = pyz - s;
+            // acc + sg * w * v (GW) or acc + sg * v
+            auto acc = [](float& x, float sg, float v) {
+                if constexpr (GW) x = __builtin_fmaf(sg * Lat::w[i], v, x);
+                else x = sg > 0.0f ? x + v : x - v;
+            };
+            acc(dr, 1.0f, s);
+            if constexpr (a != 0) acc(jx, float(a), d);
+            if constexpr (b != 0) acc(jy, float(b), d);
+            if constexpr (c != 0) acc(jz, float(c), d);
+            if constexpr (a != 0) acc(pxx, 1.0f, s);
+            if constexpr (b != 0) acc(pyy, 1.0f, s);
+            if constexpr (c != 0) acc(pzz, 1.0f, s);
+            if constexpr (a * b != 0) acc(pxy, float(a * b), s);
+            if constexpr (a * c != 0) acc(pxz, float(a * c), s);
+            if constexpr (b * c != 0) acc(pyz, float(b * c), s);
         }
     });
     m.dr = dr;
@@ -148,10 +150,21 @@ struct CollideParams {
 // uset: use the velocity ut instead of j + F/2. With om = 1, F = 0 and g = 0 the regularised collision is then
 // exactly the inlet equilibrium of lbm.py:205-216 (the same expressions, each added term an exact zero), so inlet
 // cells share this code with their neighbours instead of diverging into a second path.
-template <int MODEL, bool HYB>
+//
+// GW and NEQ (regularised only) serve the float16 storage: with GW the inputs are g_i = f_i / w_i; with NEQ the
+// outputs are the stored form nscale * (P_i - feq2_i) / w_i, where feq2_i / w_i = rho' + 3 c.u + 4.5 (c.u)^2 and
+// rho' = dr - 1.5 |u|^2 with this cell's new dr and u. Subtracting feq2 from the expressions below cancels in closed
+// form:
+//   (P_i - feq2_i) / w_i = (A - rho') + k1 c.P.c + kc (c.u)(c.F) + (1.5 c.F + t3 h3_i) for the pair head i,
+//   the same with the last bracket negated for OPP[i], and A - rho' for the rest direction, where
+//   A - rho' = -k1 tr P - 3 (1 - om/2) u.F.
+// So no equilibrium is formed and subtracted again, and nothing cancels in floating point.
+template <int MODEL, bool HYB, bool GW = false, bool NEQ = false>
 __device__ __forceinline__ void collide(float (&f)[Q], const CollideParams& p, float& dr_out, float (&u)[3],
-                                        bool uset = false, float ut0 = 0.0f, float ut1 = 0.0f, float ut2 = 0.0f) {
-    RawMoments m = raw_moments(f);
+                                        bool uset = false, float ut0 = 0.0f, float ut1 = 0.0f, float ut2 = 0.0f,
+                                        float nscale = 1.0f) {
+    static_assert(!(GW || NEQ) || (MODEL == 1 && !HYB), "float16 forms of the regularised collision only");
+    RawMoments m = raw_moments<GW>(f);
     const float F0 = p.F[0], F1 = p.F[1], F2 = p.F[2];
     const float u0 = uset ? ut0 : m.jx + F0 * 0.5f, u1 = uset ? ut1 : m.jy + F1 * 0.5f,
                 u2 = uset ? ut2 : m.jz + F2 * 0.5f;
@@ -184,9 +197,48 @@ __device__ __forceinline__ void collide(float (&f)[Q], const CollideParams& p, f
         const float trP = (Pxx + Pyy + Pzz) * (1.0f / 3.0f);
         const float k1 = 4.5f * (1.0f - om);
         const float kc = 9.0f * (1.0f - 0.5f * om);
-        const float A = dr - 1.5f * usq - k1 * trP - 3.0f * (1.0f - 0.5f * om) * uF;
         const float Pxy2 = 2.0f * Pxy, Pxz2 = 2.0f * Pxz, Pyz2 = 2.0f * Pyz;
         const float sxy = Pxx + Pyy, sxz = Pxx + Pzz, syz = Pyy + Pzz, sxyz = Pxx + Pyy + Pzz;
+        if constexpr (NEQ) {
+            // the stored form, scaled by nscale (every coefficient carries the scale once)
+            const float An = nscale * (-k1 * trP - 3.0f * (1.0f - 0.5f * om) * uF);
+            const float k1n = nscale * k1, kcn = nscale * kc, fn = nscale * 1.5f, hn = nscale * p.t3;
+            const float Fn0 = kcn * F0, Fn1 = kcn * F1, Fn2 = kcn * F2;
+            const float Fo0 = fn * F0, Fo1 = fn * F1, Fo2 = fn * F2;
+            f[0] = An;
+            static_for<1, Q>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                if constexpr (pair_head<i>()) {
+                    constexpr int a = Lat::cx[i], b = Lat::cy[i], c = Lat::cz[i];
+                    constexpr int nz = (a != 0) + (b != 0) + (c != 0);
+                    float cpc;
+                    if constexpr (nz == 1) {
+                        cpc = a != 0 ? Pxx : (b != 0 ? Pyy : Pzz);
+                    } else if constexpr (nz == 2) {
+                        if constexpr (c == 0) cpc = (a * b == 1) ? sxy + Pxy2 : sxy - Pxy2;
+                        else if constexpr (b == 0) cpc = (a * c == 1) ? sxz + Pxz2 : sxz - Pxz2;
+                        else cpc = (b * c == 1) ? syz + Pyz2 : syz - Pyz2;
+                    } else {
+                        float t = sxyz;
+                        t = (a * b == 1) ? t + Pxy2 : t - Pxy2;
+                        t = (a * c == 1) ? t + Pxz2 : t - Pxz2;
+                        t = (b * c == 1) ? t + Pyz2 : t - Pyz2;
+                        cpc = t;
+                    }
+                    const float cu = cdot<i>(u0, u1, u2);
+                    const float even = An + k1n * cpc + cu * cdot<i>(Fn0, Fn1, Fn2);
+                    const float odd = cdot<i>(Fo0, Fo1, Fo2) + hn * h3<i>(k);
+                    f[i] = even + odd;
+                    f[i + 1] = even - odd;
+                }
+            });
+            dr_out = dr;
+            u[0] = u0;
+            u[1] = u1;
+            u[2] = u2;
+            return;
+        }
+        const float A = dr - 1.5f * usq - k1 * trP - 3.0f * (1.0f - 0.5f * om) * uF;
         f[0] = Lat::w[0] * A;
         static_for<1, Q>([&](auto ic) {
             constexpr int i = decltype(ic)::value;

@@ -254,8 +254,17 @@ __device__ __forceinline__ float h16_decode(_Float16 h, const float4& m) {
     return Lat::w[I] * ((float)h * (1.0f / kH16Scale) + m.x + cu * (3.0f + 4.5f * cu));
 }
 
+// The same value divided by w_I (the input form of collide<1, false, true, ...>).
+template <int I>
+__device__ __forceinline__ float h16_decode_g(_Float16 h, const float4& m) {
+    const float cu = cdot<I>(m.y, m.z, m.w);
+    return __builtin_fmaf((float)h, 1.0f / kH16Scale, __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), m.x));
+}
+
 // Stage the previous-step moments {drho - 1.5 |u|^2, ux, uy, uz} of the brick and its one-cell shell in LDS
-// (mom: float4 {drho, ux, uy, uz} per slot). Slots of absent neighbour bricks read as zero.
+// (mom: float4 {drho, ux, uy, uz} per slot). Slots of absent neighbour bricks (boundary bricks only: every neighbour
+// of an interior brick exists) read as zero.
+template <bool BND>
 __device__ __forceinline__ void load_halo(float4* halo, const int* tbl, const float4* __restrict__ mom, int L) {
     static_for<0, (kHalo + BC - 1) / BC>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
@@ -264,11 +273,11 @@ __device__ __forceinline__ void load_halo(float4* halo, const int* tbl, const fl
             const int hx = e % 6, hy = (e / 6) % 6, hz = e / 36;
             const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
                           9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
-            const int l = ((hx - 1) & 3) + 4 * ((hy - 1) & 3) + 16 * ((hz - 1) & 3);
+            const int l = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
             const int nb = tbl[o];
-            float4 m = mom[(uint32_t)(nb < 0 ? 0 : nb) * BC + l];
+            float4 m = mom[(uint32_t)(BND && nb < 0 ? 0 : nb) * BC + l];
             m.x = m.x - 1.5f * (m.y * m.y + m.z * m.z + m.w * m.w);
-            if (nb < 0) m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (BND && nb < 0) m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             halo[e] = m;
         }
     });
@@ -320,6 +329,9 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     // it writes P_opp(i)); otherwise the block offset of population i of cell l.
     constexpr bool STAGED = PAR == 0 || !BND;
     constexpr bool ODD_STAGED = PAR == 1 && !BND;
+    // Float16 interior bricks of the regularised model: the decoded inputs are f / w and the collision writes the
+    // stored form directly (collide<1, false, true, true>).
+    constexpr bool FUSED16 = H16 && !BND && MODEL == 1;
     __shared__ int tbl[WAVES][32];
     __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
     __shared__ __align__(16) uint16_t rec_lds[ODD_STAGED ? kRecs : 1];
@@ -352,7 +364,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     T* stage = stage_all + (STAGED ? wave * Q * BC : 0);
     float4* halo = halo_all + (H16 ? wave * kHalo : 0);
     if constexpr (H16) {
-        load_halo(halo, tbl[wave], reinterpret_cast<const float4*>(a.vel_old), L);
+        load_halo<BND>(halo, tbl[wave], reinterpret_cast<const float4*>(a.vel_old), L);
         wave_sync();
     }
 
@@ -372,6 +384,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         constexpr int i = decltype(ic)::value;
         if constexpr (!H16) {
             return raw;
+        } else if constexpr (FUSED16) {
+            return h16_decode_g<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
         } else {
             if (!BND || ((mask >> i) & 1u))
                 return h16_decode<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
@@ -560,6 +574,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
             u[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
             u[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
             inlet_equilibrium(f, dr, u[0], u[1], u[2], p.t3);
+        } else if constexpr (FUSED16) {
+            collide<1, false, true, true>(f, p, dr, u, false, 0.0f, 0.0f, 0.0f, kH16Scale);
         } else {
             collide<MODEL, false>(f, p, dr, u);
         }
@@ -587,7 +603,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                 });
             }
         }
-        if constexpr (H16) {
+        if constexpr (H16 && !FUSED16) {
             // from here on f holds the stored form: the deviation from this cell's new equilibrium
             const float rp = dr - 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
             static_for<0, Q>([&](auto ic) {
