@@ -91,7 +91,7 @@ class BricksSizes(ctypes.Structure):
 class BricksStepArgs(ctypes.Structure):
     _fields_ = [("f", c_void_p), ("nbr", c_void_p), ("list", c_void_p), ("n_list", c_i64), ("bidx", c_void_p),
                 ("lmask", c_void_p), ("smask", c_void_p), ("qlink", c_void_p), ("qstart", c_void_p),
-                ("uin", c_void_p), ("side", c_void_p), ("sstart", c_void_p),
+                ("uin", c_void_p), ("side", c_void_p), ("sstart", c_void_p), ("loff", c_void_p), ("ldir", c_void_p),
                 ("om_cell", c_void_p), ("om_brick", c_void_p), ("omega", c_float), ("p_out", c_void_p),
                 ("drive", c_void_p), ("vel_old", c_void_p), ("vel", c_void_p), ("rho", c_void_p),
                 ("rho_old", c_void_p), ("n_slots", c_i64), ("force", c_float * 3), ("magic", c_float),
@@ -102,7 +102,8 @@ class BricksStepArgs(ctypes.Structure):
 
 class BricksConvertArgs(ctypes.Structure):
     _fields_ = [("f", c_void_p), ("nbr", c_void_p), ("bidx", c_void_p), ("lmask", c_void_p), ("smask", c_void_p),
-                ("side", c_void_p), ("sstart", c_void_p), ("slot_of_cell", c_void_p), ("n_cells", c_i64),
+                ("side", c_void_p), ("sstart", c_void_p), ("loff", c_void_p), ("slot_of_cell", c_void_p),
+                ("n_cells", c_i64),
                 ("f_ref", c_void_p),
                 ("parity", c_i32), ("storage", c_i32), ("vel", c_void_p), ("rho", c_void_p), ("n_slots", c_i64)]
 

@@ -107,8 +107,8 @@ def arena_bytes(bricks, records, storage, links=None, sides=None):
     sides = SIDES_PER_RECORD * records if sides is None else sides
     elem = 2 if storage == "fp16" else 4
     per_brick = BLOCK * elem + 27 * 4 + 4 + 1 + BC * 4 + 2 * 3 * BC * 4 + BC * 4 * (2 if storage == "fp16" else 1) + 4
-    per_record = BC * (4 + 4 + 3 * 4) + 2 * 4
-    return int(bricks) * per_brick + int(records) * per_record + 4 * (int(links) + int(sides))
+    per_record = BC * (4 + 4 + 3 * 4 + 4) + 2 * 4 + 2 * 4
+    return int(bricks) * per_brick + int(records) * per_record + 4 * (int(links) + int(sides) + 2)
 
 
 def case_bytes(layout, storage):
@@ -138,11 +138,13 @@ class Arena:
         self.mom = BrickMoments(self.S, batch.storage == "fp16", dev)
         self.lmask = z(self.cap_r * BC, torch.int32)
         self.smask = z(self.cap_r * BC, torch.int32)
-        self.qlink = z(self.cap_l)
+        self.qlink = z(self.cap_l + 1)                     # one element of padding: the kernels read ahead
         self.qstart = z(self.cap_r, torch.int32)
         self.uin = z(self.cap_r * 3 * BC)
-        self.side = z(self.cap_s)
+        self.side = z(self.cap_s + 1)
         self.sstart = z(self.cap_r, torch.int32)
+        self.loff = z(self.cap_r * BC, torch.int32)
+        self.ldir = z(self.cap_r * 2, torch.int32)
         self.n_open = z(batch.max_cases, torch.int32)       # flow-measuring cases of this arena
         self.bricks = _Ranges(self.cap_b)
         self.records = _Ranges(self.cap_r if records > 0 else 0)
@@ -495,6 +497,7 @@ class Batch:
         A.bidx, A.lmask, A.smask = arena.bidx.data_ptr(), arena.lmask.data_ptr(), arena.smask.data_ptr()
         A.qlink, A.qstart, A.uin = arena.qlink.data_ptr(), arena.qstart.data_ptr(), arena.uin.data_ptr()
         A.side, A.sstart = arena.side.data_ptr(), arena.sstart.data_ptr()
+        A.loff, A.ldir = arena.loff.data_ptr(), arena.ldir.data_ptr()
         A.om_cell, A.om_brick, A.omega = arena.om.data_ptr(), arena.om_brick.data_ptr(), 1.0
         A.p_out, A.drive = self.p_out.data_ptr(), self._drive_ptr()
         A.vel_old, A.vel, A.rho = arena.mom.step_ptrs(a)
@@ -710,8 +713,10 @@ class Case:
                 up(arena.uin[r0 * 3 * BC:(r0 + nr) * 3 * BC], lay["uin"].reshape(-1))
                 up(arena.qstart[r0:r0 + nr], (lay["qstart"][:-1].astype(np.int64) + q0).astype(np.int32))
                 up(arena.sstart[r0:r0 + nr], (lay["sstart"][:-1].astype(np.int64) + s0).astype(np.int32))
+                up(arena.loff[r0 * BC:(r0 + nr) * BC], lay["loff"].view(np.int32).reshape(-1))
+                up(arena.ldir[r0 * 2:(r0 + nr) * 2], lay["ldir"].view(np.int32).reshape(-1))
                 if lay["n_links"]:
-                    up(arena.qlink[q0:q0 + int(lay["n_links"])], lay["qlink"])
+                    up(arena.qlink[q0:q0 + int(lay["n_links"])], lay["qlink"][:int(lay["n_links"])])
                 if lay["n_side"]:
                     arena.side[s0:s0 + int(lay["n_side"])].zero_()
             c = cid
@@ -807,7 +812,7 @@ class Case:
         A = _native.BricksConvertArgs()
         A.f, A.nbr, A.bidx = ar.f.data_ptr(), ar.nbr.data_ptr(), ar.bidx.data_ptr()
         A.lmask, A.smask, A.side = ar.lmask.data_ptr(), ar.smask.data_ptr(), ar.side.data_ptr()
-        A.sstart = ar.sstart.data_ptr()
+        A.sstart, A.loff = ar.sstart.data_ptr(), ar.loff.data_ptr()
         A.slot_of_cell, A.n_cells = self.slot32.data_ptr(), self.N
         A.f_ref, A.parity = f_ref.data_ptr(), cur
         A.vel, A.rho = ar.mom.convert_ptrs(cur)

@@ -88,30 +88,41 @@ def build(lat):
 def compact_links(lay):
     """The per-step boundary data in compact form (only links that exist):
 
-    * ``qlink``: qc of every missing link of every boundary brick -- brick by brick, direction 1..26, then lane --
-      with ``qstart`` (n_boundary + 1) the start of each brick's run;
+    * ``qlink``: qc of every missing link of every boundary brick -- brick by brick, lane by lane, each lane's links
+      in direction order 1..26 -- with ``qstart`` (n_boundary + 1) the start of each brick's run;
     * the Bouzidi side copies (links with a bit in smask) are stored in the same order, starting at ``sstart``
-      (n_boundary + 1); ``n_side`` entries in all.
+      (n_boundary + 1); ``n_side`` entries in all;
+    * ``loff`` (n_boundary, 64) uint32: the start of each lane's run inside its brick's runs, qlink offset | side
+      offset << 16; link i of a lane sits at its offset + the number of its links below i (a popcount);
+    * ``ldir`` (n_boundary, 2) uint32: the union over the brick's cells of the missing links and of the smask bits.
 
-    A wavefront finds its lane's entry for direction i from the brick's start, the counts of the directions before
-    i and the lanes before it with the same link (ballot prefix counts)."""
+    ``qlink`` carries one element of padding after the last run: a lane reads its entry for every direction in
+    ``ldir``, also where its own link is not missing, and such an index is at most the end of its run."""
     nb = int(lay["n_boundary"])
     lm = lay["lmask"].reshape(nb, 64)
-    dirs = np.arange(27, dtype=np.uint32)
-    missing = ((lm[:, None, :] >> dirs[None, :, None]) & 1) == 0
-    missing &= (lm != 0)[:, None, :]
-    missing[:, 0, :] = False
-    qlink = np.ascontiguousarray(lay["qc"].reshape(nb, 27, 64)[missing], np.float32)
-    qstart = np.zeros(nb + 1, np.int64)
-    np.cumsum(missing.sum((1, 2)), out=qstart[1:])
     sm = lay["smask"].reshape(nb, 64)
-    sbits = ((sm[:, None, :] >> dirs[None, :, None]) & 1) == 1
-    sbits[:, 0, :] = False
+    links = np.uint32((1 << 27) - 2)                       # directions 1..26
+    miss = np.where(lm != 0, ~lm & links, 0).astype(np.uint32)
+    sbit = (sm & links).astype(np.uint32)
+    dirs = np.arange(27, dtype=np.uint32)
+    mT = ((miss[:, :, None] >> dirs[None, None, :]) & 1) == 1          # (nb, 64, 27): lane, then direction
+    sT = ((sbit[:, :, None] >> dirs[None, None, :]) & 1) == 1
+    qc = lay["qc"].reshape(nb, 27, 64).transpose(0, 2, 1)
+    qlink = np.zeros(int(mT.sum()) + 1, np.float32)
+    qlink[:-1] = qc[mT]
+    nq, ns = mT.sum(2).astype(np.int64), sT.sum(2).astype(np.int64)
+    qrel, srel = np.cumsum(nq, 1) - nq, np.cumsum(ns, 1) - ns
+    qstart = np.zeros(nb + 1, np.int64)
+    np.cumsum(nq.sum(1), out=qstart[1:])
     sstart = np.zeros(nb + 1, np.int64)
-    np.cumsum(sbits.sum((1, 2)), out=sstart[1:])
+    np.cumsum(ns.sum(1), out=sstart[1:])
     if qstart[-1] >= 2 ** 31 or sstart[-1] >= 2 ** 31:
         raise LayoutError("more than 2^31 boundary links")
+    loff = (qrel | (srel << 16)).astype(np.uint32)
+    ldir = np.stack([np.bitwise_or.reduce(miss, axis=1) if nb else np.zeros(0, np.uint32),
+                     np.bitwise_or.reduce(sbit, axis=1) if nb else np.zeros(0, np.uint32)], 1).astype(np.uint32)
     return {"qlink": qlink, "qstart": qstart.astype(np.int32), "sstart": sstart.astype(np.int32),
+            "loff": np.ascontiguousarray(loff), "ldir": np.ascontiguousarray(ldir.reshape(nb, 2)),
             "n_links": int(qstart[-1]), "n_side": int(sstart[-1])}
 
 

@@ -31,16 +31,11 @@ __device__ __forceinline__ uint32_t location(const cs_bricks_convert_args& a, in
     return brick_base(nbk) + kSlots.p[I][l];
 }
 
-// Position of the side copy of link i of lane L of boundary brick bb in the compact side array: after the copies
-// of the directions before i and of the lanes before L with link i (the order the step kernels count with ballots).
+// Position of the side copy of link i of lane L of boundary brick bb in the compact side array: the lane's run
+// (brick start + lane offset), then its side links below i (the order the step kernels use).
 __device__ uint32_t side_index(const cs_bricks_convert_args& a, int bb, int L, int i) {
-    uint32_t n = (uint32_t)a.sstart[bb];
-    const uint32_t below = (1u << i) - 2u;  // directions 1 .. i-1
-    for (int l = 0; l < BC; ++l) {
-        const uint32_t s = a.smask[(uint32_t)bb * BC + l];
-        n += (uint32_t)__popc(s & below) + ((l < L) ? ((s >> i) & 1u) : 0u);
-    }
-    return n;
+    const uint32_t s = a.smask[(uint32_t)bb * BC + L];
+    return (uint32_t)a.sstart[bb] + (a.loff[(uint32_t)bb * BC + L] >> 16) + (uint32_t)__popc(s & ((1u << i) - 1u));
 }
 
 // P_i(n) is written by n itself, so the FP16 form is encoded and decoded with n's own newest moments.
@@ -82,8 +77,9 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
     CS_REQUIRE(a && a->f && a->nbr && a->list && a->bidx && a->om_cell && a->om_brick && a->p_out && a->drive &&
                    a->vel_old && a->vel,
                "cs_bricks_step: null argument");
-    CS_REQUIRE(!a->boundary || (a->lmask && a->smask && a->qlink && a->qstart && a->uin && a->side && a->sstart),
-               "cs_bricks_step: boundary bricks need lmask/smask/qlink/qstart/uin/side/sstart");
+    CS_REQUIRE(!a->boundary || (a->lmask && a->smask && a->qlink && a->qstart && a->uin && a->side && a->sstart &&
+                                a->loff && a->ldir),
+               "cs_bricks_step: boundary bricks need lmask/smask/qlink/qstart/uin/side/sstart/loff/ldir");
     CS_REQUIRE(a->model == CS_MODEL_TRT || a->model == CS_MODEL_REGULARISED, "cs_bricks_step: unknown model %d",
                a->model);
     CS_REQUIRE(a->parity == 0 || a->parity == 1, "cs_bricks_step: parity %d", a->parity);
@@ -112,7 +108,7 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
 
 static int convert(const cs_bricks_convert_args* a, void* stream, bool set) {
     CS_REQUIRE(a && a->f && a->nbr && a->bidx && a->slot_of_cell && a->f_ref && a->lmask && a->smask && a->side &&
-                   a->sstart,
+                   a->sstart && a->loff,
                "cs_bricks convert: null argument");
     CS_REQUIRE(a->parity == 0 || a->parity == 1, "cs_bricks convert: parity %d", a->parity);
     CS_REQUIRE(a->storage == CS_STORAGE_FP32 || a->storage == CS_STORAGE_FP16, "cs_bricks convert: storage %d",

@@ -35,6 +35,7 @@ namespace cs {
 namespace {
 
 constexpr int BC = 64;
+constexpr int kLinkBatch = 8;  // boundary links evaluated per batch of loads (an even number: opposite pairs)
 constexpr uint32_t kInletBit = 1u << 27;
 constexpr uint32_t kAllLinks = (1u << Q) - 1;
 
@@ -143,11 +144,6 @@ static_assert(group_start(13) == 0 && group_size(13) == kOwn, "own group first")
 static_assert(stage_start(0) == kOwn, "records after the own group");
 __constant__ RecordTable kRecords = make_records();
 __constant__ SlotTable kOddPos = make_oddpos();
-
-// Number of lanes below this one with their bit set in the wave mask m.
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
 
 // Cross-lane LDS hand-over inside one wavefront.
 __device__ __forceinline__ void wave_sync() {
@@ -369,10 +365,14 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     }
 
     uint32_t mask = kAllLinks;
+    uint32_t sm = 0u, lo = 0u;  // boundary bricks: Bouzidi side bits and the lane's link-run offsets
     int bb = 0;
     if constexpr (BND) {
         bb = (int)li;  // the boundary list is in boundary-index order: bidx[list[li]] == li
+        // the per-cell boundary words are loaded up front, so the link loads later wait for one round trip only
         mask = a.lmask[(uint32_t)bb * BC + L];
+        sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
+        lo = a.loff[(uint32_t)bb * BC + L];
     }
     const bool live = mask != 0u;  // solid slots of boundary bricks only join the staging copies
 
@@ -444,27 +444,25 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                     uind[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
                     uind[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
                 }
-                const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
-                // compact link data: the lanes with missing link i (a ballot over the lanes active here, which are
-                // all lanes with any missing link) follow those of the directions before i
+                // compact link data: each lane's links in direction order from its offset in the brick's run, so
+                // link i sits at the offset + the number of the lane's links below i. Directions that no cell of
+                // the brick has (ldir, wave-uniform) are skipped; a lane that has not this link reads a neighbouring
+                // entry (at most the end of its run: the arrays carry one element of padding) and ignores it.
+                const uint32_t wq = a.ldir[2 * bb], ws = a.bouzidi ? a.ldir[2 * bb + 1] : 0u;
+                const uint32_t miss = ~mask & (kAllLinks - 1u);
                 const float* ql = a.qlink + a.qstart[bb];
                 const float* sd = a.side + a.sstart[bb];
-                uint32_t qoff = 0, soff = 0;
-                // links in batches of 8 (4 opposite pairs): first issue the batch's loads, then evaluate
-                static_for<0, 4>([&](auto bc) {
-                    constexpr int h0 = 1 + 8 * decltype(bc)::value;
-                    constexpr int nl = h0 + 8 <= Q ? 8 : Q - h0;
+                const uint32_t qo = lo & 0xFFFFu, so = lo >> 16;
+                // links in batches of kLinkBatch (opposite pairs): first issue the batch's loads, then evaluate
+                static_for<0, (Q - 1 + kLinkBatch - 1) / kLinkBatch>([&](auto bc) {
+                    constexpr int h0 = 1 + kLinkBatch * decltype(bc)::value;
+                    constexpr int nl = h0 + kLinkBatch <= Q ? kLinkBatch : Q - h0;
                     float qv[nl], pv[nl];
                     static_for<0, nl>([&](auto kc) {
                         constexpr int k = decltype(kc)::value;
-                        const bool miss = !((mask >> (h0 + k)) & 1u);
-                        const uint64_t bm = __ballot(miss);
-                        qv[k] = miss ? ql[qoff + lanes_below(bm)] : 0.0f;
-                        qoff += (uint32_t)__popcll(bm);
-                        const bool sb = (sm >> (h0 + k)) & 1u;
-                        const uint64_t bs = __ballot(sb);
-                        pv[k] = sb ? sd[soff + lanes_below(bs)] : 0.0f;
-                        soff += (uint32_t)__popcll(bs);
+                        constexpr uint32_t below = (1u << (h0 + k)) - 1u;
+                        qv[k] = ((wq >> (h0 + k)) & 1u) ? ql[qo + (uint32_t)__popc(miss & below)] : 0.0f;
+                        pv[k] = ((ws >> (h0 + k)) & 1u) ? sd[so + (uint32_t)__popc(sm & below)] : 0.0f;
                     });
                     if constexpr (GAL && h0 == 1)
                         static_for<0, 6>([&](auto kc) { q16[decltype(kc)::value] = qv[decltype(kc)::value]; });
@@ -590,16 +588,13 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
             a.rho[slot] = dr;
         }
         if constexpr (BND) {
-            const uint32_t sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
             if (sm) {
                 float* sd = a.side + a.sstart[bb];
-                uint32_t soff = 0;
+                const uint32_t so = lo >> 16, ws = a.ldir[2 * bb + 1];
                 static_for<1, Q>([&](auto ic) {
                     constexpr int i = decltype(ic)::value;
-                    const bool sb = (sm >> i) & 1u;
-                    const uint64_t bs = __ballot(sb);
-                    if (sb) sd[soff + lanes_below(bs)] = f[i];
-                    soff += (uint32_t)__popcll(bs);
+                    if ((ws >> i) & 1u)
+                        if ((sm >> i) & 1u) sd[so + (uint32_t)__popc(sm & ((1u << i) - 1u))] = f[i];
                 });
             }
         }

@@ -215,11 +215,14 @@ class _BricksBackend:
         self.boundary = up(layout["boundary"])
         self.lmask = up(layout["lmask"].view(np.int32)) if layout["n_boundary"] else torch.zeros(64, dtype=torch.int32, device=dev)
         self.smask = up(layout["smask"].view(np.int32)) if layout["n_boundary"] else torch.zeros(64, dtype=torch.int32, device=dev)
-        self.qlink = up(layout["qlink"]) if layout["n_links"] else torch.zeros(1, dtype=torch.float32, device=dev)
+        self.qlink = up(layout["qlink"])                        # padded by one element (kernels read ahead)
         self.qstart = up(layout["qstart"])
         self.sstart = up(layout["sstart"])
+        nbd = int(layout["n_boundary"])
+        self.loff = up(layout["loff"].view(np.int32)) if nbd else torch.zeros(64, dtype=torch.int32, device=dev)
+        self.ldir = up(layout["ldir"].view(np.int32)) if nbd else torch.zeros(2, dtype=torch.int32, device=dev)
         self.uin = up(layout["uin"]) if layout["n_boundary"] else torch.zeros(3 * 64, dtype=torch.float32, device=dev)
-        self.side = torch.zeros(max(1, layout["n_side"]), dtype=torch.float32, device=dev)
+        self.side = torch.zeros(layout["n_side"] + 1, dtype=torch.float32, device=dev)
         self.slot_of_cell = up(layout["slot_of_cell"].astype(np.int64))
         self.slot_of_cell32 = up(layout["slot_of_cell"])
         self.mom = BrickMoments(S, self.storage == _native.CS_STORAGE_FP16, dev)
@@ -243,6 +246,7 @@ class _BricksBackend:
         A.bidx, A.lmask, A.smask = self.bidx.data_ptr(), self.lmask.data_ptr(), self.smask.data_ptr()
         A.qlink, A.qstart, A.uin = self.qlink.data_ptr(), self.qstart.data_ptr(), self.uin.data_ptr()
         A.side, A.sstart = self.side.data_ptr(), self.sstart.data_ptr()
+        A.loff, A.ldir = self.loff.data_ptr(), self.ldir.data_ptr()
         A.om_cell, A.om_brick, A.omega = self.om.data_ptr(), self.om_brick.data_ptr(), self.omega
         A.p_out, A.drive = s.p_out.data_ptr(), s._drive_ptr()
         A.vel_old, A.vel, A.rho = self.mom.step_ptrs(a)
@@ -273,7 +277,7 @@ class _BricksBackend:
         A = _native.BricksConvertArgs()
         A.f, A.nbr, A.bidx = self.f.data_ptr(), self.nbr.data_ptr(), self.bidx.data_ptr()
         A.lmask, A.smask, A.side = self.lmask.data_ptr(), self.smask.data_ptr(), self.side.data_ptr()
-        A.sstart = self.sstart.data_ptr()
+        A.sstart, A.loff = self.sstart.data_ptr(), self.loff.data_ptr()
         A.slot_of_cell, A.n_cells = self.slot_of_cell32.data_ptr(), self.n_cells
         A.f_ref, A.parity = f_ref.data_ptr(), parity
         # the newest moments: those of the step that left this state (parity = the next step's parity)

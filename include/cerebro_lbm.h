@@ -230,11 +230,17 @@ typedef struct cs_bricks_step_args {
     const uint32_t* lmask;      /* [n_boundary][64] */
     const uint32_t* smask;      /* [n_boundary][64] */
     const float* qlink;         /* codes (q >= 0 wall, -1 inlet, -2-k outlet k) of the missing links only: per
-                                   boundary brick, direction 1..26, then lane */
+                                   boundary brick, lane by lane, each lane's links in direction order 1..26; one
+                                   element of padding after the last run */
     const int32_t* qstart;      /* [n_boundary + 1] start of each boundary brick's run in qlink */
     const float* uin;           /* [n_boundary][3][64] */
-    float* side;                /* copies of the Bouzidi q >= 1/2 partners (links with a bit in smask), same order */
+    float* side;                /* copies of the Bouzidi q >= 1/2 partners (links with a bit in smask), same order
+                                   and padding */
     const int32_t* sstart;      /* [n_boundary + 1] start of each boundary brick's run in side */
+    const uint32_t* loff;       /* [n_boundary][64] lane's run inside its brick's runs: qlink offset | side offset
+                                   << 16 (link i of the lane then sits at offset + number of its links below i) */
+    const uint32_t* ldir;       /* [n_boundary][2] union over the brick's cells of the missing links, of the smask
+                                   bits (directions no cell needs are skipped by the whole wavefront) */
     const float* om_cell;       /* [n_slots] per-cell 1 / tau, read only where om_brick != 0 */
     const uint8_t* om_brick;    /* [n_bricks] 1 = the brick's cells have their own omega */
     float omega;                /* 1 / tau of every other cell */
@@ -277,6 +283,7 @@ typedef struct cs_bricks_convert_args {
     const uint32_t* smask;
     float* side;                /* compact side copies (cs_bricks_step_args.side) */
     const int32_t* sstart;
+    const uint32_t* loff;
     const int32_t* slot_of_cell; /* [N] */
     int64_t n_cells;
     float* f_ref;               /* device (Q, N) */

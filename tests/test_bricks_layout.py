@@ -195,8 +195,9 @@ def test_odd_step_staging_maps_to_direct_slots():
 
 
 def test_compact_link_arrays():
-    """qlink / side order: brick by brick, direction 1..26, lane; qstart / sstart the runs. Checked against the dense
-    qc, lmask and smask."""
+    """qlink / side order: brick by brick, lane by lane, direction 1..26; qstart / sstart the brick runs, loff the lane
+    runs, ldir the per-brick unions; and the kernels' index (lane offset + popcount of the lane's links below i) finds
+    every entry. Checked against the dense qc, lmask and smask with plain loops."""
     lat = tilted_pipe()
     lay = bricks.build(lat)
     nb = lay["n_boundary"]
@@ -204,16 +205,27 @@ def test_compact_link_arrays():
     expect, sides = [], []
     starts, sstarts = [0], [0]
     for bb in range(nb):
-        for i in range(1, 27):
-            for l in range(64):
-                m = int(lay["lmask"][bb, l])
+        wq = ws = 0
+        for l in range(64):
+            m, sm = int(lay["lmask"][bb, l]), int(lay["smask"][bb, l])
+            off = int(lay["loff"][bb, l])
+            assert (off & 0xFFFF) == len(expect) - starts[-1] and (off >> 16) == len(sides) - sstarts[-1]
+            nq = ns = 0
+            for i in range(1, 27):
                 if m and not (m >> i) & 1:
+                    assert (off & 0xFFFF) + nq == len(expect) - starts[-1]
                     expect.append(qc[bb, i, l])
-                if (int(lay["smask"][bb, l]) >> i) & 1:
+                    nq += 1
+                    wq |= 1 << i
+                if (sm >> i) & 1:
+                    assert (off >> 16) + ns == len(sides) - sstarts[-1]
                     sides.append((bb, i, l))
+                    ns += 1
+                    ws |= 1 << i
+        assert lay["ldir"][bb].tolist() == [wq, ws]
         starts.append(len(expect))
         sstarts.append(len(sides))
-    assert np.array_equal(lay["qlink"], np.asarray(expect, np.float32))
+    assert np.array_equal(lay["qlink"][:-1], np.asarray(expect, np.float32)) and lay["qlink"][-1] == 0.0
     assert lay["qstart"].tolist() == starts and lay["sstart"].tolist() == sstarts
     assert lay["n_links"] == len(expect) and lay["n_side"] == len(sides)
     # every side link is a missing wall link with q >= 1/2
