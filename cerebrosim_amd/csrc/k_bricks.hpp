@@ -160,10 +160,21 @@ __device__ __forceinline__ void load_table(uint32_t* lds, const SlotTable& src, 
     }
 }
 
-// Move the footprint of brick b between global memory and the wave's LDS stage with full-row accesses.
+// Direct global -> LDS load of `size` bytes per lane (16 or 4): lane L's bytes land at lds + L * size. The LDS base
+// is wave-uniform; the data arrive without passing through registers (wait for vmcnt before reading them).
+template <int SIZE>
+__device__ __forceinline__ void load_to_lds(const void* src, void* lds) {
+    static_assert(SIZE == 16 || SIZE == 4, "16- or 4-byte lanes");
+    auto* dst = (__attribute__((address_space(3))) void*)lds;
+    if constexpr (SIZE == 16) __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+    else __builtin_amdgcn_global_load_lds(src, dst, 4, 0, 0);
+}
+
+// Move the footprint of brick b between global memory and the wave's LDS stage with full-row accesses. DMA: loads
+// go straight to LDS (interior bricks; boundary bricks keep the register copy, which times faster for them).
 // Even steps: the whole own block. Odd steps (interior bricks): the own group as 16-byte vectors, then the 26
-// neighbour records packed lane by lane (rec: the record table in LDS).
-template <int PAR, bool STORE, typename T>
+// neighbour records packed lane by lane (rec: the record table, in LDS or constant memory).
+template <int PAR, bool STORE, bool DMA, typename T>
 __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int* tbl, const uint16_t* rec,
                                            uint32_t base, int L) {
     if constexpr (PAR == 0) {
@@ -176,9 +187,11 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
             const int k = j * BC + L;
             if (NV % BC == 0 || (j + 1) * BC <= NV || k < NV) {
                 if constexpr (STORE) g[k] = l[k];
+                else if constexpr (DMA) load_to_lds<16>(g + k, l + j * BC);
                 else l[k] = g[k];
             }
         });
+        if constexpr (!STORE && DMA) __builtin_amdgcn_s_waitcnt(0);
     } else {
         static_assert(kOwn * sizeof(T) % 16 == 0, "own group in 16-byte vectors");
         constexpr int NV = kOwn * (int)sizeof(T) / 16;
@@ -189,6 +202,7 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
             const int k = j * BC + L;
             if (NV % BC == 0 || (j + 1) * BC <= NV || k < NV) {
                 if constexpr (STORE) g[k] = l[k];
+                else if constexpr (DMA) load_to_lds<16>(g + k, l + j * BC);
                 else l[k] = g[k];
             }
         });
@@ -199,9 +213,11 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
                 const uint32_t e = rec[p];
                 const uint32_t at = brick_base(tbl[e >> 11]) + (e & 2047u);
                 if constexpr (STORE) F[at] = stage[kOwn + p];
+                else if constexpr (DMA && sizeof(T) == 4) load_to_lds<4>(F + at, stage + kOwn + j * BC);
                 else stage[kOwn + p] = F[at];
             }
         });
+        if constexpr (!STORE && DMA) __builtin_amdgcn_s_waitcnt(0);
     }
 }
 
@@ -330,13 +346,17 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     constexpr bool FUSED16 = H16 && !BND && MODEL == 1;
     __shared__ int tbl[WAVES][32];
     __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
-    __shared__ __align__(16) uint16_t rec_lds[ODD_STAGED ? kRecs : 1];
+    // the odd-step record table: in LDS for float16 storage (whose registers limit the occupancy anyway), read from
+    // constant memory for float32 (five workgroups of four waves then fit the LDS of a compute unit)
+    constexpr bool REC_LDS = ODD_STAGED && H16;
+    __shared__ __align__(16) uint16_t rec_lds[REC_LDS ? kRecs : 1];
     __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
     __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
     const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
     load_table(slots32, ODD_STAGED ? kOddPos : kSlots, 64 * W);
-    if constexpr (ODD_STAGED)
+    if constexpr (REC_LDS)
         for (int k = threadIdx.x; k < kRecs; k += 64 * W) rec_lds[k] = kRecords.e[k];
+    const uint16_t* rec = REC_LDS ? rec_lds : kRecords.e;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int L = threadIdx.x & 63;
     // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
@@ -393,7 +413,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         }
     };
     if constexpr (STAGED) {
-        stage_copy<PAR, false>(F, stage, tbl[wave], rec_lds, base, L);
+        stage_copy<PAR, false, !BND>(F, stage, tbl[wave], rec, base, L);
         wave_sync();
         if (live)
             static_for<0, Q>([&](auto ic) {
@@ -635,7 +655,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                 stage[slt[io * BC + L]] = (T)f[i];
             });
         wave_sync();
-        stage_copy<PAR, true>(F, stage, tbl[wave], rec_lds, base, L);
+        stage_copy<PAR, true, !BND>(F, stage, tbl[wave], rec, base, L);
     }
 }
 
