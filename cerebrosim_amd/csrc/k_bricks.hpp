@@ -300,10 +300,12 @@ __device__ __forceinline__ void load_halo(float4* halo, const int* tbl, const fl
 //   fo        raw value of slot I: P_opp(n)
 //   opp_fluid link opp(I) is fluid; f_opp its pulled population f_in[opp I, n + c_I]
 //   p_self    P_I(n) of the previous step (for q >= 1/2)
-template <int I>
+// GW: the populations are given divided by w_I (so are the results): the inlet and outlet terms drop the weight.
+template <int I, bool GW = false>
 __device__ __forceinline__ float link_value(float code, float fo, bool opp_fluid, float f_opp, float p_self, int bouzidi,
                                             const float (&uind)[3], const float (&up)[3], float upsq,
                                             const float* p_out) {
+    constexpr float wI = GW ? 1.0f : Lat::w[I];
     if (code >= 0.0f) {
         float v = fo;
         if (bouzidi) {
@@ -319,11 +321,11 @@ __device__ __forceinline__ float link_value(float code, float fo, bool opp_fluid
         return v;
     }
     if (code > -1.5f) {  // inlet: bounce-back with the prescribed velocity
-        return fo + 6.0f * Lat::w[I] * cdot<I>(uind[0], uind[1], uind[2]);
+        return fo + 6.0f * wI * cdot<I>(uind[0], uind[1], uind[2]);
     }
     const int k = (int)(-2.0f - code);  // outlet k: anti-bounce-back at the prescribed pressure
     const float cu = cdot<I>(up[0], up[1], up[2]);
-    return -fo + 2.0f * Lat::w[I] * (3.0f * p_out[k] + 4.5f * cu * cu - 1.5f * upsq);
+    return -fo + 2.0f * wI * (3.0f * p_out[k] + 4.5f * cu * cu - 1.5f * upsq);
 }
 
 // One wavefront per brick, W bricks per workgroup. xcd: remap workgroup ids so that each XCD (which takes
@@ -341,9 +343,10 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     // it writes P_opp(i)); otherwise the block offset of population i of cell l.
     constexpr bool STAGED = PAR == 0 || !BND;
     constexpr bool ODD_STAGED = PAR == 1 && !BND;
-    // Float16 interior bricks of the regularised model: the decoded inputs are f / w and the collision writes the
-    // stored form directly (collide<1, false, true, true>).
-    constexpr bool FUSED16 = H16 && !BND && MODEL == 1;
+    // Float16 storage, regularised model: the decoded inputs are g = f / w, the boundary links are evaluated in that
+    // form (every formula is linear in the populations; the Bouzidi side copies stay actual populations), and the
+    // collision writes the stored form directly (collide<1, false, true, true>).
+    constexpr bool FUSED16 = H16 && MODEL == 1;
     __shared__ int tbl[WAVES][32];
     __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
     // the odd-step record table: in LDS for float16 storage (whose registers limit the occupancy anyway), read from
@@ -405,7 +408,9 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         if constexpr (!H16) {
             return raw;
         } else if constexpr (FUSED16) {
-            return h16_decode_g<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
+            if (!BND || ((mask >> i) & 1u))
+                return h16_decode_g<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
+            return h16_decode_g<Lat::opp[i]>(raw, halo[halo_index<0, 0, 0>(x, y, z)]);
         } else {
             if (!BND || ((mask >> i) & 1u))
                 return h16_decode<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
@@ -493,13 +498,15 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                         const bool fi = (mask >> i) & 1u, fj = (mask >> j) & 1u;
                         const float ri = f[i], rj = f[j];
                         float vi = ri, vj = rj;
+                        // side copies are actual populations: divided by w_i in the g form (w_i = w_j)
+                        constexpr float sw = FUSED16 ? 1.0f / Lat::w[i] : 1.0f;
                         if (!fi) {
-                            const float ps = ((sm >> i) & 1u) ? pv[k] : rj;
-                            vi = link_value<i>(qv[k], ri, fj, rj, ps, a.bouzidi, uind, up, upsq, p_out);
+                            const float ps = ((sm >> i) & 1u) ? pv[k] * sw : rj;
+                            vi = link_value<i, FUSED16>(qv[k], ri, fj, rj, ps, a.bouzidi, uind, up, upsq, p_out);
                         }
                         if (!fj) {
-                            const float ps = ((sm >> j) & 1u) ? pv[k + 1] : ri;
-                            vj = link_value<j>(qv[k + 1], rj, fi, ri, ps, a.bouzidi, uind, up, upsq, p_out);
+                            const float ps = ((sm >> j) & 1u) ? pv[k + 1] * sw : ri;
+                            vj = link_value<j, FUSED16>(qv[k + 1], rj, fi, ri, ps, a.bouzidi, uind, up, upsq, p_out);
                         }
                         f[i] = vi;
                         f[j] = vj;
@@ -585,7 +592,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                 p.F[0] = p.F[1] = p.F[2] = 0.0f;
                 p.g[0] = p.g[1] = p.g[2] = 0.0f;
             }
-            collide<MODEL, false>(f, p, dr, u, inlet_cell, ut[0], ut[1], ut[2]);
+            if constexpr (FUSED16) collide<1, false, true, true>(f, p, dr, u, inlet_cell, ut[0], ut[1], ut[2], kH16Scale);
+            else collide<MODEL, false>(f, p, dr, u, inlet_cell, ut[0], ut[1], ut[2]);
         } else if (inlet_cell) {
             dr = density_deviation(f);
             u[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
@@ -611,10 +619,20 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
             if (sm) {
                 float* sd = a.side + a.sstart[bb];
                 const uint32_t so = lo >> 16, ws = a.ldir[2 * bb + 1];
+                // FUSED16: f holds the stored form; P_i = w_i (f_i / scale + rho' + c.u (3 + 4.5 c.u))
+                const float rp = dr - 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
                 static_for<1, Q>([&](auto ic) {
                     constexpr int i = decltype(ic)::value;
                     if ((ws >> i) & 1u)
-                        if ((sm >> i) & 1u) sd[so + (uint32_t)__popc(sm & ((1u << i) - 1u))] = f[i];
+                        if ((sm >> i) & 1u) {
+                            float v = f[i];
+                            if constexpr (FUSED16) {
+                                const float cu = cdot<i>(u[0], u[1], u[2]);
+                                v = Lat::w[i] * __builtin_fmaf(v, 1.0f / kH16Scale,
+                                                               __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), rp));
+                            }
+                            sd[so + (uint32_t)__popc(sm & ((1u << i) - 1u))] = v;
+                        }
                 });
             }
         }

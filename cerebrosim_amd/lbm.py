@@ -329,7 +329,9 @@ class _BricksBackend:
 
 
 def sponge_tau(lat, tau0, length, tau_max, kinds=("inlet", "outlet")):
-    """Per-cell tau with sponge layers at the openings of the given kinds (lbm.py:398-416)."""
+    """Per-cell tau with sponge layers at the openings of the given kinds (lbm.py:398-416). The side test and the
+    smoothstep run only on the cells closer to the opening's plane than ``length`` (elsewhere the weight is 0 and tau
+    keeps its value); each cell's arithmetic is the full formula's."""
     P = lat["origin"] + lat["coords"] * lat["dx"]
     tau = np.full(lat["nbr"].shape[1], tau0)
     for o in lat["openings"]:
@@ -340,10 +342,14 @@ def sponge_tau(lat, tau0, length, tau_max, kinds=("inlet", "outlet")):
         rel = P - centre
         along = rel @ normal
         depth = -along / lat["dx"]
-        side = np.linalg.norm(rel - along[:, None] * normal, axis=1) < 1.5 * o["radius"]
-        w = np.clip(1.0 - depth / length, 0.0, 1.0)
+        near = np.nonzero(depth < length)[0] if length > 0 else np.arange(len(depth))
+        if not near.size:
+            continue
+        r, a, d = rel[near], along[near], depth[near]
+        side = np.linalg.norm(r - a[:, None] * normal, axis=1) < 1.5 * o["radius"]
+        w = np.clip(1.0 - d / length, 0.0, 1.0)
         w = w * w * (3.0 - 2.0 * w)
-        tau = np.maximum(tau, np.where(side, tau0 + (tau_max - tau0) * w, tau0))
+        tau[near] = np.maximum(tau[near], np.where(side, tau0 + (tau_max - tau0) * w, tau0))
     return tau
 
 
