@@ -21,7 +21,8 @@ The chosen forms are measured after the headline (``--forms``, default all three
 
 Prints one JSON line. ``roofline.achieved`` = algorithmic bytes of the step kernel per launch (244 B per fluid cell:
 27 populations read + 27 written x 4 B, previous velocity read 12 B, velocity + density written 16 B) / the kernel's
-mean duration measured with HIP events on the solver's stream. ``cpu_baseline`` times the oracle (the C restatement
+mean duration measured with HIP events on the solver's stream; ``copy_gbs`` is a 1 GiB device copy's (read + write)
+bandwidth on the same GPU, the achievable ceiling beside the 8 TB/s peak. ``cpu_baseline`` times the oracle (the C restatement
 of the reference step, FP32, all host threads) on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -78,6 +79,27 @@ def kernel_times_ms(s, steps=40):
         s.steps += 1
     st.synchronize()
     return {name: (float(np.mean([a.elapsed_time(b) for a, b in ev[name]])), cells) for name, cells in parts}
+
+
+def copy_bandwidth_gbs(nbytes=1 << 30, reps=10):
+    """Achievable HBM bandwidth on this device: a device-to-device copy of nbytes, (read + write) bytes over the
+    mean duration (HIP events), for comparison with the 8 TB/s peak."""
+    import torch
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    secs = e0.elapsed_time(e1) * 1e-3 / reps
+    del x, y
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / secs / 1e9
 
 
 def cpu_baseline(lat, steps):
@@ -338,6 +360,8 @@ def main():
                      "bytes_per_cell": bpl,
                      "launches_ms": {k: round(v[0], 5) for k, v in kt.items()}},
     }
+    copy_gbs = copy_bandwidth_gbs()
+    out["roofline"].update({"copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)})
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(lat, args.cpu_steps)
     forms = [f for f in args.forms.split(",") if f and f != "none"]
