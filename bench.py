@@ -271,7 +271,8 @@ def form_serving(full, rate=None):
             "requests": n, "rate_per_s": rate, "storage": "fp16", "chunk": rep["chunk"],
             "cases_per_hour": round(rep["cases_per_hour"], 1), "p50_s": round(rep["p50_s"], 3),
             "p99_s": round(rep["p99_s"], 3), "wall_s": round(rep["wall_s"], 2), "mlups": round(rep["mlups"], 1),
-            "steady_cases_per_hour": round(steady, 1),
+            "steady_cases_per_hour": round(steady, 1), "gpu_chunk_s": round(rep["gpu_chunk_s"], 2),
+            "host_s": rep["host_s"],
             "ok": ok, "sequential_fp32_cases_per_hour": round(seq["cases_per_hour"], 1),
             "sequential_fp32_wall_s": round(seq["wall_s"], 2),
             "speedup_vs_sequential": round(rep["cases_per_hour"] / seq["cases_per_hour"], 2)}

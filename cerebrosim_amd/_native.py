@@ -85,7 +85,8 @@ class LatticeIn(ctypes.Structure):
 
 
 class BricksSizes(ctypes.Structure):
-    _fields_ = [("n_bricks", c_i64), ("n_boundary", c_i64), ("n_interior", c_i64), ("n_slots", c_i64)]
+    _fields_ = [("n_bricks", c_i64), ("n_boundary", c_i64), ("n_interior", c_i64), ("n_slots", c_i64),
+                ("n_links", c_i64), ("n_side", c_i64)]
 
 
 class BricksStepArgs(ctypes.Structure):
@@ -97,7 +98,7 @@ class BricksStepArgs(ctypes.Structure):
                 ("rho_old", c_void_p), ("n_slots", c_i64), ("force", c_float * 3), ("magic", c_float),
                 ("bouzidi", c_i32), ("model", c_i32), ("galilean", c_i32), ("parity", c_i32), ("boundary", c_i32),
                 ("storage", c_i32), ("case_omega", c_void_p), ("case_force", c_void_p), ("p_out_stride", c_i32),
-                ("variant", c_i32)]
+                ("variant", c_i32), ("brec", c_void_p)]
 
 
 class BricksConvertArgs(ctypes.Structure):

@@ -7,11 +7,13 @@ Design (MI355X-first):
   bricks and a contiguous range of boundary records from first-fit allocators; their neighbour ids are offset by
   the range start on upload, so the kernels address the arena exactly as they address one lattice.
 * The brick lists of an arena (interior, boundary) hold ``brick | case << 22``; the kernel reads the case's drive,
-  base omega, body force and opening pressures from per-case tables. The boundary list is indexed by boundary
-  record (unused records hold a skip marker), so a boundary brick's record index is its list position.
-* One step = one clock launch for all cases + two step launches per arena (interior, boundary) + one flow launch
-  per arena when cases measure flows (simulate.py's k_flux + k_bc). ``run`` captures chunks of steps in a graph,
-  re-captured only when cases join or leave.
+  base omega, body force and opening pressures from per-case tables. Both lists hold live entries only; beside the
+  boundary list, ``brec`` gives each entry's boundary record (records keep their places while cases come and go).
+* One step = one clock launch for all cases + two step launches per arena (interior, boundary; the boundary ones on
+  a side stream, concurrent with the interior ones) + one flow launch per arena when cases measure flows
+  (simulate.py's k_flux + k_bc). ``run`` captures chunks of steps in a graph. The launches cover the live extent
+  of each list, so a graph is captured again when cases join or leave (a capture costs about as much as launching
+  the chunk's steps one by one).
 * An arena holds at most 2^32 / 1728 - 1 bricks (32-bit population offsets, about 159 M cells); a batch holds
   several arenas (one storage type each). Cases are independent, so results do not depend on what else is in the
   batch: the tests check that a case gives the same bits alone and in a mix.
@@ -23,11 +25,12 @@ resistance-outlet flow measurement of ``artery.ArterySolver`` (``set_outlets``, 
 from __future__ import annotations
 
 import ctypes
+import time
 
 import numpy as np
 
 from . import _native
-from .lbm import BrickMoments, _require_gpu, sponge_tau
+from .lbm import BrickMoments, StreamFork, _require_gpu, sponge_tau
 
 BC = 64
 BLOCK = 27 * BC
@@ -151,7 +154,7 @@ class Arena:
         self.links = _Ranges(self.cap_l)
         self.sides = _Ranges(self.cap_s)
         self.cases = []
-        self.interior = self.boundary = None
+        self.interior = self.boundary = self.brec = None
         self.n_interior = self.n_boundary = 0
         self.flux = None
         self.dirty = True
@@ -181,8 +184,8 @@ class Arena:
 
     def rebuild_lists(self):
         """Device lists from the cases now in the arena. The list buffers have power-of-two capacities and hold skip
-        entries past the live ones, and the launches always cover the whole capacity, so a captured graph stays valid
-        when cases join or leave. Returns True when a buffer had to grow (the graphs must be captured again)."""
+        entries past the live ones; the launches cover the live entries (``n_interior``, ``n_boundary``). Returns
+        True when a buffer had to grow."""
         torch = _torch()
         dev = self.batch.dev
         shift = _native.CS_CASE_SHIFT
@@ -190,13 +193,15 @@ class Arena:
         inner = [(c.lay["interior"].astype(np.uint32) + np.uint32(c.b0)) | np.uint32(c.cid << shift)
                  for c in self.cases]
         inner = np.concatenate(inner) if inner else np.zeros(0, np.uint32)
-        hi = self.records.high() if self.records.capacity else 0
-        bnd = np.full(hi, skip, np.uint32)
+        # boundary list: live entries only, with the record of each entry beside it (brec), so that free records
+        # between cases cost no launched workgroups
+        bnd, brec = [np.zeros(0, np.uint32)], [np.zeros(0, np.int32)]
         for c in self.cases:
             nbd = int(c.lay["n_boundary"])
             if nbd:
-                bnd[c.r0:c.r0 + nbd] = ((c.lay["boundary"].astype(np.uint32) + np.uint32(c.b0)) |
-                                        np.uint32(c.cid << shift))
+                bnd.append((c.lay["boundary"].astype(np.uint32) + np.uint32(c.b0)) | np.uint32(c.cid << shift))
+                brec.append(np.arange(c.r0, c.r0 + nbd, dtype=np.int32))
+        bnd, brec = np.concatenate(bnd), np.concatenate(brec)
         grew = False
 
         def put(buf, data, dtype, fill):
@@ -212,6 +217,7 @@ class Arena:
 
         self.interior = put(self.interior, inner, torch.int32, skip)
         self.boundary = put(self.boundary, bnd, torch.int32, skip)
+        self.brec = put(self.brec, brec, torch.int32, 0)
         self.n_interior, self.n_boundary = len(inner), len(bnd)
         # flow-measurement lists (cases with openings under measurement); once present, the flow launch stays in the
         # step (cases without measurement have n_open 0 and their workgroups return at once)
@@ -255,11 +261,12 @@ class Batch:
 
     storage "fp32" / "fp16"; model, magic, bouzidi, galilean as in ``lbm.Solver`` (shared by all cases). Arenas
     are created on demand with ``arena_bricks`` / ``arena_records`` capacity while the total stays within
-    ``budget_bytes`` (default: the device's free memory less 2 GiB)."""
+    ``budget_bytes`` (default: the device's free memory less 2 GiB). ``staging_bytes``: the pinned host ring that
+    uploads go through; the host waits only when it wraps onto copies still queued behind the steps in flight."""
 
     def __init__(self, storage="fp32", model="regularised", magic=3.0 / 16.0, bouzidi=True, galilean=None,
                  device=None, arena_bricks=65536, arena_records=16384, budget_bytes=None, max_cases=256,
-                 max_open=8, max_harmonics=16, variant=0):
+                 max_open=8, max_harmonics=16, variant=0, staging_bytes=64 << 20):
         torch = _torch()
         if storage not in STORAGES:
             raise ValueError(f"unknown storage {storage!r}")
@@ -280,6 +287,7 @@ class Batch:
         self.variant = int(variant)
         self._tstream = torch.cuda.Stream(device=self.dev)          # default priority
         self.stream_handle = ctypes.c_void_p(self._tstream.cuda_stream)
+        self._fork = StreamFork(self._tstream, self.dev)
         M, K = self.max_cases, self.max_open
         with torch.cuda.stream(self._tstream):
             z = lambda shape, d=torch.float32: torch.zeros(shape, dtype=d, device=self.dev)
@@ -300,7 +308,7 @@ class Batch:
             self._work = z(_native.CS_REDUCE_WORK, torch.float64)
             self._red = z(2, torch.float64)
         self._tstream.synchronize()
-        self._ring = torch.empty(64 << 20, dtype=torch.uint8, pin_memory=True)   # upload staging
+        self._ring = torch.empty(int(staging_bytes), dtype=torch.uint8, pin_memory=True)   # upload staging
         self._ring_np = self._ring.numpy()
         self._ring_pos, self._ring_busy, self._ring_pending = 0, [], []
         self.arenas = []
@@ -309,6 +317,9 @@ class Batch:
         self.cur = 0
         self.steps = 0
         self.graphs = {}
+        self._retired = []                 # (graphs, event after their last replay) awaiting destruction
+        self.captures, self.capture_s = 0, 0.0
+        self._args_key, self._args = None, {}
 
     def _stage(self, x):
         """A pinned host tensor holding x: a region of the batch's staging ring, reused once the copies that last
@@ -430,14 +441,16 @@ class Batch:
         return c
 
     def remove(self, case):
-        """Release a case's bricks, records and slot."""
+        """Release a case's bricks, records and slot. Stream-ordered, without waiting for the device: the launches and
+        copies already queued (the case's last steps, reads of its results) run before anything that reuses its
+        ranges or its slot (the uploads of a new case, the rebuilt lists). Only an arena left empty is freed after a
+        synchronisation."""
         torch = _torch()
-        self.synchronize()
         a = case.arena
         a.cases.remove(case)
         a.release(case.lay, case.spot)
         with torch.cuda.stream(self._tstream):
-            self.period[case.cid] = 0
+            self.period[case.cid:case.cid + 1].zero_()     # a fill kernel: no wait for the queued steps
         del self.case_by_id[case.cid]
         case.arena = None
         self._topology_changed(a)
@@ -445,7 +458,6 @@ class Batch:
             self.synchronize()
             self.arenas.remove(a)
             self.topo += 1
-        self.synchronize()
 
     def _topology_changed(self, arena):
         """The arena's membership changed: its lists are rebuilt before the next launch."""
@@ -493,7 +505,7 @@ class Batch:
         A = _native.BricksStepArgs()
         lst = arena.boundary if boundary else arena.interior
         A.f, A.nbr, A.list = arena.f.data_ptr(), arena.nbr.data_ptr(), lst.data_ptr()
-        A.n_list = lst.numel()           # the whole capacity: entries past the live ones are skips
+        A.n_list = arena.n_boundary if boundary else arena.n_interior    # the live extent of the list
         A.bidx, A.lmask, A.smask = arena.bidx.data_ptr(), arena.lmask.data_ptr(), arena.smask.data_ptr()
         A.qlink, A.qstart, A.uin = arena.qlink.data_ptr(), arena.qstart.data_ptr(), arena.uin.data_ptr()
         A.side, A.sstart = arena.side.data_ptr(), arena.sstart.data_ptr()
@@ -506,6 +518,7 @@ class Batch:
         A.parity, A.boundary = a, int(boundary)
         A.case_omega, A.case_force = self.omega.data_ptr(), self.force.data_ptr()
         A.p_out_stride, A.variant = self.max_open, self.variant
+        A.brec = arena.brec.data_ptr() if boundary else None
         return A
 
     def _flux(self, arena, a):
@@ -537,13 +550,40 @@ class Batch:
         if A.n_list:
             _native.call("cs_bricks_step", ctypes.byref(A), self.stream_handle)
 
+    def _step_launches(self, a):
+        """[(arena, boundary, args)] of the step launches of parity a, with the drive pointer of table row self._k.
+        The argument blocks are kept while the arenas, their list extents and the variant stay the same (graph
+        captures then cost little more than the launches themselves)."""
+        key = (self.topo, self.variant, tuple((id(ar), ar.n_interior, ar.n_boundary,
+                                               ar.interior.data_ptr() if ar.interior is not None else 0,
+                                               ar.boundary.data_ptr() if ar.boundary is not None else 0,
+                                               ar.brec.data_ptr() if ar.brec is not None else 0)
+                                              for ar in self.arenas))
+        if key != self._args_key:
+            self._args_key, self._args = key, {}
+        got = self._args.get(a)
+        if got is None:
+            got = [(ar, boundary, self.step_args(ar, a, boundary)) for ar in self.arenas for boundary in (False, True)]
+            got = self._args[a] = [x for x in got if x[2].n_list]
+        drive = self._drive_ptr()
+        for _, _, A in got:
+            A.drive = drive
+        return got
+
     def _launch(self, a):
-        """The launches of one step of parity a (drives at table row self._k)."""
+        """The launches of one step of parity a (drives at table row self._k). The brick launches of a step are
+        independent (as in lbm._BricksBackend.step; arenas share nothing), so the boundary bricks run on a side
+        stream concurrently with the interior bricks; the flow launches read the step's velocities after the join."""
+        launches = self._step_launches(a)
+        forked = any(b for _, b, _ in launches) and not all(b for _, b, _ in launches)
+        if forked:
+            self._fork.fork()
+        for ar, boundary, A in launches:
+            st = self._fork.handle if forked and boundary else self.stream_handle
+            _native.call("cs_bricks_step", ctypes.byref(A), st)
+        if forked:
+            self._fork.join()
         for ar in self.arenas:
-            for boundary in (False, True):
-                A = self.step_args(ar, a, boundary)
-                if A.n_list:
-                    _native.call("cs_bricks_step", ctypes.byref(A), self.stream_handle)
             if ar.flux is not None:
                 self._flux(ar, a)
 
@@ -556,8 +596,19 @@ class Batch:
         full, rest = divmod(int(steps), chunk)
         if full:
             self._grow_table(chunk)
-            key = (self.cur, chunk, self.variant, self.topo)
+            # the launches cover the live extent of each list (skipped entries still cost a workgroup dispatch each),
+            # so a graph is captured for each set of extents; graphs of other extents are retired and destroyed once
+            # the device has passed their last replay
+            sizes = tuple((ar.n_interior, ar.n_boundary) for ar in self.arenas)
+            key = (self.cur, chunk, self.variant, self.topo, sizes)
+            self._purge_retired()
             if key not in self.graphs:
+                stale = [k for k in self.graphs if k[3:] != key[3:]]
+                if stale:
+                    ev = _torch().cuda.Event()
+                    ev.record(self._tstream)
+                    self._retired.append(([self.graphs.pop(k) for k in stale], ev))
+                tc = time.perf_counter()
                 with _native.capture(self.stream_handle) as cap:
                     self._clock_table(chunk)
                     a = self.cur
@@ -566,6 +617,8 @@ class Batch:
                         self._launch(a)
                         a = 1 - a
                 self.graphs[key] = cap.graph
+                self.capture_s += time.perf_counter() - tc
+                self.captures += 1
             g = self.graphs[key]
             for _ in range(full):
                 g.launch(self.stream_handle)
@@ -577,6 +630,9 @@ class Batch:
         self.steps += int(steps)
         for c in self.case_by_id.values():
             c.steps += int(steps)
+
+    def _purge_retired(self):
+        self._retired = [(gs, ev) for gs, ev in self._retired if not ev.query()]
 
     def synchronize(self):
         self._tstream.synchronize()
@@ -676,6 +732,25 @@ class SetupRecorder:
             getattr(case, name)(*a, **k)
 
 
+class PendingFields:
+    """(pressure, velocity) of a case on their way to pinned host memory (``Case.read_async``)."""
+
+    def __init__(self, event, rho, u, device_copies):
+        self.event, self._rho, self._u = event, rho, u
+        self._keep = device_copies           # the gathered device tensors stay referenced until the copy is done
+
+    def ready(self):
+        return self.event.query()
+
+    def result(self):
+        """Wait for the copies; (pressure, velocity) as ``Case.fields`` returns them, and whether every density
+        is finite (``Case.ok`` over the fluid cells)."""
+        self.event.synchronize()
+        self._keep = None
+        rho = self._rho.numpy()
+        return (rho / 3.0, self._u.numpy()), bool(np.isfinite(rho).all())
+
+
 class Case:
     """One lattice inside a Batch, with the lbm.Solver surface (lbm.py:346-455) and the resistance-outlet flow
     measurement of simulate.py (ArterySolver)."""
@@ -719,10 +794,12 @@ class Case:
                     up(arena.qlink[q0:q0 + int(lay["n_links"])], lay["qlink"][:int(lay["n_links"])])
                 if lay["n_side"]:
                     arena.side[s0:s0 + int(lay["n_side"])].zero_()
+            # per-case table entries by fill kernels (an indexed assignment of a Python number copies it from host
+            # memory, which waits for the work queued on the stream)
             c = cid
-            batch.tick[c] = 0
-            batch.drive[:, c] = 0.0
-            batch.omega[c] = float(np.float32(1.0 / self.tau))
+            batch.tick[c:c + 1].zero_()
+            batch.drive[:, c].zero_()
+            batch.omega[c:c + 1].fill_(float(np.float32(1.0 / self.tau)))
             batch.force[c].zero_()
             batch.p_out[c].zero_()
             batch.q_last[c].zero_()
@@ -797,6 +874,22 @@ class Case:
         self.batch.synchronize()
         n = max(1, self.n_open, self.lat["n_outlets"])
         return self.batch.p_out[self.cid, :n].cpu().numpy()
+
+    def read_async(self):
+        """Queue the copy of the case's (pressure, velocity) to pinned host memory behind the work in flight and
+        return a ``PendingFields`` (``ready()``, ``result()``) without waiting. The case may leave the batch right
+        after: the copies run before anything that reuses its ranges."""
+        torch = _torch()
+        bt = self.batch
+        with torch.cuda.stream(bt._tstream):
+            rho, u = self.arena.mom.gather(bt.cur, self.slot_dev)
+            rh = torch.empty(rho.shape, dtype=rho.dtype, pin_memory=True)
+            uh = torch.empty(u.shape, dtype=u.dtype, pin_memory=True)
+            rh.copy_(rho, non_blocking=True)
+            uh.copy_(u, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(bt._tstream)
+        return PendingFields(ev, rh, uh, (rho, u))
 
     def fields(self):
         """(pressure, velocity) per cell in the lattice's order."""

@@ -21,9 +21,14 @@ ARRAYS = {
     "cell_of_slot": (np.int32, lambda z, N: (z.n_slots,)),
     "lmask": (np.uint32, lambda z, N: (z.n_boundary, 64)),
     "smask": (np.uint32, lambda z, N: (z.n_boundary, 64)),
-    "qc": (np.float32, lambda z, N: (z.n_boundary, 27, 64)),
     "uin": (np.float32, lambda z, N: (z.n_boundary, 3, 64)),
+    "qlink": (np.float32, lambda z, N: (z.n_links + 1,)),
+    "qstart": (np.int32, lambda z, N: (z.n_boundary + 1,)),
+    "sstart": (np.int32, lambda z, N: (z.n_boundary + 1,)),
+    "loff": (np.uint32, lambda z, N: (z.n_boundary, 64)),
+    "ldir": (np.uint32, lambda z, N: (z.n_boundary, 2)),
 }
+DENSE = {"qc": (np.float32, lambda z, N: (z.n_boundary, 27, 64))}
 
 
 class LayoutError(ValueError):
@@ -48,9 +53,11 @@ def infer_periodic(lat):
     return periodic
 
 
-def build(lat):
-    """Brick layout of `lat` as a dict of numpy arrays plus the sizes. Raises LayoutError when the lattice cannot be
-    represented (then the indexed layout serves it)."""
+def build(lat, dense=False):
+    """Brick layout of `lat` as a dict of numpy arrays plus the sizes, all from the native builder (which runs
+    outside the GIL). dense: also return ``qc``, the link codes of every boundary brick as (n_boundary, 27, 64)
+    (the step reads only their compact form, ``qlink``). Raises LayoutError when the lattice cannot be represented
+    (then the indexed layout serves it)."""
     N = int(lat["nbr"].shape[1])
     coords = np.ascontiguousarray(lat["coords"], np.int32)
     nbr = np.ascontiguousarray(lat["nbr"], np.int32)
@@ -73,20 +80,20 @@ def build(lat):
         z = _native.BricksSizes()
         _native.call("cs_bricks_sizes_of", h, ctypes.byref(z))
         out = {"n_bricks": z.n_bricks, "n_boundary": z.n_boundary, "n_interior": z.n_interior,
-               "n_slots": z.n_slots, "n_cells": N, "periodic": periodic}
-        for name, (dt, shape) in ARRAYS.items():
+               "n_slots": z.n_slots, "n_cells": N, "periodic": periodic, "n_links": z.n_links, "n_side": z.n_side}
+        for name, (dt, shape) in (dict(ARRAYS, **DENSE) if dense else ARRAYS).items():
             arr = np.empty(shape(z, N), dt)
             if arr.size:
                 _native.call("cs_bricks_copy", h, name.encode(), ctypes.c_void_p(arr.ctypes.data))
             out[name] = arr
     finally:
         lib.cs_bricks_free(h)
-    out.update(compact_links(out))
     return out
 
 
 def compact_links(lay):
-    """The per-step boundary data in compact form (only links that exist):
+    """Numpy restatement of the native builder's compact boundary data (the tests' checker; needs ``build(lat,
+    dense=True)``). The per-step boundary data in compact form (only links that exist):
 
     * ``qlink``: qc of every missing link of every boundary brick -- brick by brick, lane by lane, each lane's links
       in direction order 1..26 -- with ``qstart`` (n_boundary + 1) the start of each brick's run;

@@ -26,7 +26,60 @@ struct BrickHost {
     std::vector<int32_t> coords, nbr, bidx, interior, boundary, slot_of_cell, cell_of_slot;
     std::vector<uint32_t> lmask, smask;
     std::vector<float> qc, uin;
+    // compact boundary link data (cs_bricks_copy "qlink" ... "ldir")
+    int64_t n_links = 0, n_side = 0;
+    std::vector<float> qlink;
+    std::vector<int32_t> qstart, sstart;
+    std::vector<uint32_t> loff, ldir;
 };
+
+// The per-step boundary data in compact form, only for links that exist: qlink holds qc of every missing link,
+// brick by brick, lane by lane, each lane's links in direction order 1..26 (plus one element of padding, since a
+// lane reads its entry for every direction of its brick's union, also where its own link is not missing);
+// qstart / sstart are each brick's run start in qlink / in the Bouzidi side copies (links with an smask bit, same
+// order); loff is each lane's run offset inside its brick's runs (qlink offset | side offset << 16), so link i of a
+// lane is at its offset + the number of its links below i; ldir is the per-brick union of the missing links and of
+// the side links.
+int compact_links(BrickHost& h) {
+    constexpr uint32_t kLinks = (1u << Q) - 2u;  // directions 1..26
+    const int64_t nbb = h.nbb;
+    h.qstart.assign(nbb + 1, 0);
+    h.sstart.assign(nbb + 1, 0);
+    h.loff.assign(nbb * BC, 0u);
+    h.ldir.assign(nbb * 2, 0u);
+    int64_t nq = 0;
+    for (int64_t k = 0; k < nbb * BC; ++k) {
+        const uint32_t m = h.lmask[k];
+        nq += __builtin_popcount(m ? (~m & kLinks) : 0u);
+    }
+    h.qlink.assign(nq + 1, 0.0f);
+    int64_t q = 0, s = 0;
+    for (int64_t bb = 0; bb < nbb; ++bb) {
+        CS_REQUIRE(q < (int64_t(1) << 31) && s < (int64_t(1) << 31), "cs_bricks_build: more than 2^31 boundary links");
+        h.qstart[bb] = (int32_t)q;
+        h.sstart[bb] = (int32_t)s;
+        uint32_t wq = 0u, ws = 0u;
+        for (int l = 0; l < BC; ++l) {
+            const uint32_t m = h.lmask[bb * BC + l];
+            const uint32_t miss = m ? (~m & kLinks) : 0u;
+            const uint32_t sb = h.smask[bb * BC + l] & kLinks;
+            h.loff[bb * BC + l] = (uint32_t)(q - h.qstart[bb]) | ((uint32_t)(s - h.sstart[bb]) << 16);
+            for (int i = 1; i < Q; ++i)
+                if ((miss >> i) & 1u) h.qlink[q++] = h.qc[(bb * Q + i) * BC + l];
+            s += __builtin_popcount(sb);
+            wq |= miss;
+            ws |= sb;
+        }
+        h.ldir[2 * bb] = wq;
+        h.ldir[2 * bb + 1] = ws;
+    }
+    CS_REQUIRE(q < (int64_t(1) << 31) && s < (int64_t(1) << 31), "cs_bricks_build: more than 2^31 boundary links");
+    h.qstart[nbb] = (int32_t)q;
+    h.sstart[nbb] = (int32_t)s;
+    h.n_links = q;
+    h.n_side = s;
+    return CS_OK;
+}
 
 uint64_t spread3(uint64_t v) {
     // interleave the low 21 bits of v with two zero bits between each
@@ -213,6 +266,7 @@ extern "C" int cs_bricks_build(const cs_lattice_in* in, void** handle) {
     int rc;
     try {
         rc = cs::build(in, *h);
+        if (rc == CS_OK) rc = cs::compact_links(*h);
     } catch (const std::bad_alloc&) {
         cs::set_error("cs_bricks_build: out of host memory");
         rc = CS_E_NOMEM;
@@ -232,6 +286,8 @@ extern "C" int cs_bricks_sizes_of(const void* handle, cs_bricks_sizes* out) {
     out->n_boundary = h->nbb;
     out->n_interior = h->nbi;
     out->n_slots = h->nb * cs::BC;
+    out->n_links = h->n_links;
+    out->n_side = h->n_side;
     return CS_OK;
 }
 
@@ -254,6 +310,11 @@ extern "C" int cs_bricks_copy(const void* handle, const char* name, void* dst) {
     if (n == "smask") return put(h->smask);
     if (n == "qc") return put(h->qc);
     if (n == "uin") return put(h->uin);
+    if (n == "qlink") return put(h->qlink);
+    if (n == "qstart") return put(h->qstart);
+    if (n == "sstart") return put(h->sstart);
+    if (n == "loff") return put(h->loff);
+    if (n == "ldir") return put(h->ldir);
     cs::set_error("cs_bricks_copy: unknown array '%s'", name);
     return CS_E_ARG;
 }

@@ -356,15 +356,23 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
     __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
     const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
+    // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
+    const int per = gridDim.x >> 3;
+    const int wg = xcd ? (int)((blockIdx.x & 7) * per + (blockIdx.x >> 3)) : (int)blockIdx.x;
+    // a workgroup whose entries are all skips (the free capacity of a batch's lists) returns before any set-up; the
+    // test is uniform over the workgroup, so no barrier is left waiting
+    bool any = false;
+    for (int w = 0; w < WAVES; ++w) {
+        const int64_t lw = (int64_t)wg * WAVES + w;
+        any |= lw < a.n_list && a.list[lw] != CS_LIST_SKIP;
+    }
+    if (!any) return;
     load_table(slots32, ODD_STAGED ? kOddPos : kSlots, 64 * W);
     if constexpr (REC_LDS)
         for (int k = threadIdx.x; k < kRecs; k += 64 * W) rec_lds[k] = kRecords.e[k];
     const uint16_t* rec = REC_LDS ? rec_lds : kRecords.e;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int L = threadIdx.x & 63;
-    // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
-    const int per = gridDim.x >> 3;
-    const int wg = xcd ? (int)((blockIdx.x & 7) * per + (blockIdx.x >> 3)) : (int)blockIdx.x;
     const int64_t li = (int64_t)wg * WAVES + wave;
     const uint32_t ent = li < a.n_list ? a.list[li] : CS_LIST_SKIP;
     const bool valid = ent != CS_LIST_SKIP;
@@ -391,7 +399,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     uint32_t sm = 0u, lo = 0u;  // boundary bricks: Bouzidi side bits and the lane's link-run offsets
     int bb = 0;
     if constexpr (BND) {
-        bb = (int)li;  // the boundary list is in boundary-index order: bidx[list[li]] == li
+        bb = a.brec ? a.brec[li] : (int)li;  // the entry's boundary record: bidx[brick] == bb
         // the per-cell boundary words are loaded up front, so the link loads later wait for one round trip only
         mask = a.lmask[(uint32_t)bb * BC + L];
         sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;

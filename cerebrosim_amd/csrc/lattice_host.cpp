@@ -14,6 +14,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <sched.h>
 #include <thread>
 #include <vector>
 
@@ -31,6 +32,15 @@ struct LatticeHost {
     std::vector<float> uin;
     std::vector<int64_t> counts;  // wall, inlet, outlet 0..n_out-1
 };
+
+// CPUs this process may run on (its affinity mask, which a container or a job's CPU share narrows), not the
+// machine's count.
+int cpus_available() {
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) return std::max(1, CPU_COUNT(&set));
+    return (int)std::max(1u, std::thread::hardware_concurrency());
+}
 
 template <class F>
 void parallel_for(int64_t n, int threads, F&& f) {
@@ -78,7 +88,7 @@ extern "C" int cs_lattice_build(const cs_grid_in* in, void** handle) {
     CS_REQUIRE(in->n_openings >= 0 && (in->n_openings == 0 || in->openings), "cs_lattice_build: openings");
     const int64_t nx = in->shape[0], ny = in->shape[1], nz = in->shape[2];
     const int64_t G = nx * ny * nz;
-    int threads = in->threads > 0 ? in->threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    int threads = in->threads > 0 ? in->threads : cs::cpus_available();
     auto* L = new (std::nothrow) cs::LatticeHost();
     CS_REQUIRE(L, "cs_lattice_build: out of memory");
     try {

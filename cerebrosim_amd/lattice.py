@@ -12,7 +12,9 @@ These mirror the module-level API of CerebroSim's ``lbm.py`` so that callers wri
 """
 from __future__ import annotations
 
+import contextlib
 import os
+import threading
 
 import numpy as np
 
@@ -87,12 +89,30 @@ def _classify_openings(points, code, inlets, outlets, dx, slack):
     return code
 
 
-def build_lattice(phi, origin, dx, openings=(), periodic=(), opening_slack=1.0, native=True, threads=0):
+_local = threading.local()
+
+
+@contextlib.contextmanager
+def builder_threads(n):
+    """Within the block, lattices built on this thread use n native builder threads (default: every CPU of the
+    process). A pool of builder threads sets 1, so that the pool and not each build spreads over the CPUs."""
+    old = getattr(_local, "threads", 0)
+    _local.threads = int(n)
+    try:
+        yield
+    finally:
+        _local.threads = old
+
+
+def build_lattice(phi, origin, dx, openings=(), periodic=(), opening_slack=1.0, native=True, threads=None):
     """Sparse fluid-cell lattice from a signed distance sampled on grid nodes (negative = fluid), lbm.py:264-330.
 
     native=True (default) builds with the threaded C++ builder (cs_lattice_build); native=False with the numpy
-    restatement below (build_lattice_numpy), which gives identical arrays (tested). Arguments and result as
+    restatement below (build_lattice_numpy), which gives identical arrays (tested). threads: native builder threads
+    (default: as set by builder_threads, else every CPU the process may use). Arguments and result as
     build_lattice_numpy."""
+    if threads is None:
+        threads = getattr(_local, "threads", 0)
     if not native:
         return build_lattice_numpy(phi, origin, dx, openings, periodic, opening_slack)
     import ctypes

@@ -188,6 +188,29 @@ class BrickMoments:
                 t[lo:hi].zero_()
 
 
+class StreamFork:
+    """A side stream for launches that may run concurrently with those of a main stream. fork() makes the side stream
+    wait for everything queued on the main stream so far, join() makes the main stream wait for everything queued on
+    the side stream. Both are event dependencies, so a graph captured from the main stream records them as edges and
+    its replays run the two branches concurrently. Default priority (both streams)."""
+
+    def __init__(self, main, dev):
+        torch = _torch()
+        self.main = main
+        self.side = torch.cuda.Stream(device=dev)
+        self.handle = ctypes.c_void_p(self.side.cuda_stream)
+
+    def fork(self):
+        e = _torch().cuda.Event()
+        e.record(self.main)
+        self.side.wait_event(e)
+
+    def join(self):
+        e = _torch().cuda.Event()
+        e.record(self.side)
+        self.main.wait_event(e)
+
+
 class _BricksBackend:
     """4^3 brick storage with AA in-place streaming (see csrc/k_bricks.hip): populations [bricks][27][64] once,
     moments per slot (BrickMoments). Two launches per step: the interior bricks (no boundary data) and the boundary
@@ -259,14 +282,24 @@ class _BricksBackend:
         return A
 
     def step(self, s, a):
-        for boundary in (False, True):
-            self.step_part(s, a, boundary)
+        """The step's two launches. They are independent: in the AA pattern every slot is read and then written by
+        one thread of the step (interior or boundary brick alike), the moments read are the previous step's, and
+        the Bouzidi side copies are a boundary cell's own. So the boundary bricks run on a side stream, concurrently
+        with the interior bricks (the boundary kernels are bound by instruction issue, the interior ones by HBM)."""
+        if not (self.L["n_interior"] and self.L["n_boundary"]):
+            for boundary in (False, True):
+                self.step_part(s, a, boundary)
+            return
+        s._fork.fork()
+        self.step_part(s, a, False)
+        self.step_part(s, a, True, s._fork.handle)
+        s._fork.join()
 
-    def step_part(self, s, a, boundary):
-        """One of the step's two launches: interior bricks or boundary bricks."""
+    def step_part(self, s, a, boundary, stream=None):
+        """One of the step's two launches: interior bricks or boundary bricks (on the solver stream by default)."""
         A = self._args(s, a, boundary)
         if A.n_list:
-            _native.call("cs_bricks_step", ctypes.byref(A), s.stream_handle)
+            _native.call("cs_bricks_step", ctypes.byref(A), stream if stream is not None else s.stream_handle)
 
     def launch_parts(self):
         """(name, cells processed) of the launches of one step, in order."""
@@ -380,6 +413,7 @@ class Solver:
         dev = self.dev
         self._tstream = torch.cuda.Stream(device=dev)          # default priority
         self.stream_handle = ctypes.c_void_p(self._tstream.cuda_stream)
+        self._fork = StreamFork(self._tstream, dev)
         with torch.cuda.stream(self._tstream):
             self.p_out = torch.zeros(max(1, lat["n_outlets"]), dtype=torch.float32, device=dev)
             self.tick = torch.zeros(1, dtype=torch.int32, device=dev)

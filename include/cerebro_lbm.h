@@ -182,6 +182,8 @@ typedef struct cs_bricks_sizes {
     int64_t n_boundary;         /* bricks with a solid cell or a missing link */
     int64_t n_interior;
     int64_t n_slots;            /* n_bricks * 64 */
+    int64_t n_links;            /* missing links of the boundary bricks (entries of qlink before its padding) */
+    int64_t n_side;             /* Bouzidi side copies (links with an smask bit) */
 } cs_bricks_sizes;
 
 /* Build the brick layout on the host; *handle owns the arrays until cs_bricks_free. */
@@ -198,7 +200,15 @@ int cs_bricks_sizes_of(const void* handle, cs_bricks_sizes* out);
  *  "lmask"         uint32 [n_boundary][64] bit i: link i is fluid; bit 27: inlet cell; 0 for solid slots
  *  "smask"         uint32 [n_boundary][64] bit i: wall link i with q >= 1/2 whose opposite link is fluid
  *  "qc"            float [n_boundary][27][64] wall fraction q (>= 0), -1 inlet, -2 - k outlet k, 0 fluid
- *  "uin"           float [n_boundary][3][64] inlet profile of inlet cells */
+ *  "uin"           float [n_boundary][3][64] inlet profile of inlet cells
+ * and the compact boundary link data the step reads:
+ *  "qlink"         float [n_links + 1]     qc of every missing link: brick by brick, lane by lane, directions 1..26;
+ *                                          one element of padding (a lane reads ahead where its link is not missing)
+ *  "qstart"        int32 [n_boundary + 1]  start of each boundary brick's run in qlink
+ *  "sstart"        int32 [n_boundary + 1]  start of each boundary brick's run of side copies (smask links, same order)
+ *  "loff"          uint32 [n_boundary][64] lane's run offset in its brick: qlink offset | side offset << 16 (link i
+ *                                          of a lane: offset + number of the lane's links below i)
+ *  "ldir"          uint32 [n_boundary][2]  per brick: union of the missing links, union of the smask bits */
 int cs_bricks_copy(const void* handle, const char* name, void* dst);
 int cs_bricks_free(void* handle);
 
@@ -224,7 +234,8 @@ typedef struct cs_bricks_step_args {
     void* f;                    /* [n_bricks][1728] float or half, slot order of cs_bricks_slot_table */
     const int32_t* nbr;         /* [n_bricks][27] */
     const uint32_t* list;       /* bricks to process (entries brick | case << CS_CASE_SHIFT, or CS_LIST_SKIP); for
-                                   boundary bricks, position li in the list is the brick's boundary record */
+                                   boundary bricks, the boundary record of entry li is brec[li], or li when brec is
+                                   NULL */
     int64_t n_list;
     const int32_t* bidx;        /* [n_bricks] */
     const uint32_t* lmask;      /* [n_boundary][64] */
@@ -267,6 +278,8 @@ typedef struct cs_bricks_step_args {
     const float* case_force;
     int32_t p_out_stride;
     int32_t variant;            /* launch variant, 0 .. cs_bricks_variant_info(-1) - 1 (0 = default) */
+    const int32_t* brec;        /* boundary lists: record of each entry (NULL: the entry's position), so that a list
+                                   can hold its live entries only while records keep their places */
 } cs_bricks_step_args;
 int cs_bricks_step(const cs_bricks_step_args* a, void* stream);
 /* Launch variant `variant`: wavefronts (bricks) per workgroup and XCD-aware workgroup order. Returns CS_OK, or the

@@ -199,9 +199,14 @@ def test_compact_link_arrays():
     runs, ldir the per-brick unions; and the kernels' index (lane offset + popcount of the lane's links below i) finds
     every entry. Checked against the dense qc, lmask and smask with plain loops."""
     lat = tilted_pipe()
-    lay = bricks.build(lat)
+    lay = bricks.build(lat, dense=True)
     nb = lay["n_boundary"]
     qc = lay["qc"].reshape(nb, 27, 64)
+    # the native builder's arrays equal the numpy restatement's
+    ref = bricks.compact_links(lay)
+    for k, v in ref.items():
+        assert np.array_equal(np.asarray(lay[k]), np.asarray(v)) and np.asarray(lay[k]).dtype == np.asarray(v).dtype \
+            if isinstance(v, np.ndarray) else lay[k] == v, k
     expect, sides = [], []
     starts, sstarts = [0], [0]
     for bb in range(nb):

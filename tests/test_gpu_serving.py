@@ -55,7 +55,7 @@ def test_budget_queues_requests():
     rep = srv.serve(reqs, workloads.tube_case)
     recs = sorted(rep["records"], key=lambda r: r["admitted"])
     for a, b in zip(recs, recs[1:]):
-        assert b["admitted"] >= a["done"] - 1e-6          # one at a time
+        assert b["admitted"] >= a["left"] - 1e-6          # one at a time: admitted after the previous case left
     seq = serving.run_sequential(reqs, workloads.tube_case, storage="fp32", chunk=100, keep="fields")
     got = {r["name"]: r for r in rep["records"]}
     for s in seq["records"]:
