@@ -320,6 +320,7 @@ class Batch:
         self._retired = []                 # (graphs, event after their last replay) awaiting destruction
         self.captures, self.capture_s = 0, 0.0
         self._args_key, self._args = None, {}
+        self._blocks = None
 
     def _stage(self, x):
         """A pinned host tensor holding x: a region of the batch's staging ring, reused once the copies that last
@@ -570,6 +571,36 @@ class Batch:
             A.drive = drive
         return got
 
+    def _native_blocks(self):
+        """(argument blocks of both parities, blocks per parity, side-stream flags) for cs_bricks_run, with the drive
+        pointers at table row 0; kept while the step argument blocks and the drive table stay the same."""
+        self._k = 0
+        b0, b1 = self._step_launches(0), self._step_launches(1)
+        key = (self._args_key, self.drive.data_ptr())
+        if self._blocks is None or self._blocks[0] != key:
+            assert [(id(ar), bnd) for ar, bnd, _ in b0] == [(id(ar), bnd) for ar, bnd, _ in b1]
+            n = len(b0)
+            arr = (_native.BricksStepArgs * max(1, 2 * n))(*[A for _, _, A in b0 + b1])
+            side = (ctypes.c_int32 * max(1, n))(*[int(bnd) for _, bnd, _ in b0])
+            self._blocks = (key, arr, n, side)
+        return self._blocks[1:]
+
+    def _steps(self, n, a):
+        """The launches of n steps from parity a, drives at table rows 0..n-1: one native call (cs_bricks_run) that
+        forks the boundary launches onto the side stream each step; the per-step Python launches when cases measure
+        flows (their launch follows each step)."""
+        if any(ar.flux is not None for ar in self.arenas):
+            for k in range(n):
+                self._k = k
+                self._launch(a)
+                a = 1 - a
+            return
+        arr, nargs, side = self._native_blocks()
+        if nargs:
+            _native.call("cs_bricks_run", arr, nargs, side, int(n), int(a), self.max_cases, self.stream_handle,
+                         self._fork.handle)
+        self._k = n - 1
+
     def _launch(self, a):
         """The launches of one step of parity a (drives at table row self._k). The brick launches of a step are
         independent (as in lbm._BricksBackend.step; arenas share nothing), so the boundary bricks run on a side
@@ -611,11 +642,7 @@ class Batch:
                 tc = time.perf_counter()
                 with _native.capture(self.stream_handle) as cap:
                     self._clock_table(chunk)
-                    a = self.cur
-                    for k in range(chunk):
-                        self._k = k
-                        self._launch(a)
-                        a = 1 - a
+                    self._steps(chunk, self.cur)
                 self.graphs[key] = cap.graph
                 self.capture_s += time.perf_counter() - tc
                 self.captures += 1

@@ -19,6 +19,7 @@ namespace {
 
 constexpr int B = 4;
 constexpr int BC = B * B * B;
+static_assert((B & (B - 1)) == 0 && BC == 64, "brick edge a power of two (bit masks), 64 cells (shift by 6)");
 constexpr uint32_t kInletBit = 1u << 27;
 
 struct BrickHost {
@@ -167,37 +168,37 @@ int build(const cs_lattice_in* in, BrickHost& h) {
     std::vector<uint8_t> is_boundary(nb, 0);
     for (int64_t s = 0; s < nb * BC; ++s)
         if (h.cell_of_slot[s] < 0) is_boundary[s / BC] = 1;
-    for (int64_t n = 0; n < N; ++n) {
-        const int32_t* c = in->coords + 3 * n;
-        const int64_t b = h.slot_of_cell[n] / BC;
-        for (int i = 0; i < Q; ++i) {
-            const int32_t src = in->nbr[(int64_t)i * N + n];
+    // direction-major: row i of nbr is read in order, and its sources (n - c_i) and their rows opp(i) follow n
+    // closely, so the checks stream through memory instead of jumping across the 27 rows for every cell
+    for (int i = 0; i < Q; ++i) {
+        const int ci[3] = {Lat::cx[i], Lat::cy[i], Lat::cz[i]};
+        const int32_t* row = in->nbr + (int64_t)i * N;
+        const int32_t* mirror = in->nbr + (int64_t)Lat::opp[i] * N;
+        for (int64_t n = 0; n < N; ++n) {
+            const int32_t src = row[n];
+            const int64_t b = h.slot_of_cell[n] >> 6;  // slot / BC (slots are non-negative)
             if (src < 0) {
                 CS_REQUIRE(src >= -3 - 64, "cs_bricks_build: link code %d out of range", src);
                 is_boundary[b] = 1;
                 continue;
             }
             CS_REQUIRE(src < N, "cs_bricks_build: nbr[%d, %lld] = %d out of range", i, (long long)n, src);
-            // implicit source: position c - c_i inside brick b's neighbourhood
-            int p[3] = {c[0] - Lat::cx[i], c[1] - Lat::cy[i], c[2] - Lat::cz[i]};
+            // implicit source: position c - c_i inside brick b's neighbourhood (coordinates are non-negative, B = 4)
+            const int32_t* c = in->coords + 3 * n;
             int o[3], l[3];
-            bool ok = true;
             for (int a = 0; a < 3; ++a) {
-                int local = c[a] % B - (a == 0 ? Lat::cx[i] : a == 1 ? Lat::cy[i] : Lat::cz[i]);
+                const int local = (c[a] & (B - 1)) - ci[a];
                 o[a] = local < 0 ? -1 : (local >= B ? 1 : 0);
-                l[a] = (local + B) % B;
-                (void)p;
+                l[a] = local & (B - 1);
             }
-            int64_t nbk = h.nbr[27 * b + (o[0] + 1) + 3 * (o[1] + 1) + 9 * (o[2] + 1)];
-            if (nbk < 0) ok = false;
-            else ok = h.slot_of_cell[src] == nbk * BC + l[0] + B * l[1] + B * B * l[2];
-            if (!ok) {
+            const int64_t nbk = h.nbr[27 * b + (o[0] + 1) + 3 * (o[1] + 1) + 9 * (o[2] + 1)];
+            if (nbk < 0 || h.slot_of_cell[src] != nbk * BC + l[0] + B * l[1] + B * B * l[2]) {
                 set_error("cs_bricks_build: link %d of cell %lld pulls from cell %d, which is not its grid neighbour "
                           "in the brick layout",
                           i, (long long)n, src);
                 return CS_E_LAYOUT;
             }
-            if (in->nbr[(int64_t)Lat::opp[i] * N + src] != (int32_t)n) {
+            if (mirror[src] != (int32_t)n) {
                 set_error("cs_bricks_build: link %d of cell %lld is not mirrored by link %d of cell %d", i,
                           (long long)n, Lat::opp[i], src);
                 return CS_E_LAYOUT;
