@@ -355,6 +355,16 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     __shared__ __align__(16) uint16_t rec_lds[REC_LDS ? kRecs : 1];
     __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
     __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
+    // FP16 boundary bricks: the brick's runs of link codes and side copies, copied into LDS with coalesced loads so
+    // that each lane's popcount-indexed reads hit LDS. Even steps reuse the stage (between the population reads and
+    // the write-back the populations are in registers); odd steps have their own buffer. Entries past the buffer
+    // (rare: more than kLq missing links in one brick) are read from global memory. Measured on the Womersley pipe:
+    // FP16 boundary launches 55.7 -> 52.4 us; FP32 ones slower (61.8 -> 68.5 us: the even kernel's registers grow
+    // from 80 to 96), so FP32 reads the links from global memory.
+    constexpr bool LINK_LDS = BND && H16;
+    constexpr int kLinkLds = STAGED ? Q * BC * (int)sizeof(T) / 4 : 768;
+    constexpr int kLq = kLinkLds * 2 / 3, kLs = kLinkLds - kLq;
+    __shared__ __align__(16) float links_all[LINK_LDS && !STAGED ? WAVES * kLinkLds : 1];
     const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
     // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
     const int per = gridDim.x >> 3;
@@ -451,6 +461,30 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         });
     }
 
+    const float* lnk = nullptr;
+    int nlq = 0, nls = 0;
+    if constexpr (LINK_LDS) {
+        float* buf;
+        if constexpr (STAGED) {
+            wave_sync();  // every lane has read its populations from the stage, which now takes the link runs
+            buf = reinterpret_cast<float*>(stage);
+        } else {
+            buf = links_all + wave * kLinkLds;
+        }
+        // run lengths: the last lane's run offset + its own count (solid lanes count zero)
+        const uint32_t miss_l = mask ? (~mask & (kAllLinks - 1u)) : 0u;
+        const int q_end = __builtin_amdgcn_readlane((int)((lo & 0xFFFFu) + (uint32_t)__popc(miss_l)), 63);
+        const int s_end = __builtin_amdgcn_readlane((int)((lo >> 16) + (uint32_t)__popc(sm & (kAllLinks - 1u))), 63);
+        nlq = q_end < kLq ? q_end : kLq;
+        nls = !a.bouzidi ? 0 : (s_end < kLs ? s_end : kLs);
+        const float* ql = a.qlink + a.qstart[bb];
+        const float* sd = a.side + a.sstart[bb];
+        for (int k = L; k < nlq; k += BC) buf[k] = ql[k];
+        for (int k = L; k < nls; k += BC) buf[kLq + k] = sd[k];
+        wave_sync();
+        lnk = buf;
+    }
+
     if (live) {
         const float om = a.om_brick[b] ? a.om_cell[slot] : (a.case_omega ? a.case_omega[cs] : a.omega);
         const float d = a.drive[cs];
@@ -494,8 +528,18 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                     static_for<0, nl>([&](auto kc) {
                         constexpr int k = decltype(kc)::value;
                         constexpr uint32_t below = (1u << (h0 + k)) - 1u;
-                        qv[k] = ((wq >> (h0 + k)) & 1u) ? ql[qo + (uint32_t)__popc(miss & below)] : 0.0f;
-                        pv[k] = ((ws >> (h0 + k)) & 1u) ? sd[so + (uint32_t)__popc(sm & below)] : 0.0f;
+                        qv[k] = 0.0f;
+                        pv[k] = 0.0f;
+                        if ((wq >> (h0 + k)) & 1u) {
+                            const uint32_t j = qo + (uint32_t)__popc(miss & below);
+                            if (LINK_LDS && (int)j < nlq) qv[k] = lnk[j];
+                            else qv[k] = ql[j];
+                        }
+                        if ((ws >> (h0 + k)) & 1u) {
+                            const uint32_t j = so + (uint32_t)__popc(sm & below);
+                            if (LINK_LDS && (int)j < nls) pv[k] = lnk[kLq + j];
+                            else pv[k] = sd[j];
+                        }
                     });
                     if constexpr (GAL && h0 == 1)
                         static_for<0, 6>([&](auto kc) { q16[decltype(kc)::value] = qv[decltype(kc)::value]; });

@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import collections
 import concurrent.futures as cf
+import os
 import time
 
 import numpy as np
@@ -53,11 +54,11 @@ class Server:
     storage: population storage of the cases ("fp16" or "fp32"); budget_bytes: HBM the arenas may take (default
     288 GB, capped by the device's free memory less 2 GiB); chunk: steps per graph launch; arena_bricks /
     arena_records: arena capacity (an arena grows to fit a larger case); workers: host threads building
-    lattices; keep: "summary" or "fields" (also keep the final (p, u) of each request); staging_bytes: pinned
+    lattices (default: the process's CPUs less one, at most 15); keep: "summary" or "fields" (also keep the final (p, u) of each request); staging_bytes: pinned
     host ring for the uploads of admitted cases."""
 
     def __init__(self, storage="fp16", budget_bytes=None, chunk=200, arena_bricks=1 << 20, arena_records=1 << 18,
-                 workers=12, keep="summary", max_cases=256, variant=0, device=None, staging_bytes=512 << 20,
+                 workers=None, keep="summary", max_cases=256, variant=0, device=None, staging_bytes=512 << 20,
                  **solver_kw):
         import torch
         free = None
@@ -70,7 +71,8 @@ class Server:
         self.batch = B.Batch(storage=storage, budget_bytes=budget, arena_bricks=arena_bricks,
                              arena_records=arena_records, max_cases=max_cases, variant=variant, device=device,
                              staging_bytes=staging_bytes, **solver_kw)
-        self.workers = int(workers)
+        # builder threads: one per CPU the process may use, less one for this loop, at most 15
+        self.workers = int(workers) if workers else max(1, min(15, len(os.sched_getaffinity(0)) - 1))
         self.keep = keep
 
     def serve(self, requests, make_case, time_scale=1.0, log=None):
