@@ -275,7 +275,8 @@ def form_serving(full, rate=None):
             "host_s": rep["host_s"],
             "ok": ok, "sequential_fp32_cases_per_hour": round(seq["cases_per_hour"], 1),
             "sequential_fp32_wall_s": round(seq["wall_s"], 2),
-            "speedup_vs_sequential": round(rep["cases_per_hour"] / seq["cases_per_hour"], 2)}
+            "speedup_vs_sequential": round(rep["cases_per_hour"] / seq["cases_per_hour"], 2),
+            "steady_speedup_vs_sequential": round(steady / seq["cases_per_hour"], 2)}
 
 
 def load_traffic(layout, storage="fp32"):
