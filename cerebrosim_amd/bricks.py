@@ -142,7 +142,7 @@ def slot_table():
 
 def stage_table():
     """(27, 64) uint16: staged position of the population an interior brick's lane l pulls for direction i in an
-    odd step (own group first, then the neighbour records in offset order 0..12, 14..26)."""
+    odd step (own group first, then the neighbour records: faces, edges, corners, each in offset order)."""
     out = np.empty((27, 64), np.uint16)
     _native.call("cs_bricks_stage_table", ctypes.c_void_p(out.ctypes.data))
     return out

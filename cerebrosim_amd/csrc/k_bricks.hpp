@@ -8,7 +8,8 @@
 // Slot order. An even step touches exactly its brick's own block. An odd step reads and then writes slot (m, k) from
 // the cell m - c_k, which lies in m's brick or in one of its 26 neighbours. slot[][] groups a block by that accessing
 // brick: first the 1000 slots the brick's own wavefront reaches, then one contiguous record per neighbour offset (100
-// slots for a face neighbour, 10 for an edge, 1 for a corner), population-major inside each group. In an odd step a
+// slots for a face neighbour, 10 for an edge, 1 for a corner; faces, then edges, then corners: group_at),
+// population-major inside each group. In an odd step a
 // wavefront therefore writes its own group and 26 records, each touched by no other wavefront, so cache lines fill
 // from one wavefront instead of collecting partial writes from two.
 //
@@ -43,12 +44,42 @@ struct SlotTable {
     uint16_t p[Q][BC];
 };
 
+// Slots of a block accessed from the brick at offset g ((ox+1) + 3 (oy+1) + 9 (oz+1)): 1000 from the brick itself,
+// 100 from a face neighbour, 10 from an edge neighbour, 1 from a corner neighbour.
+constexpr int group_size(int g) {
+    int n = 0;
+    for (int a = 0, v = g; a < 3; ++a, v /= 3) n += (v % 3 != 1);
+    return n == 0 ? 1000 : (n == 1 ? 100 : (n == 2 ? 10 : 1));
+}
+
+// The gi-th group of a block: the own group (13) first, then the face records, the edge records and the corner
+// records, each in offset order. Face and edge records hold an even number of slots, so each of them starts at an
+// even slot: the float16 odd step moves them as 4-byte pairs. The corners share the block's last 8 slots.
+struct GroupOrder {
+    int g[27];
+};
+
+constexpr GroupOrder make_group_order() {
+    GroupOrder t{};
+    int k = 0;
+    t.g[k++] = 13;
+    const int sizes[3] = {100, 10, 1};
+    for (int s : sizes)
+        for (int g = 0; g < 27; ++g)
+            if (g != 13 && group_size(g) == s) t.g[k++] = g;
+    return t;
+}
+
+constexpr GroupOrder kGroupOrder = make_group_order();
+
+constexpr int group_at(int gi) { return kGroupOrder.g[gi]; }
+
 constexpr SlotTable make_slots() {
     SlotTable t{};
     int pos = 0;
-    // group g = brick offset (ox+1) + 3 (oy+1) + 9 (oz+1) of the accessing cell l - c_k; own brick (13) first
+    // group g = brick offset (ox+1) + 3 (oy+1) + 9 (oz+1) of the accessing cell l - c_k, in group_at order
     for (int gi = 0; gi < 27; ++gi) {
-        const int g = gi == 0 ? 13 : (gi <= 13 ? gi - 1 : gi);
+        const int g = group_at(gi);
         for (int k = 0; k < Q; ++k)
             for (int l = 0; l < BC; ++l) {
                 const int x = (l & 3) - Lat::cx[k], y = ((l >> 2) & 3) - Lat::cy[k], z = (l >> 4) - Lat::cz[k];
@@ -71,17 +102,11 @@ constexpr int offset_index(int x, int y, int z) {
     return (x < 0 ? 0 : (x > 3 ? 2 : 1)) + 3 * (y < 0 ? 0 : (y > 3 ? 2 : 1)) + 9 * (z < 0 ? 0 : (z > 3 ? 2 : 1));
 }
 
-constexpr int group_size(int g) {
-    int n = 0;
-    for (int a = 0, v = g; a < 3; ++a, v /= 3) n += (v % 3 != 1);
-    return n == 0 ? 1000 : (n == 1 ? 100 : (n == 2 ? 10 : 1));
-}
-
-// Start of group g inside a block (the order of make_slots: own group 13, then 0..12, 14..26).
+// Start of group g inside a block (the order of make_slots: group_at).
 constexpr int group_start(int g) {
     int pos = 0;
     for (int gi = 0; gi < 27; ++gi) {
-        const int h = gi == 0 ? 13 : (gi <= 13 ? gi - 1 : gi);
+        const int h = group_at(gi);
         if (h == g) return pos;
         pos += group_size(h);
     }
@@ -89,13 +114,13 @@ constexpr int group_start(int g) {
 }
 
 // The odd-step footprint of a brick, staged in LDS: its own group, then for every neighbour offset o != 13 the
-// record of brick (b + o) that holds the slots accessed from b, i.e. group 26 - o of that brick. Staged order = the
-// block order of the groups (own first, then o = 0..12, 14..26), so stage_start(o) = group_start(26 - o) for o != 13
-// would not hold; it is computed here.
+// record of brick (b + o) that holds the slots accessed from b, i.e. group 26 - o of that brick. Staged order: own
+// group, then the offsets o in group_at order (faces, edges, corners), so that the face and edge records sit at
+// even staged positions too; stage_start(o) is computed here.
 constexpr int stage_start(int o) {
     int pos = 0;
     for (int oi = 0; oi < 27; ++oi) {
-        const int h = oi == 0 ? 13 : (oi <= 13 ? oi - 1 : oi);
+        const int h = group_at(oi);
         if (h == o) return pos;
         pos += group_size(h == 13 ? 13 : 26 - h);
     }
@@ -132,7 +157,7 @@ constexpr RecordTable make_records() {
     RecordTable t{};
     int p = 0;
     for (int oi = 0; oi < 27; ++oi) {
-        const int h = oi == 0 ? 13 : (oi <= 13 ? oi - 1 : oi);
+        const int h = group_at(oi);
         if (h == 13) continue;
         const int g = 26 - h;
         for (int k = 0; k < group_size(g); ++k) t.e[p++] = (uint16_t)((h << 11) | (group_start(g) + k));
@@ -141,9 +166,38 @@ constexpr RecordTable make_records() {
 }
 
 static_assert(group_start(13) == 0 && group_size(13) == kOwn, "own group first");
-static_assert(stage_start(0) == kOwn, "records after the own group");
+static_assert(stage_start(group_at(1)) == kOwn, "records after the own group");
 __constant__ RecordTable kRecords = make_records();
 __constant__ SlotTable kOddPos = make_oddpos();
+
+// Float16 odd step: the face and edge records as 4-byte pairs (staged positions kOwn + 2p, 2p + 1; entry (o << 11)
+// | block offset of the pair's first slot), then the 8 corner slots one by one (staged positions kOwn + 2 kNPairs
+// + c).
+constexpr int kNPairs = (6 * 100 + 12 * 10) / 2;
+constexpr int kNCorners = 8;
+struct PairTable {
+    uint16_t e[kNPairs];
+    uint16_t c[kNCorners];
+};
+
+constexpr PairTable make_pairs() {
+    PairTable t{};
+    constexpr RecordTable r = make_records();
+    for (int p = 0; p < kNPairs; ++p) t.e[p] = r.e[2 * p];
+    for (int c = 0; c < kNCorners; ++c) t.c[c] = r.e[2 * kNPairs + c];
+    return t;
+}
+
+constexpr bool pairs_valid() {
+    constexpr RecordTable r = make_records();
+    if (2 * kNPairs + kNCorners != kRecs) return false;
+    for (int p = 0; p < kNPairs; ++p)
+        if ((r.e[2 * p] & 1) != 0 || r.e[2 * p + 1] != r.e[2 * p] + 1) return false;  // same record, even offset
+    return true;
+}
+static_assert(pairs_valid(), "face and edge records start at even offsets and pair up inside one record");
+static_assert(kOwn % 8 == 0, "the staged records start 16-byte aligned in float16");
+__constant__ PairTable kPairs = make_pairs();
 
 // Cross-lane LDS hand-over inside one wavefront.
 __device__ __forceinline__ void wave_sync() {
@@ -206,17 +260,39 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
                 else l[k] = g[k];
             }
         });
-        static_for<0, (kRecs + BC - 1) / BC>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            const int p = j * BC + L;
-            if ((j + 1) * BC <= kRecs || p < kRecs) {
-                const uint32_t e = rec[p];
+        if constexpr (sizeof(T) == 2) {
+            // float16: face and edge records as 4-byte pairs (straight to LDS when DMA), corners one by one
+            uint32_t* lp = reinterpret_cast<uint32_t*>(stage + kOwn);
+            static_for<0, (kNPairs + BC - 1) / BC>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const int p = j * BC + L;
+                if ((j + 1) * BC <= kNPairs || p < kNPairs) {
+                    const uint32_t e = kPairs.e[p];
+                    uint32_t* g = reinterpret_cast<uint32_t*>(F + brick_base(tbl[e >> 11]) + (e & 2047u));
+                    if constexpr (STORE) *g = lp[p];
+                    else if constexpr (DMA) load_to_lds<4>(g, lp + j * BC);
+                    else lp[p] = *g;
+                }
+            });
+            if (L < kNCorners) {
+                const uint32_t e = kPairs.c[L];
                 const uint32_t at = brick_base(tbl[e >> 11]) + (e & 2047u);
-                if constexpr (STORE) F[at] = stage[kOwn + p];
-                else if constexpr (DMA && sizeof(T) == 4) load_to_lds<4>(F + at, stage + kOwn + j * BC);
-                else stage[kOwn + p] = F[at];
+                if constexpr (STORE) F[at] = stage[kOwn + 2 * kNPairs + L];
+                else stage[kOwn + 2 * kNPairs + L] = F[at];
             }
-        });
+        } else {
+            static_for<0, (kRecs + BC - 1) / BC>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const int p = j * BC + L;
+                if ((j + 1) * BC <= kRecs || p < kRecs) {
+                    const uint32_t e = rec[p];
+                    const uint32_t at = brick_base(tbl[e >> 11]) + (e & 2047u);
+                    if constexpr (STORE) F[at] = stage[kOwn + p];
+                    else if constexpr (DMA) load_to_lds<4>(F + at, stage + kOwn + j * BC);
+                    else stage[kOwn + p] = F[at];
+                }
+            });
+        }
         if constexpr (!STORE && DMA) __builtin_amdgcn_s_waitcnt(0);
     }
 }
@@ -349,10 +425,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     constexpr bool FUSED16 = H16 && MODEL == 1;
     __shared__ int tbl[WAVES][32];
     __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
-    // the odd-step record table: in LDS for float16 storage (whose registers limit the occupancy anyway), read from
-    // constant memory for float32 (five workgroups of four waves then fit the LDS of a compute unit)
-    constexpr bool REC_LDS = ODD_STAGED && H16;
-    __shared__ __align__(16) uint16_t rec_lds[REC_LDS ? kRecs : 1];
+    // the odd-step record table (FP32) is read from constant memory, so that five workgroups of four waves fit the
+    // LDS of a compute unit; FP16 moves the records as pairs (kPairs, also constant memory)
     __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
     __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
     // FP16 boundary bricks: the brick's runs of link codes and side copies, copied into LDS with coalesced loads so
@@ -378,9 +452,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     }
     if (!any) return;
     load_table(slots32, ODD_STAGED ? kOddPos : kSlots, 64 * W);
-    if constexpr (REC_LDS)
-        for (int k = threadIdx.x; k < kRecs; k += 64 * W) rec_lds[k] = kRecords.e[k];
-    const uint16_t* rec = REC_LDS ? rec_lds : kRecords.e;
+    const uint16_t* rec = kRecords.e;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int L = threadIdx.x & 63;
     const int64_t li = (int64_t)wg * WAVES + wave;

@@ -215,8 +215,9 @@ int cs_bricks_free(void* handle);
 /* Slot order inside a brick's block of 27 x 64 populations: population k of cell l (x + 4 y + 16 z) is at offset
  * out[k * 64 + l] of the block (27 x 64 uint16). Slots are grouped by the brick whose odd step accesses them. */
 int cs_bricks_slot_table(uint16_t* out);
-/* Staging order of an interior brick's odd-step footprint (its own group, then the record of each neighbour in the
- * order o = 0..12, 14..26): position of the population lane l reads for direction i is out[i * 64 + l]. */
+/* Staging order of an interior brick's odd-step footprint (its own group, then the record of each neighbour: the 6
+ * face neighbours, the 12 edge neighbours, the 8 corner neighbours, each in offset order o): position of the
+ * population lane l reads for direction i is out[i * 64 + l]. */
 int cs_bricks_stage_table(uint16_t* out);
 /* The 728 neighbour records of an interior brick's staged odd-step footprint, after its own 1000-slot group: entry
  * p = (neighbour offset o << 11) | block offset, for staged position 1000 + p. */

@@ -160,7 +160,9 @@ def test_odd_step_staging_maps_to_direct_slots():
     in LDS. Every staged position must stand for exactly the global slot the direct path accesses."""
     slots = bricks.slot_table().astype(np.int64)
     stage = bricks.stage_table().astype(np.int64)
-    order = [13] + [o for o in range(27) if o != 13]
+    # staged order: own group, then the face, edge and corner neighbours, each in offset order
+    nonown = lambda o: sum(d != 1 for d in (o % 3, (o // 3) % 3, o // 9))
+    order = [13] + sorted((o for o in range(27) if o != 13), key=lambda o: (nonown(o), o))
     # group g of a block: its slots, in block order
     l = np.arange(64)
     pos = np.stack([l % 4, (l // 4) % 4, l // 16], 1)
@@ -188,6 +190,18 @@ def test_odd_step_staging_maps_to_direct_slots():
         assert staged == direct, i
     # a bijection: the 27 x 64 reads cover the footprint once
     assert sorted(stage.ravel().tolist()) == list(range(27 * 64))
+    # face and edge records start at even block offsets and even staged positions (float16 moves them as 4-byte
+    # pairs); the corners are the last 8 slots of a block
+    for g, v in members.items():
+        if len(v) in (100, 10):
+            assert start[g] % 2 == 0 and sorted(v) == list(range(start[g], start[g] + len(v))), g
+        if len(v) == 1:
+            assert start[g] >= 27 * 64 - 8, g
+    pos_staged = 1000
+    for o in order[1:]:
+        size = len(members[26 - o])
+        assert size == 1 or pos_staged % 2 == 0, o
+        pos_staged += size
     # the packed record table the kernels load the footprint with
     rec = bricks.record_table().astype(np.int64)
     assert foot[:1000] == [(13, j) for j in range(1000)]
