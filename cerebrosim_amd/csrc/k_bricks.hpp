@@ -8,7 +8,8 @@
 // Slot order. An even step touches exactly its brick's own block. An odd step reads and then writes slot (m, k) from
 // the cell m - c_k, which lies in m's brick or in one of its 26 neighbours. slot[][] groups a block by that accessing
 // brick: first the 1000 slots the brick's own wavefront reaches, then one contiguous record per neighbour offset (100
-// slots for a face neighbour, 10 for an edge, 1 for a corner; faces, then edges, then corners: group_at),
+// slots for a face neighbour, 10 for an edge, 1 for a corner; plane by plane of the neighbour's z offset, faces,
+// edges, corners inside a plane: group_at),
 // population-major inside each group. In an odd step a
 // wavefront therefore writes its own group and 26 records, each touched by no other wavefront, so cache lines fill
 // from one wavefront instead of collecting partial writes from two.
@@ -52,27 +53,40 @@ constexpr int group_size(int g) {
     return n == 0 ? 1000 : (n == 1 ? 100 : (n == 2 ? 10 : 1));
 }
 
-// The gi-th group of a block: the own group (13) first, then the face records, the edge records and the corner
-// records, each in offset order. Face and edge records hold an even number of slots, so each of them starts at an
-// even slot: the float16 odd step moves them as 4-byte pairs. The corners share the block's last 8 slots.
+// Group orders. In a block (group_at): the own group (13) first, then the records plane by plane of the accessing
+// neighbour's z offset (oz = -1, 0, +1, as the offset index runs), inside a plane faces, then edges, then corners
+// (each in offset order). Every plane holds an even number of slots and face and edge records hold an even number,
+// so each face and edge record starts at an even slot: the float16 odd step moves them as 4-byte pairs. The planes
+// stay together as in the plain offset order, which the direct accesses of boundary bricks walk. In an interior
+// brick's staged odd-step footprint (stage_at, over neighbour offsets o): own group, faces, edges, corners, so that
+// the pairs are contiguous in LDS.
 struct GroupOrder {
     int g[27];
 };
 
-constexpr GroupOrder make_group_order() {
+constexpr GroupOrder make_group_order(bool block) {
     GroupOrder t{};
     int k = 0;
     t.g[k++] = 13;
     const int sizes[3] = {100, 10, 1};
-    for (int s : sizes)
-        for (int g = 0; g < 27; ++g)
-            if (g != 13 && group_size(g) == s) t.g[k++] = g;
+    if (block) {
+        for (int gz = 0; gz < 3; ++gz)
+            for (int sz : sizes)
+                for (int g = 9 * gz; g < 9 * gz + 9; ++g)
+                    if (g != 13 && group_size(g) == sz) t.g[k++] = g;
+    } else {
+        for (int sz : sizes)
+            for (int g = 0; g < 27; ++g)
+                if (g != 13 && group_size(g) == sz) t.g[k++] = g;
+    }
     return t;
 }
 
-constexpr GroupOrder kGroupOrder = make_group_order();
+constexpr GroupOrder kBlockOrder = make_group_order(true);
+constexpr GroupOrder kStageOrder = make_group_order(false);
 
-constexpr int group_at(int gi) { return kGroupOrder.g[gi]; }
+constexpr int group_at(int gi) { return kBlockOrder.g[gi]; }
+constexpr int stage_at(int oi) { return kStageOrder.g[oi]; }
 
 constexpr SlotTable make_slots() {
     SlotTable t{};
@@ -115,12 +129,12 @@ constexpr int group_start(int g) {
 
 // The odd-step footprint of a brick, staged in LDS: its own group, then for every neighbour offset o != 13 the
 // record of brick (b + o) that holds the slots accessed from b, i.e. group 26 - o of that brick. Staged order: own
-// group, then the offsets o in group_at order (faces, edges, corners), so that the face and edge records sit at
-// even staged positions too; stage_start(o) is computed here.
+// group, then the offsets o in stage_at order (faces, edges, corners), so that the face and edge records are
+// contiguous 4-byte pairs; stage_start(o) is computed here.
 constexpr int stage_start(int o) {
     int pos = 0;
     for (int oi = 0; oi < 27; ++oi) {
-        const int h = group_at(oi);
+        const int h = stage_at(oi);
         if (h == o) return pos;
         pos += group_size(h == 13 ? 13 : 26 - h);
     }
@@ -157,7 +171,7 @@ constexpr RecordTable make_records() {
     RecordTable t{};
     int p = 0;
     for (int oi = 0; oi < 27; ++oi) {
-        const int h = group_at(oi);
+        const int h = stage_at(oi);
         if (h == 13) continue;
         const int g = 26 - h;
         for (int k = 0; k < group_size(g); ++k) t.e[p++] = (uint16_t)((h << 11) | (group_start(g) + k));
@@ -166,7 +180,7 @@ constexpr RecordTable make_records() {
 }
 
 static_assert(group_start(13) == 0 && group_size(13) == kOwn, "own group first");
-static_assert(stage_start(group_at(1)) == kOwn, "records after the own group");
+static_assert(stage_start(stage_at(1)) == kOwn, "records after the own group");
 __constant__ RecordTable kRecords = make_records();
 __constant__ SlotTable kOddPos = make_oddpos();
 
@@ -429,16 +443,6 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     // LDS of a compute unit; FP16 moves the records as pairs (kPairs, also constant memory)
     __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
     __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
-    // FP16 boundary bricks: the brick's runs of link codes and side copies, copied into LDS with coalesced loads so
-    // that each lane's popcount-indexed reads hit LDS. Even steps reuse the stage (between the population reads and
-    // the write-back the populations are in registers); odd steps have their own buffer. Entries past the buffer
-    // (rare: more than kLq missing links in one brick) are read from global memory. Measured on the Womersley pipe:
-    // FP16 boundary launches 55.7 -> 52.4 us; FP32 ones slower (61.8 -> 68.5 us: the even kernel's registers grow
-    // from 80 to 96), so FP32 reads the links from global memory.
-    constexpr bool LINK_LDS = BND && H16;
-    constexpr int kLinkLds = STAGED ? Q * BC * (int)sizeof(T) / 4 : 768;
-    constexpr int kLq = kLinkLds * 2 / 3, kLs = kLinkLds - kLq;
-    __shared__ __align__(16) float links_all[LINK_LDS && !STAGED ? WAVES * kLinkLds : 1];
     const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
     // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
     const int per = gridDim.x >> 3;
@@ -533,30 +537,6 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         });
     }
 
-    const float* lnk = nullptr;
-    int nlq = 0, nls = 0;
-    if constexpr (LINK_LDS) {
-        float* buf;
-        if constexpr (STAGED) {
-            wave_sync();  // every lane has read its populations from the stage, which now takes the link runs
-            buf = reinterpret_cast<float*>(stage);
-        } else {
-            buf = links_all + wave * kLinkLds;
-        }
-        // run lengths: the last lane's run offset + its own count (solid lanes count zero)
-        const uint32_t miss_l = mask ? (~mask & (kAllLinks - 1u)) : 0u;
-        const int q_end = __builtin_amdgcn_readlane((int)((lo & 0xFFFFu) + (uint32_t)__popc(miss_l)), 63);
-        const int s_end = __builtin_amdgcn_readlane((int)((lo >> 16) + (uint32_t)__popc(sm & (kAllLinks - 1u))), 63);
-        nlq = q_end < kLq ? q_end : kLq;
-        nls = !a.bouzidi ? 0 : (s_end < kLs ? s_end : kLs);
-        const float* ql = a.qlink + a.qstart[bb];
-        const float* sd = a.side + a.sstart[bb];
-        for (int k = L; k < nlq; k += BC) buf[k] = ql[k];
-        for (int k = L; k < nls; k += BC) buf[kLq + k] = sd[k];
-        wave_sync();
-        lnk = buf;
-    }
-
     if (live) {
         const float om = a.om_brick[b] ? a.om_cell[slot] : (a.case_omega ? a.case_omega[cs] : a.omega);
         const float d = a.drive[cs];
@@ -600,18 +580,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                     static_for<0, nl>([&](auto kc) {
                         constexpr int k = decltype(kc)::value;
                         constexpr uint32_t below = (1u << (h0 + k)) - 1u;
-                        qv[k] = 0.0f;
-                        pv[k] = 0.0f;
-                        if ((wq >> (h0 + k)) & 1u) {
-                            const uint32_t j = qo + (uint32_t)__popc(miss & below);
-                            if (LINK_LDS && (int)j < nlq) qv[k] = lnk[j];
-                            else qv[k] = ql[j];
-                        }
-                        if ((ws >> (h0 + k)) & 1u) {
-                            const uint32_t j = so + (uint32_t)__popc(sm & below);
-                            if (LINK_LDS && (int)j < nls) pv[k] = lnk[kLq + j];
-                            else pv[k] = sd[j];
-                        }
+                        qv[k] = ((wq >> (h0 + k)) & 1u) ? ql[qo + (uint32_t)__popc(miss & below)] : 0.0f;
+                        pv[k] = ((ws >> (h0 + k)) & 1u) ? sd[so + (uint32_t)__popc(sm & below)] : 0.0f;
                     });
                     if constexpr (GAL && h0 == 1)
                         static_for<0, 6>([&](auto kc) { q16[decltype(kc)::value] = qv[decltype(kc)::value]; });

@@ -191,12 +191,12 @@ def test_odd_step_staging_maps_to_direct_slots():
     # a bijection: the 27 x 64 reads cover the footprint once
     assert sorted(stage.ravel().tolist()) == list(range(27 * 64))
     # face and edge records start at even block offsets and even staged positions (float16 moves them as 4-byte
-    # pairs); the corners are the last 8 slots of a block
+    # pairs); the records of the neighbours below (oz = -1) come before those of oz = 0, then oz = +1
     for g, v in members.items():
         if len(v) in (100, 10):
             assert start[g] % 2 == 0 and sorted(v) == list(range(start[g], start[g] + len(v))), g
-        if len(v) == 1:
-            assert start[g] >= 27 * 64 - 8, g
+    by_plane = sorted((g for g in members if g != 13), key=lambda g: start[g])
+    assert [g // 9 for g in by_plane] == sorted(g // 9 for g in by_plane)
     pos_staged = 1000
     for o in order[1:]:
         size = len(members[26 - o])
