@@ -289,11 +289,18 @@ def load_traffic(layout, storage="fp32"):
 
 
 def pmc_probe(layout, storage):
-    """Short run for hardware-counter passes: the bench workload, 4 eager steps (2 even, 2 odd) and a 1 GiB device
-    copy whose counter readings calibrate FETCH_SIZE / WRITE_SIZE."""
+    """Short run for hardware-counter passes: the bench workload, 4 eager steps (2 even, 2 odd) with the interior and
+    boundary launches in sequence, and a 1 GiB device copy whose counter readings calibrate FETCH_SIZE / WRITE_SIZE."""
     import torch
     s = make_solver(build_workload(), layout, storage)
-    s.run(4, chunk=1000)
+    # the step's launches one after the other on the solver stream (in a run they overlap on two streams, and a
+    # counter read around one dispatch would then include the other's traffic)
+    for _ in range(4):
+        s._clock()
+        for part in (False, True):
+            s.backend.step_part(s, s.cur, part)
+        s.cur = 1 - s.cur
+        s.steps += 1
     s.synchronize()
     x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
