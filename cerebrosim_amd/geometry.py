@@ -187,3 +187,45 @@ def wall_points(info, spacing=2.0, end_margin=6.0):
     Nn = np.concatenate(nrm)
     keep = np.abs(sdf(P)) < 0.25                     # on the union's wall, not inside another segment
     return P[keep], Nn[keep]
+
+
+def artery_y(dx_mm=0.1, R_parent=1.6, R_a=1.25, R_b=1.0, L_parent=14.0, L_branch=10.0, angle_deg=35.0, pad_mm=0.3):
+    """A synthetic Y-shaped artery in millimetres, the stand-in for the reference's story artery (vessel.py, outside
+    this build): a parent tube (the ICA end) splitting into two branches (MCA, ACA) in the y-z plane. Every end is an
+    opening of kind "outlet", as in simulate.py (the first is driven by the inlet pressure). Returns (lattice with
+    "wall_mm", the distance of each fluid cell to the wall; ends; probes), in the form simulate.py's setup builds
+    (simulate.py:162-200)."""
+    ang = math.radians(angle_deg)
+    ax_a = np.array([0.0, math.sin(ang), math.cos(ang)])
+    ax_b = np.array([0.0, -math.sin(ang), math.cos(ang)])
+    J = np.array([0.0, 0.0, L_parent])
+    A0 = np.zeros(3)
+    segs = [(A0, np.array([0.0, 0.0, 1.0]), L_parent + 0.5 * R_parent, R_parent), (J, ax_a, L_branch, R_a),
+            (J, ax_b, L_branch, R_b)]
+
+    def sdf(P):
+        out = np.full(len(P), np.inf)
+        for A, ax, L, R in segs:
+            out = np.minimum(out, _segment_sdf(P, A, ax, L, lambda s, R=R: np.full_like(s, R)))
+        return out
+
+    ends_pts = [A0] + [A + ax * L for A, ax, L, _ in segs[1:]]
+    lo = np.min([p - max(R_parent, R_a, R_b) for p in ends_pts + [J]], axis=0) - pad_mm
+    hi = np.max([p + max(R_parent, R_a, R_b) for p in ends_pts + [J]], axis=0) + pad_mm
+    lo[2], hi[2] = -pad_mm, max(p[2] for p in ends_pts) + pad_mm
+    n = np.ceil((hi - lo) / dx_mm).astype(int) + 1
+    origin = lo + OFFSET * dx_mm * np.array([1, 1, 0])
+    I, Jg, K = np.meshgrid(*[np.arange(k) for k in n], indexing="ij")
+    P = origin + np.stack([I.ravel(), Jg.ravel(), K.ravel()], 1) * dx_mm
+    phi = sdf(P).reshape(tuple(n)).astype(np.float32)
+    ends = [{"kind": "outlet", "label": "inlet (ICA)", "centre": A0, "normal": np.array([0.0, 0.0, -1.0]),
+             "radius": R_parent}]
+    for (A, ax, L, R), label in zip(segs[1:], ("MCA outlet", "ACA outlet")):
+        ends.append({"kind": "outlet", "label": label, "centre": A + ax * L, "normal": ax, "radius": R})
+    lat = build_lattice(phi, origin, dx_mm, openings=ends)
+    lat["wall_mm"] = -phi[tuple(lat["coords"].T)]
+    probes = [{"name": "ICA", "point": np.array([0.0, 0.0, 0.4 * L_parent]), "radius": 0.5 * R_parent},
+              {"name": "junction", "point": J.copy(), "radius": 1.0},
+              {"name": "MCA distal", "point": J + ax_a * 0.7 * L_branch, "radius": 0.5 * R_a},
+              {"name": "ACA distal", "point": J + ax_b * 0.7 * L_branch, "radius": 0.5 * R_b}]
+    return lat, ends, probes
