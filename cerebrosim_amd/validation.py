@@ -5,6 +5,9 @@
 * ``womersley_exact`` -- validate.py:86-89: oscillating body force F0 cos(w t), Bessel-function profile.
 * ``womersley_flow_profile`` -- the flow-rate-driven Womersley profile for a pipe whose mean velocity is prescribed as
   a Fourier series (used for the pulsatile pipe with a velocity inlet).
+* ``womersley_line_flow`` -- how one harmonic of the flow rate varies along a rigid pipe of weakly compressible
+  fluid (the lattice fluid, sound speed 1/sqrt(3)) between a velocity inlet and a pressure outlet: Womersley's
+  propagation constant for a rigid tube, J. Physiol. 127 (1955) 553-563.
 * ``hagen_poiseuille_gradient`` -- validate.py:204: G = 8 nu Q / (pi R^4).
 """
 from __future__ import annotations
@@ -12,7 +15,7 @@ from __future__ import annotations
 import math
 
 import numpy as np
-from scipy.special import jv
+from scipy.special import jv, jve
 
 
 def l2(a, b):
@@ -45,6 +48,20 @@ def womersley_flow_profile(r, R, nu, harmonics, period, t):
         amp = complex(a, -b)                               # a cos + b sin = Re[(a - i b) e^{i w t}]
         u = u + np.real(amp * shape / mean * np.exp(1j * n * w * t))
     return u
+
+
+def womersley_line_flow(z, L, R, nu, period, n=1, c=1 / math.sqrt(3)):
+    """Q_n(z) / Q_n(0) for harmonic n of the flow rate in a rigid pipe of length L filled with a fluid of sound speed
+    c, the flow rate imposed at z = 0 and the pressure held at z = L (p_n(L) = 0). Linearised, per harmonic
+    (w = 2 pi n / period): continuity  i w A p / (rho c^2) = -dQ/dz  and Womersley's momentum balance for the
+    cross-section flow  i w rho Q = -A (1 - F10) dp/dz,  F10 = 2 J1(K) / (K J0(K)),  K = i^1.5 R sqrt(w / nu).
+    Together  Q'' + k^2 Q = 0  with  k = w / (c sqrt(1 - F10)),  Q(0) = Q0,  Q'(L) = 0, so
+    Q(z) = Q0 cos(k (L - z)) / cos(k L). The pipe resonates where Re(k) L nears pi / 2."""
+    w = 2 * math.pi * n / period
+    K = (1j ** 1.5) * R * math.sqrt(w / nu)
+    F10 = 2 * jve(1, K) / (K * jve(0, K))           # scaled Bessel functions: the same ratio, no overflow
+    k = w / (c * np.sqrt(1 - F10))
+    return np.cos(k * (L - np.asarray(z, float))) / np.cos(k * L)
 
 
 def hagen_poiseuille_gradient(nu, Q, R):
