@@ -79,6 +79,12 @@ class WssArgs(ctypes.Structure):
                 ("acc_tau", c_void_p), ("acc_mag", c_void_p)]
 
 
+class SdfTubesArgs(ctypes.Structure):
+    _fields_ = [("pts", c_void_p), ("br_start", c_void_p), ("br_count", c_void_p), ("n_branch", c_i32),
+                ("cap_p", c_void_p), ("cap_n", c_void_p), ("cap_r", c_void_p), ("blend", c_float),
+                ("origin", c_float * 3), ("dx", c_float), ("n", c_i64 * 3), ("out", c_void_p)]
+
+
 class LatticeIn(ctypes.Structure):
     _fields_ = [("n_cells", c_i64), ("shape", c_i32 * 3), ("periodic", c_i32 * 3), ("coords", c_void_p),
                 ("nbr", c_void_p), ("q", c_void_p), ("uin", c_void_p)]
@@ -131,6 +137,7 @@ SIGNATURES = {
     "cs_gather": [ctypes.POINTER(GatherArgs), c_void_p],
     "cs_reduce_finite": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
     "cs_wss": [ctypes.POINTER(WssArgs), c_void_p],
+    "cs_sdf_tubes": [ctypes.POINTER(SdfTubesArgs), c_void_p],
     "cs_lattice_build": [ctypes.POINTER(GridIn), ctypes.POINTER(c_void_p)],
     "cs_lattice_sizes": [c_void_p, ctypes.POINTER(c_i64), c_void_p, c_i32],
     "cs_lattice_copy": [c_void_p, ctypes.c_char_p, c_void_p],

@@ -190,8 +190,8 @@ def wall_points(info, spacing=2.0, end_margin=6.0):
 
 
 def artery_y(dx_mm=0.1, R_parent=1.6, R_a=1.25, R_b=1.0, L_parent=14.0, L_branch=10.0, angle_deg=35.0, pad_mm=0.3):
-    """A synthetic Y-shaped artery in millimetres, the stand-in for the reference's story artery (vessel.py, outside
-    this build): a parent tube (the ICA end) splitting into two branches (MCA, ACA) in the y-z plane. Every end is an
+    """A synthetic Y-shaped artery in millimetres, a small and quick stand-in for the story artery
+    (``vessel.artery_case``): a parent tube (the ICA end) splitting into two branches (MCA, ACA) in the y-z plane. Every end is an
     opening of kind "outlet", as in simulate.py (the first is driven by the inlet pressure). Returns (lattice with
     "wall_mm", the distance of each fluid cell to the wall; ends; probes), in the form simulate.py's setup builds
     (simulate.py:162-200)."""

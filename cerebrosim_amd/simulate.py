@@ -13,8 +13,9 @@ the lumped circulation around the 3D segment, sampled every millisecond, written
 * Outputs (simulate.py:356-371): ``series.npz`` (t, Q, p_open, p_probe, umax), ``meta.json`` and ``fields.npz``
   (coords int16, origin, dx_mm, wall_mm, t, p_mmhg, u_m_s), the files wss.py and the explorer read.
 
-The geometry comes from the caller (a lattice with ``wall_mm``, its open ends and probes): the reference builds it
-from vessel.py / patient.py, which this build does not include; ``geometry.artery_y`` is a synthetic stand-in.
+The geometry comes from the caller (a lattice with ``wall_mm``, its open ends and probes): ``vessel.artery_case``
+builds the story artery as the reference's setup does (simulate.py:155-200); ``geometry.artery_y`` is a smaller
+synthetic Y artery.
 ``heart_period`` may be shortened for tests; the reference's is 1 s.
 """
 from __future__ import annotations

@@ -2,7 +2,8 @@
 point, summarised in a solver.json-shaped file with the reference's published values beside each result.
 
     python -m cerebrosim_amd.validate [--out solver.json] [--layout auto|bricks|indexed] [--storage fp32|fp16]
-                                      [--quick] [--tests poiseuille,womersley,open_pipe,stenosis]
+                                      [--quick] [--tests poiseuille,womersley,open_pipe,stenosis,artery]
+                                      [--segment segment.json]
 
 1. Poiseuille: periodic pipes D = 10 / 20 / 40 with interpolated (Bouzidi) walls and D = 20 half-way, driven by a
    body force for 14 decay times of the slowest mode (validate.py:59-83).
@@ -13,9 +14,12 @@ point, summarised in a solver.json-shaped file with the reference's published va
 4. The 70 % cosine stenosis at narrow70 Re 100 / 200 and narrow70_fine Re 200: pressure drop against Young & Tsai
    (validate.py:218-291).
 
-Test 5 of the reference (the artery, validate.py:294-342) and its time-per-heartbeat table need the reference's
-vessel geometry (vessel.py), which is outside this build. The periodic pipes are 4 nodes long instead of 2: the flow
-does not vary along the axis, so the per-cell values are the same, and the brick layout can wrap a multiple of 4.
+5. The artery (healthy, 0.1 mm; ``vessel.artery_lattice`` from the story's segment file): lattice build, 1000
+   warm-up steps, 20000 timed steps, speed, stability and the maximum speed; then the time per heartbeat at the
+   three operating points (validate.py:294-342, 368-375). Runs when a segment file is given.
+
+The periodic pipes are 4 nodes long instead of 2: the flow does not vary along the axis, so the per-cell values are
+the same, and the brick layout can wrap a multiple of 4.
 """
 from __future__ import annotations
 
@@ -178,12 +182,55 @@ def stenosis(key, Re0, layout="auto", storage="fp32", sev=0.7, D0_mm=2.0, Ls_mm=
     return res
 
 
-TESTS = ("poiseuille", "womersley", "open_pipe", "stenosis")
+SPONGE_CELLS = 20                  # validate.py:41
+PUBLISHED_ARTERY = {"cells": 620028, "steps": 21000, "mlups": 384.8784235294374,            # solver.json:179-194
+                    "u_max_m_s": 1.4885234832763672,
+                    "counts": {"wall": 747143, "inlet": 11400, "outlet": [2649, 2682]}}
 
 
-def run(tests=TESTS, layout="auto", storage="fp32", quick=False, log=print):
+def artery(segment, layout="auto", storage="fp32", steps=20000, warmup=1000):
+    """Test 5 (validate.py:311-340): the healthy artery at 0.1 mm with a steady velocity inlet (the sourced mean
+    inflow, ramped over 10000 steps), zero outlet pressures and 20-cell sponges; speed over `steps` after `warmup`."""
+    from .artery import Q_INLET_MEAN
+    from .vessel import artery_lattice
+    un = units("healthy")
+    t0 = time.perf_counter()
+    lat, g = artery_lattice("healthy", un.dx * 1e3, segment=segment)
+    build_s = time.perf_counter() - t0
+    R_in = lat["openings"][0]["radius"] * 1e-3
+    U_mean = Q_INLET_MEAN * 1e-6 / 60 / (math.pi * R_in ** 2)
+    s = _solver(lat, un.nu_lu, layout, storage)
+    s.set_sponge(SPONGE_CELLS, SPONGE_TAU)
+    s.set_drive([float(un.u_lu(U_mean))], ramp=10000)
+    s.set_outlet_pressure([0.0] * lat["n_outlets"])
+    s.run(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.run(steps)
+    s.synchronize()
+    secs = time.perf_counter() - t0
+    _, u = s.fields()
+    N = lat["nbr"].shape[1]
+    res = {"cells": N, "build_s": build_s, "steps": s.steps, "secs": secs, "mlups": N * steps / secs / 1e6,
+           "ok": s.ok(), "u_max_m_s": float(un.u(np.linalg.norm(u, axis=1).max())), "counts": lat["counts"],
+           "openings": [{k: (v.tolist() if isinstance(v, np.ndarray) else v) for k, v in o.items()}
+                        for o in lat["openings"]], "U_mean_inlet": U_mean, "published": PUBLISHED_ARTERY}
+    beat = {}
+    for key in ("healthy", "narrow70", "narrow70_fine"):       # validate.py:368-375
+        uk = units(key)
+        cells = N * (1e-4 / uk.dx) ** 3
+        n = 1.0 / uk.dt                                         # steps per 1 s heartbeat
+        beat[key] = {"dx_mm": uk.dx * 1e3, "dt_us": uk.dt * 1e6, "tau": uk.tau, "cells_M": cells / 1e6,
+                     "steps_per_beat": n, "minutes_per_beat": cells * n / (res["mlups"] * 1e6) / 60}
+    return res, beat
+
+
+TESTS = ("poiseuille", "womersley", "open_pipe", "stenosis", "artery")
+
+
+def run(tests=TESTS, layout="auto", storage="fp32", quick=False, log=print, segment=None):
     """Run the selected tests; returns the solver.json-shaped summary. quick: Poiseuille D = 10 and 20 (half-way)
-    and Womersley at Wo 5.6 only."""
+    and Womersley at Wo 5.6 only. The artery runs when `segment` (the story's segment file) is given."""
     unh = units("healthy")
     T0 = time.perf_counter()
     out = {"solver": f"cerebrosim_amd: sparse D3Q27 lattice Boltzmann (HIP, gfx950), regularised, Galilean "
@@ -214,6 +261,12 @@ def run(tests=TESTS, layout="auto", storage="fp32", quick=False, log=print):
             log(f"stenosis {r['key']} Re0 {r['Re0']}: dp {r['dp_Pa']:.0f} Pa (published "
                 f"{r['published']['dp_Pa']:.0f}), vs Young-Tsai {r['vs_young_tsai']:+.3f}, spread "
                 f"{r['spread_last_flow_through']:.2e}, {r['steps']} steps, {r['mlups']:.0f} MLUPS")
+    if "artery" in tests and segment is not None and not quick:
+        out["artery"], out["per_heartbeat"] = artery(segment, layout, storage)
+        r = out["artery"]
+        log(f"artery healthy 0.1 mm: {r['cells'] / 1e6:.2f} M cells (published {PUBLISHED_ARTERY['cells']}), "
+            f"{r['mlups']:.0f} MLUPS, ok {r['ok']}, max speed {r['u_max_m_s']:.3f} m/s (published "
+            f"{PUBLISHED_ARTERY['u_max_m_s']:.3f}); links {r['counts']}")
     out["total_s"] = time.perf_counter() - T0
     return out
 
@@ -225,8 +278,10 @@ def main(argv=None):
     ap.add_argument("--storage", default="fp32", choices=("fp32", "fp16"))
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--tests", default=",".join(TESTS))
+    ap.add_argument("--segment", default=None, help="the story's segment file (enables test 5, the artery)")
     args = ap.parse_args(argv)
-    res = run(tuple(t for t in args.tests.split(",") if t), args.layout, args.storage, args.quick)
+    res = run(tuple(t for t in args.tests.split(",") if t), args.layout, args.storage, args.quick,
+              segment=args.segment)
     with open(args.out, "w") as fh:
         json.dump(res, fh, indent=1, default=float)
     print(f"wrote {args.out} ({res['total_s']:.1f} s)")

@@ -418,6 +418,29 @@ int cs_wss(const cs_wss_args* a, void* stream);
 #define CS_REDUCE_WORK (2 * CS_REDUCE_BLOCKS)
 int cs_reduce_finite(const float* x, int64_t n, double* work, double* result, void* stream);
 
+/* ---------------------------------------------------------------------------------------------------------------
+ * Vessel signed distance -- replaces k_sdf (vessel.py:183-207) launched by sdf_grid (vessel.py:234-245): at every
+ * voxel centre p = origin + (i, j, k) dx of an (n[0], n[1], n[2]) grid (k fastest), the distance to each branch's
+ * chain of round cones (stations pts[br_start[b] ..][x, y, z, radius], radius interpolated along each segment),
+ * cut flat by the branch's cap half-spaces within 2.5 cap radii of each cap (cap_r 0 = no cap), combined over the
+ * branches with the polynomial smooth minimum of width `blend`. Negative inside. FP32, mm.
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct cs_sdf_tubes_args {
+    const float* pts;           /* device, [n_pts][4]: x, y, z, radius */
+    const int32_t* br_start;    /* device, [n_branch] first station */
+    const int32_t* br_count;    /* device, [n_branch] stations (>= 2) */
+    int32_t n_branch;           /* at most 16 */
+    const float* cap_p;         /* device, [2 n_branch][3]: start cap, end cap of each branch */
+    const float* cap_n;         /* device, [2 n_branch][3] outward unit normals */
+    const float* cap_r;         /* device, [2 n_branch] cap radius; 0 = no cap */
+    float blend;                /* > 0 */
+    float origin[3];
+    float dx;
+    int64_t n[3];
+    float* out;                 /* device, n[0] n[1] n[2] floats */
+} cs_sdf_tubes_args;
+int cs_sdf_tubes(const cs_sdf_tubes_args* a, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

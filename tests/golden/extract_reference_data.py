@@ -3,7 +3,10 @@ reference code is run) into tests/golden/reference_data.json:
 
 * geometry/physics.json: the inflow waveform over one beat, 200 samples rounded to 5 decimals (vessel.py:253-265
   generated it; physics.json:"inlet");
-* reports/step4_report.md:28: the calibrated circuit's resistances (GPa s/m^3).
+* reports/step4_report.md:28: the calibrated circuit's resistances (GPa s/m^3);
+* geometry/solver.json:"artery": the healthy artery at 0.1 mm (validate.py:311-340): cells, link counts, openings,
+  steps, speed and the maximum speed after them;
+* reports/step4_report.md:7-21: the healthy case's mean inflow, MCA and ACA flows and probe pressures.
 
     python tests/golden/extract_reference_data.py [/path/to/reference]
 """
@@ -27,6 +30,17 @@ def main(ref):
                                "source": "geometry/physics.json: inlet"},
            "circuit_GPa_s_m3": {"R_mca": float(m.group(1)), "R_aca": float(m.group(2)), "R_seg": float(m.group(3)),
                                 "source": "reports/step4_report.md:28"}}
+    art = json.load(open(os.path.join(ref, "geometry", "solver.json")))["artery"]
+    out["artery"] = {k: art[k] for k in ("cells", "steps", "secs", "mlups", "u_max_m_s", "counts", "openings",
+                                         "U_mean_inlet")}
+    out["artery"]["source"] = "geometry/solver.json: artery"
+    row = re.search(r"\| healthy \| 0.1 mm \| (\d+) \| (\d+) mL/min \| (\d+) \| (\d+) \|", report)
+    probes = {}
+    for name in ("ICA", "junction", "MCA before clot", "clot throat", "MCA after clot", "MCA distal", "ACA distal"):
+        probes[name] = float(re.search(r"\| " + name + r" \| ([\d.]+) \|", report).group(1))
+    out["step4_healthy"] = {"beats": int(row.group(1)), "inflow_mL_min": float(row.group(2)),
+                            "mca_mL_min": float(row.group(3)), "aca_mL_min": float(row.group(4)),
+                            "probe_mmHg": probes, "source": "reports/step4_report.md:7-21"}
     json.dump(out, open(os.path.join(HERE, "reference_data.json"), "w"), indent=1)
     print("wrote reference_data.json")
 
