@@ -6,7 +6,8 @@ reference code is run) into tests/golden/reference_data.json:
 * reports/step4_report.md:28: the calibrated circuit's resistances (GPa s/m^3);
 * geometry/solver.json:"artery": the healthy artery at 0.1 mm (validate.py:311-340): cells, link counts, openings,
   steps, speed and the maximum speed after them;
-* reports/step4_report.md:7-21: the healthy case's mean inflow, MCA and ACA flows and probe pressures.
+* reports/step4_report.md:7-21: each case's grid, beats, mean inflow, MCA and ACA flows, peak speed and probe
+  pressures (None where the report says there is no lumen).
 
     python tests/golden/extract_reference_data.py [/path/to/reference]
 """
@@ -34,13 +35,20 @@ def main(ref):
     out["artery"] = {k: art[k] for k in ("cells", "steps", "secs", "mlups", "u_max_m_s", "counts", "openings",
                                          "U_mean_inlet")}
     out["artery"]["source"] = "geometry/solver.json: artery"
-    row = re.search(r"\| healthy \| 0.1 mm \| (\d+) \| (\d+) mL/min \| (\d+) \| (\d+) \|", report)
-    probes = {}
+    cases = ("healthy", "narrow70", "occluded")
+    probes = {c: {} for c in cases}
     for name in ("ICA", "junction", "MCA before clot", "clot throat", "MCA after clot", "MCA distal", "ACA distal"):
-        probes[name] = float(re.search(r"\| " + name + r" \| ([\d.]+) \|", report).group(1))
-    out["step4_healthy"] = {"beats": int(row.group(1)), "inflow_mL_min": float(row.group(2)),
-                            "mca_mL_min": float(row.group(3)), "aca_mL_min": float(row.group(4)),
-                            "probe_mmHg": probes, "source": "reports/step4_report.md:7-21"}
+        cells = re.search(r"\| " + name + r" \| ([^|]+) \| ([^|]+) \| ([^|]+) \|", report).groups()
+        for c, v in zip(cases, cells):
+            v = v.strip()
+            probes[c][name] = None if v.startswith("(") else float(v)
+    for c in cases:
+        row = re.search(r"\| " + c + r" \| ([\d.]+) mm \| (\d+) \| (\d+) mL/min \| (-?\d+) \| (-?\d+) \| [^|]+ \| "
+                        r"([\d.]+) m/s", report)
+        out["step4_" + c] = {"dx_mm": float(row.group(1)), "beats": int(row.group(2)),
+                             "inflow_mL_min": float(row.group(3)), "mca_mL_min": float(row.group(4)),
+                             "aca_mL_min": float(row.group(5)), "peak_speed_m_s": float(row.group(6)),
+                             "probe_mmHg": probes[c], "source": "reports/step4_report.md:7-21"}
     json.dump(out, open(os.path.join(HERE, "reference_data.json"), "w"), indent=1)
     print("wrote reference_data.json")
 
