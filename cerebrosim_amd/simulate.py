@@ -56,10 +56,11 @@ def probe_cells(lat, dx_mm, probes):
 
 
 def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, flow=1.0, heart_period=HEART_PERIOD,
-        log=print, **solver_kw):
+        log=print, progress_every=None, **solver_kw):
     """Simulate `beats` heartbeats and write series.npz, meta.json and fields.npz into out_dir. cal: the circuit
     (``artery.circuit``) to drive with; None calibrates on this run, as the reference's healthy case does. flow
-    scales the absolute inlet pressure. Returns (series, meta)."""
+    scales the absolute inlet pressure. progress_every: also log the flows every that many samples. Returns
+    (series, meta)."""
     os.makedirs(out_dir, exist_ok=True)
     un = units(dx, dt)
     dx_mm = dx * 1e3
@@ -106,6 +107,9 @@ def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, 
         pp[pcount == 0] = np.nan
         series["p_probe"].append(pp)
         series["umax"].append(float(un.u(umax)))
+        if progress_every and len(series["t"]) % progress_every == 0:
+            log(f"{case}: t {s.steps * un.dt:.3f} s ({time.perf_counter() - T0:.1f} s): flows "
+                + ", ".join("%.1f" % q for q in Q) + " mL/min")
         if s.steps % spb == 0:
             beat = s.steps // spb
             if not s.ok():
