@@ -58,13 +58,21 @@ class FluxCasesArgs(ctypes.Structure):
     _fields_ = [("idx", c_void_p), ("w", c_void_p), ("slot", c_void_p), ("start", c_void_p), ("vel", c_void_p),
                 ("vel_elem", c_i64), ("vel_comp", c_i64), ("R", c_void_p), ("drive", c_void_p), ("p_out", c_void_p),
                 ("q_last", c_void_p), ("n_open", c_void_p), ("pref", c_void_p), ("alpha", c_void_p),
-                ("n_cases", c_i32), ("open_stride", c_i32)]
+                ("n_cases", c_i32), ("open_stride", c_i32), ("work", c_void_p), ("work_chunks", c_i32)]
 
 
 class FluxArgs(ctypes.Structure):
     _fields_ = [("idx", c_void_p), ("w", c_void_p), ("slot", c_void_p), ("m", c_i64), ("vel", c_void_p),
                 ("vel_elem", c_i64), ("vel_comp", c_i64), ("R", c_void_p), ("drive", c_void_p), ("p_out", c_void_p),
-                ("q_last", c_void_p), ("n_open", c_i32), ("pref", c_float), ("alpha", c_float)]
+                ("q_last", c_void_p), ("n_open", c_i32), ("pref", c_float), ("alpha", c_float), ("work", c_void_p),
+                ("work_chunks", c_i32)]
+
+
+def flux_chunks(m):
+    """Chunks the flow kernels split m slab cells into (max(1, ceil(m / 262144)) cells per thread of 1024-thread
+    workgroups), the minimum work_chunks of cs_flux_bc / cs_flux_cases for that lattice."""
+    per = max(1, -(-m // (1024 * 256)))
+    return -(-m // (1024 * per))
 
 
 class GatherArgs(ctypes.Structure):

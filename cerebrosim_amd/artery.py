@@ -125,6 +125,8 @@ class ArterySolver(Solver):
             self.n_open = len(R_lu)
             self.Qlast = torch.zeros(self.n_open, dtype=torch.float32, device=dev)
             self.R = torch.from_numpy(np.asarray(R_lu, np.float32)).to(dev)
+            self.f_chunks = max(1, _native.flux_chunks(int(self.f_idx.numel())))
+            self.f_work = torch.zeros(self.n_open * self.f_chunks, dtype=torch.float32, device=dev)
         self.synchronize()
         self.pref = float(pref_lu)
         self.alpha = float(alpha)
@@ -138,6 +140,7 @@ class ArterySolver(Solver):
         A.R, A.drive, A.p_out, A.q_last = self.R.data_ptr(), self._drive_ptr(), self.p_out.data_ptr(), \
             self.Qlast.data_ptr()
         A.n_open, A.pref, A.alpha = self.n_open, self.pref, self.alpha
+        A.work, A.work_chunks = self.f_work.data_ptr(), self.f_chunks
         _native.call("cs_flux_bc", ctypes.byref(A), self.stream_handle)
 
     def gather(self, idx, slot, n_probe):

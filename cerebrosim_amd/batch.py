@@ -243,6 +243,13 @@ class Arena:
             fx["idx"] = put(fx["idx"], np.concatenate(idx).astype(np.int32), torch.int32, 0)
             fx["slot"] = put(fx["slot"], np.concatenate(slot).astype(np.int32), torch.int32, 0)
             fx["w"] = put(fx["w"], np.concatenate(w).astype(np.float32).reshape(-1), torch.float32, 0.0)
+            # chunk sums of the flow kernels: room for the largest case's chunks (a power of two, at most 256)
+            need = max([1] + [_native.flux_chunks(int(start[c + 1] - start[c])) for c in range(self.batch.max_cases)])
+            if fx.get("work") is None or fx["chunks"] < need:
+                fx["chunks"] = min(256, _pow2(need, lo=1))
+                fx["work"] = torch.zeros(self.batch.max_cases * self.batch.max_open * fx["chunks"],
+                                         dtype=torch.float32, device=dev)
+                grew = True
             self.batch._upload(fx["start"], start)
             self.flux = fx
         self.batch._upload(self.n_open, nopen)
@@ -533,6 +540,7 @@ class Batch:
             self.q_last.data_ptr()
         A.n_open, A.pref, A.alpha = arena.n_open.data_ptr(), self.pref.data_ptr(), self.alpha.data_ptr()
         A.n_cases, A.open_stride = self.max_cases, self.max_open
+        A.work, A.work_chunks = fx["work"].data_ptr(), fx["chunks"]
         _native.call("cs_flux_cases", ctypes.byref(A), self.stream_handle)
 
     def launch_parts(self):

@@ -80,52 +80,105 @@ __device__ float block_max(float v, float* lds) {
     return m;
 }
 
-// ---- k_flux + k_bc (one workgroup) ----------------------------------------------------------------------------
+// ---- k_flux + k_bc: chunked partial sums, then one wave per lattice -------------------------------------------
 constexpr int kMaxOpen = 64;
+constexpr int kFluxGroup = 8;   // openings summed per pass over the slab cells
 
-// Flow of each opening's slab over cells [0, m) and the opening-pressure update, for one lattice (workgroup).
-__device__ __forceinline__ void flux_bc(const int32_t* idx, const float* w, const int32_t* slot, int64_t m,
-                                        const float* vel, int64_t vel_elem, int64_t vel_comp, const float* R,
-                                        float drive, float* p_out, float* q_last, int n_open, float pref, float alpha,
-                                        float* lds, float* qs) {
+// Chunking of a lattice's m slab cells, a function of m alone (so a lattice gives the same bits alone or packed):
+// each thread of a 1024-thread workgroup takes flux_per_thread(m) cells of the chunk, at most 256 chunks.
+__device__ __forceinline__ int64_t flux_per_thread(int64_t m) {
+    const int64_t per = (m + (int64_t)kRedThreads * 256 - 1) / ((int64_t)kRedThreads * 256);
+    return per < 1 ? 1 : per;
+}
+__device__ __forceinline__ int64_t flux_chunks(int64_t m) {
+    const int64_t ch = (int64_t)kRedThreads * flux_per_thread(m);
+    return (m + ch - 1) / ch;
+}
+
+// Phase 1: chunk b of one lattice's slab cells; work[k * wc + b] = sum over the chunk's cells of opening k of
+// vel . w (per thread in cell order, then the fixed block tree).
+__device__ __forceinline__ void flux_partial(const int32_t* idx, const float* w, const int32_t* slot, int64_t m,
+                                             const float* vel, int64_t vel_elem, int64_t vel_comp, int n_open,
+                                             float* work, int wc, int64_t b, float* lds) {
+    const int64_t per = flux_per_thread(m), chunk = (int64_t)kRedThreads * per;
+    for (int k0 = 0; k0 < n_open; k0 += kFluxGroup) {
+        float acc[kFluxGroup];
+#pragma unroll
+        for (int q = 0; q < kFluxGroup; ++q) acc[q] = 0.0f;
+        for (int64_t j = 0; j < per; ++j) {
+            const int64_t t = b * chunk + j * kRedThreads + threadIdx.x;
+            if (t >= m) break;
+            const int kk = slot[t] - k0;
+            if (kk < 0 || kk >= kFluxGroup) continue;
+            const float* v = vel + (int64_t)idx[t] * vel_elem;
+            const float val = v[0] * w[3 * t] + v[vel_comp] * w[3 * t + 1] + v[2 * vel_comp] * w[3 * t + 2];
+#pragma unroll
+            for (int q = 0; q < kFluxGroup; ++q) acc[q] += (q == kk) ? val : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < kFluxGroup; ++q) {
+            if (k0 + q < n_open) {   // workgroup-uniform
+                const float tot = block_sum(acc[q], lds);
+                if (threadIdx.x == 0) work[(int64_t)(k0 + q) * wc + b] = tot;
+            }
+        }
+    }
+}
+
+// Phase 2 (one wave): Q[k] = sum of the G chunk sums (per lane in chunk order, then a fixed butterfly), then
+// p_out[0] = drive and p_out[k] += alpha (R[k] Q[k] - pref - p_out[k]) for k >= 1; q_last = Q. valid = false
+// (a work buffer too small for the chunks) writes NaN, which the health check reports.
+__device__ __forceinline__ void flux_final(const float* work, int wc, int64_t G, bool valid, int n_open,
+                                           const float* R, float drive, float* p_out, float* q_last, float pref,
+                                           float alpha) {
+    const int lane = threadIdx.x;
+    if (lane == 0) p_out[0] = drive;
     for (int k = 0; k < n_open; ++k) {
-        float acc = 0.0f;
-        for (int64_t t = threadIdx.x; t < m; t += kRedThreads) {
-            if (slot[t] != k) continue;
-            const int64_t s = idx[t];
-            const float* v = vel + s * vel_elem;
-            acc += v[0] * w[3 * t] + v[vel_comp] * w[3 * t + 1] + v[2 * vel_comp] * w[3 * t + 2];
-        }
-        const float tot = block_sum(acc, lds);
-        if (threadIdx.x == 0) qs[k] = tot;
-    }
-    if (threadIdx.x == 0) {
-        p_out[0] = drive;
-        for (int k = 0; k < n_open; ++k) {
-            q_last[k] = qs[k];
-            if (k > 0) p_out[k] = p_out[k] + alpha * (R[k] * qs[k] - pref - p_out[k]);
+        float s = 0.0f;
+        if (valid)
+            for (int64_t b = lane; b < G; b += 64) s += work[(int64_t)k * wc + b];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) {
+            const float q = valid ? s : __builtin_nanf("");
+            q_last[k] = q;
+            if (k > 0) p_out[k] = p_out[k] + alpha * (R[k] * q - pref - p_out[k]);
         }
     }
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_flux_bc(cs_flux_args a) {
+__global__ __launch_bounds__(kRedThreads) void k_flux_partial(cs_flux_args a) {
     __shared__ float lds[kRedThreads / 64];
-    __shared__ float qs[kMaxOpen];
-    flux_bc(a.idx, a.w, a.slot, a.m, a.vel, a.vel_elem, a.vel_comp, a.R, a.drive[0], a.p_out, a.q_last, a.n_open,
-            a.pref, a.alpha, lds, qs);
+    flux_partial(a.idx, a.w, a.slot, a.m, a.vel, a.vel_elem, a.vel_comp, a.n_open, a.work, a.work_chunks,
+                 blockIdx.x, lds);
 }
 
-// one workgroup per case of a packed set of lattices
-__global__ __launch_bounds__(kRedThreads) void k_flux_cases(cs_flux_cases_args a) {
+__global__ __launch_bounds__(64) void k_flux_final(cs_flux_args a) {
+    flux_final(a.work, a.work_chunks, flux_chunks(a.m), true, a.n_open, a.R, a.drive[0], a.p_out, a.q_last, a.pref,
+               a.alpha);
+}
+
+// packed lattices: grid (work_chunks, n_cases), case c's chunks b < flux_chunks(m_c)
+__global__ __launch_bounds__(kRedThreads) void k_flux_partial_cases(cs_flux_cases_args a) {
     __shared__ float lds[kRedThreads / 64];
-    __shared__ float qs[kMaxOpen];
-    const int c = blockIdx.x;
+    const int c = blockIdx.y;
     const int n_open = a.n_open[c];
     if (n_open <= 0) return;  // workgroup-uniform
     const int64_t s0 = a.start[c], m = a.start[c + 1] - s0;
+    if ((int64_t)blockIdx.x >= flux_chunks(m)) return;
+    float* work = a.work + (int64_t)c * a.open_stride * a.work_chunks;
+    flux_partial(a.idx + s0, a.w + 3 * s0, a.slot + s0, m, a.vel, a.vel_elem, a.vel_comp, n_open, work,
+                 a.work_chunks, blockIdx.x, lds);
+}
+
+__global__ __launch_bounds__(64) void k_flux_final_cases(cs_flux_cases_args a) {
+    const int c = blockIdx.x;
+    const int n_open = a.n_open[c];
+    if (n_open <= 0) return;  // workgroup-uniform
+    const int64_t m = a.start[c + 1] - a.start[c];
+    const int64_t G = flux_chunks(m);
     const int64_t o = (int64_t)c * a.open_stride;
-    flux_bc(a.idx + s0, a.w + 3 * s0, a.slot + s0, m, a.vel, a.vel_elem, a.vel_comp, a.R + o, a.drive[c],
-            a.p_out + o, a.q_last + o, n_open, a.pref[c], a.alpha[c], lds, qs);
+    flux_final(a.work + o * a.work_chunks, a.work_chunks, G, G <= a.work_chunks, n_open, a.R + o, a.drive[c],
+               a.p_out + o, a.q_last + o, a.pref[c], a.alpha[c]);
 }
 
 // ---- k_gather (one workgroup) ---------------------------------------------------------------------------------
@@ -232,11 +285,20 @@ extern "C" int cs_clock(const cs_clock_args* a, void* stream) {
 }
 
 extern "C" int cs_flux_bc(const cs_flux_args* a, void* stream) {
-    CS_REQUIRE(a && a->R && a->drive && a->p_out && a->q_last, "cs_flux_bc: null argument");
+    CS_REQUIRE(a && a->R && a->drive && a->p_out && a->q_last && a->work, "cs_flux_bc: null argument");
     CS_REQUIRE(a->n_open >= 1 && a->n_open <= cs::kMaxOpen, "cs_flux_bc: n_open %d not in [1, 64]", a->n_open);
-    CS_REQUIRE(a->m == 0 || (a->idx && a->w && a->slot && a->vel), "cs_flux_bc: null cell list");
-    cs::k_flux_bc<<<1, cs::kRedThreads, 0, cs::as_stream(stream)>>>(*a);
-    CS_LAUNCHED("k_flux_bc");
+    CS_REQUIRE(a->m >= 0 && (a->m == 0 || (a->idx && a->w && a->slot && a->vel)), "cs_flux_bc: null cell list");
+    const int64_t per = (a->m + (int64_t)cs::kRedThreads * 256 - 1) / ((int64_t)cs::kRedThreads * 256);
+    const int64_t chunk = (int64_t)cs::kRedThreads * (per < 1 ? 1 : per);
+    const int64_t G = (a->m + chunk - 1) / chunk;
+    CS_REQUIRE(a->work_chunks >= 1 && G <= a->work_chunks, "cs_flux_bc: %lld chunks need work_chunks >= %lld",
+               (long long)G, (long long)(G < 1 ? 1 : G));
+    if (G > 0) {
+        cs::k_flux_partial<<<(unsigned)G, cs::kRedThreads, 0, cs::as_stream(stream)>>>(*a);
+        CS_LAUNCHED("k_flux_partial");
+    }
+    cs::k_flux_final<<<1, 64, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_flux_final");
     return CS_OK;
 }
 
@@ -264,14 +326,17 @@ extern "C" int cs_clock_cases(const cs_clock_cases_args* a, void* stream) {
 
 extern "C" int cs_flux_cases(const cs_flux_cases_args* a, void* stream) {
     CS_REQUIRE(a && a->start && a->R && a->drive && a->p_out && a->q_last && a->n_open && a->pref && a->alpha &&
-                   a->vel,
+                   a->vel && a->work,
                "cs_flux_cases: null argument");
     CS_REQUIRE(a->n_cases >= 0 && a->n_cases <= CS_MAX_CASES && a->open_stride >= 1 &&
-                   a->open_stride <= cs::kMaxOpen,
-               "cs_flux_cases: n_cases %d / open_stride %d", a->n_cases, a->open_stride);
+                   a->open_stride <= cs::kMaxOpen && a->work_chunks >= 1 && a->work_chunks <= 256,
+               "cs_flux_cases: n_cases %d / open_stride %d / work_chunks %d", a->n_cases, a->open_stride,
+               a->work_chunks);
     if (a->n_cases == 0) return CS_OK;
-    cs::k_flux_cases<<<a->n_cases, cs::kRedThreads, 0, cs::as_stream(stream)>>>(*a);
-    CS_LAUNCHED("k_flux_cases");
+    cs::k_flux_partial_cases<<<dim3(a->work_chunks, a->n_cases), cs::kRedThreads, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_flux_partial_cases");
+    cs::k_flux_final_cases<<<a->n_cases, 64, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_flux_final_cases");
     return CS_OK;
 }
 

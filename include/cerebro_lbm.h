@@ -322,6 +322,9 @@ int cs_bricks_set_f(const cs_bricks_convert_args* a, void* stream);
  * Per-step flow measurement and boundary update -- replaces k_flux + k_bc (simulate.py:65-82) launched after each
  * step at simulate.py:111-114. Q[slot] = sum over the M slab cells of vel[idx] . w (deterministic order), then
  * p_out[0] = drive[0]; for k in 1..n_open-1: p_out[k] += alpha (R[k] Q[k] - pref - p_out[k]); q_last = Q.
+ * The M cells are summed in G chunks, G a function of M alone: P = max(1, ceil(M / 262144)) cells per thread,
+ * chunks of 1024 P cells, G = ceil(M / (1024 P)) <= 256. work holds n_open x work_chunks floats (chunk sums,
+ * overwritten each call), work_chunks >= max(G, 1).
  * Cells are given as storage slots of the layout in use; component c of slot s is vel[s * vel_elem + c * vel_comp]
  * ((N, 3) AoS: vel_elem = 3, vel_comp = 1; component planes of S slots: vel_elem = 1, vel_comp = S).
  * ------------------------------------------------------------------------------------------------------------- */
@@ -340,12 +343,15 @@ typedef struct cs_flux_args {
     int32_t n_open;          /* at most 64 */
     float pref;
     float alpha;
+    float* work;             /* device, [n_open][work_chunks] */
+    int32_t work_chunks;
 } cs_flux_args;
 int cs_flux_bc(const cs_flux_args* a, void* stream);
 
-/* k_flux + k_bc for packed lattices: one workgroup per case c with n_open[c] > 0, over the slab cells
- * start[c] .. start[c+1]-1 of the concatenated lists; per-case arrays at c * open_stride. Same arithmetic and
- * summation order as cs_flux_bc on the case alone. */
+/* k_flux + k_bc for packed lattices: every case c with n_open[c] > 0, over the slab cells start[c] .. start[c+1]-1
+ * of the concatenated lists; per-case arrays at c * open_stride. Same chunks, arithmetic and summation order as
+ * cs_flux_bc on the case alone. work holds n_cases x open_stride x work_chunks floats; work_chunks (at most 256)
+ * must cover every case's G, else that case's flows and pressures are written as NaN. */
 typedef struct cs_flux_cases_args {
     const int32_t* idx;
     const float* w;
@@ -363,6 +369,8 @@ typedef struct cs_flux_cases_args {
     const float* alpha;         /* [n_cases] */
     int32_t n_cases;
     int32_t open_stride;        /* at most 64 */
+    float* work;                /* device, [n_cases][open_stride][work_chunks] */
+    int32_t work_chunks;
 } cs_flux_cases_args;
 int cs_flux_cases(const cs_flux_cases_args* a, void* stream);
 

@@ -52,6 +52,7 @@ STRUCTS = {"cs_clock_args": _native.ClockArgs, "cs_step_args": _native.StepArgs,
            "cs_bricks_sizes": _native.BricksSizes, "cs_bricks_step_args": _native.BricksStepArgs,
            "cs_bricks_convert_args": _native.BricksConvertArgs, "cs_clock_cases_args": _native.ClockCasesArgs,
            "cs_flux_cases_args": _native.FluxCasesArgs, "cs_clock_table_args": _native.ClockTableArgs,
+           "cs_sdf_tubes_args": _native.SdfTubesArgs,
            "cs_opening": _native.Opening, "cs_grid_in": _native.GridIn}
 
 
@@ -83,3 +84,12 @@ def test_solver_refuses_cpu():
     lat, _, _ = periodic_pipe(6)
     with pytest.raises(RuntimeError, match="GPU"):
         lbm.Solver(lat, 0.1)
+
+
+def test_flux_chunks_cover_the_cells():
+    """The flow kernels' chunking (cells per thread P = max(1, ceil(m / 262144)), chunks of 1024 P cells): at most
+    256 chunks, and together they cover exactly the m cells."""
+    for m in (0, 1, 1023, 1024, 1025, 16000, 150000, 262144, 262145, 1_000_000, 5_000_000):
+        G = _native.flux_chunks(m)
+        per = max(1, -(-m // 262144))
+        assert G <= 256 and (G - 1) * 1024 * per < max(m, 1) <= G * 1024 * per or (m == 0 and G == 0)
