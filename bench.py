@@ -19,6 +19,10 @@ The chosen forms are measured after the headline (``--forms``, default all three
   FP16 under the HBM budget -- cases/hour with p50/p99 time-to-solution -- against the same requests solved one
   after the other by the plain FP32 solver (both include building each lattice on the host).
 
+``artery`` (skip with ``--no-artery``): the reference's own speed benchmark, test 5 of validate.py on the story
+artery (healthy, 0.1 mm, 620,028 cells, 1000 + 20000 steps), beside its published RTX 3090 figures
+(geometry/solver.json:179-194, reports/step3_report.md:58-64), with the time per heartbeat derived the same way.
+
 Prints one JSON line. ``roofline.achieved`` = algorithmic bytes of the step kernel per launch (244 B per fluid cell:
 27 populations read + 27 written x 4 B, previous velocity read 12 B, velocity + density written 16 B) / the kernel's
 mean duration measured with HIP events on the solver's stream; ``copy_gbs`` is a 1 GiB device copy's (read + write)
@@ -311,6 +315,26 @@ def pmc_probe(layout, storage):
     print("pmc probe done", flush=True)
 
 
+# the reference's published speed on the story artery (RTX 3090): solver.json:179-194, step3_report.md:58-64
+ARTERY_PUBLISHED = {"mlups": 384.9, "minutes_per_beat": {"healthy": 5.4, "narrow70": 13.4, "narrow70_fine": 214.8}}
+
+
+def artery_speed():
+    """validate.py's test 5 on the story artery (the segment file committed with the tests), FP32."""
+    from cerebrosim_amd import validate
+    seg = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "segment_story.json")
+    res, beat = validate.artery(seg)
+    pub = validate.PUBLISHED_ARTERY
+    return {"workload": "story_artery_healthy_0.1mm_validate_test5", "cells": res["cells"],
+            "steps": res["steps"], "mlups": round(res["mlups"], 1), "ok": res["ok"],
+            "published_rtx3090_mlups": ARTERY_PUBLISHED["mlups"],
+            "speedup_vs_published": round(res["mlups"] / ARTERY_PUBLISHED["mlups"], 1),
+            "counts_match_published": res["counts"] == pub["counts"] and res["cells"] == pub["cells"],
+            "u_max_m_s": res["u_max_m_s"], "published_u_max_m_s": pub["u_max_m_s"],
+            "minutes_per_beat": {k: round(v["minutes_per_beat"], 3) for k, v in beat.items()},
+            "published_minutes_per_beat": ARTERY_PUBLISHED["minutes_per_beat"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -325,6 +349,7 @@ def main():
                     help="comma-separated chosen forms to measure after the headline ('none' to skip)")
     ap.add_argument("--full", action="store_true", help="full-size batching / serving workloads")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch variant")
+    ap.add_argument("--no-artery", action="store_true", help="skip the story-artery speed comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (the step of one lattice "
@@ -388,6 +413,10 @@ def main():
         if "serving" in forms:
             out["forms"]["serving"] = form_serving(args.full)
             print(f"bench: serving form done", file=sys.stderr, flush=True)
+    if not args.no_artery:
+        torch.cuda.empty_cache()
+        out["artery"] = artery_speed()
+        print(f"bench: artery {out['artery']['mlups']} MLUPS", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

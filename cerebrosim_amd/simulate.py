@@ -152,3 +152,42 @@ def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, 
                         p_mmhg=np.array(frames_p), u_m_s=np.array(frames_u))
     log(f"{case}: done in {secs:.1f} s ({int(lat['nbr'].shape[1])} cells, {s.steps} steps)")
     return ser, meta
+
+
+def main(argv=None):
+    """Command line of the reference's simulate.py for the story artery:
+
+        python -m cerebrosim_amd.simulate CASE --segment segment.json [--out DIR] [--beats N] [--cal meta.json]
+
+    The healthy case calibrates its inlet; other cases take the calibration from a healthy run's meta.json
+    (--cal), as simulate.py:323-332 does. --heart-period shortens the beat (seconds) for quick runs."""
+    import argparse
+
+    from .vessel import GRIDS, artery_case
+    ap = argparse.ArgumentParser(description="simulate the story artery over heartbeats")
+    ap.add_argument("case", choices=sorted(GRIDS))
+    ap.add_argument("--segment", required=True, help="the story's segment file")
+    ap.add_argument("--out", default=None, help="output folder (default results/CASE)")
+    ap.add_argument("--beats", type=int, default=BEATS)
+    ap.add_argument("--cal", default=None, help="meta.json of a healthy run (its calibration drives this case)")
+    ap.add_argument("--heart-period", type=float, default=HEART_PERIOD)
+    ap.add_argument("--storage", default="fp32", choices=("fp32", "fp16"))
+    args = ap.parse_args(argv)
+    cal = None
+    if args.cal:
+        with open(args.cal) as fh:
+            cal = json.load(fh)["calibration"]
+    elif args.case != "healthy":
+        raise SystemExit(f"{args.case}: pass --cal with a healthy run's meta.json")
+    lat, ends, probes = artery_case(args.case, segment=args.segment)
+    dx, dt = GRIDS[args.case]
+    dt = SAMPLE_S / round(SAMPLE_S / dt)
+    ser, meta = run(lat, ends, probes, dx, dt, args.out or os.path.join("results", args.case), case=args.case,
+                    beats=args.beats, cal=cal, heart_period=args.heart_period, progress_every=100,
+                    storage=args.storage)
+    last = ser["t"] > ser["t"][-1] - args.heart_period
+    print(f"{args.case}: last beat mean flows " + ", ".join("%.1f" % q for q in ser["Q"][last].mean(0)) + " mL/min")
+
+
+if __name__ == "__main__":
+    main()
