@@ -133,15 +133,25 @@ __device__ __forceinline__ void flux_final(const float* work, int wc, int64_t G,
                                            float alpha) {
     const int lane = threadIdx.x;
     if (lane == 0) p_out[0] = drive;
-    for (int k = 0; k < n_open; ++k) {
-        float s = 0.0f;
-        if (valid)
-            for (int64_t b = lane; b < G; b += 64) s += work[(int64_t)k * wc + b];
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) {
-            const float q = valid ? s : __builtin_nanf("");
-            q_last[k] = q;
-            if (k > 0) p_out[k] = p_out[k] + alpha * (R[k] * q - pref - p_out[k]);
+    for (int k0 = 0; k0 < n_open; k0 += kFluxGroup) {
+        float s[kFluxGroup];   // the group's loads are issued together; each sum keeps its order
+#pragma unroll
+        for (int q = 0; q < kFluxGroup; ++q) {
+            s[q] = 0.0f;
+            if (valid && k0 + q < n_open)
+                for (int64_t b = lane; b < G; b += 64) s[q] += work[(int64_t)(k0 + q) * wc + b];
+        }
+#pragma unroll
+        for (int q = 0; q < kFluxGroup; ++q) {
+            if (k0 + q >= n_open) break;   // wave-uniform
+            float t = s[q];
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+            if (lane == 0) {
+                const int k = k0 + q;
+                const float qk = valid ? t : __builtin_nanf("");
+                q_last[k] = qk;
+                if (k > 0) p_out[k] = p_out[k] + alpha * (R[k] * qk - pref - p_out[k]);
+            }
         }
     }
 }

@@ -22,6 +22,9 @@ How each function is pinned (so that a plausible mistake fails a check that is n
   the tilted open pipe's published pressure-gradient error (geometry/solver.json:92-112, slow test) and by the
   brute-force moment checks of ``_h3``.
 * ``orc_flux_bc`` / ``orc_gather`` (simulate.py:65-90): closed forms on hand-made inputs (sums, relaxation update).
+* ``oracle/sdf.py`` (vessel.py:183-207, the vessel signed distance): closed forms on a capped cylinder, a cone and
+  the smooth minimum (tests/test_vessel_host.py); the lattice built from the checked HIP kernel reproduces the
+  published artery counts (geometry/solver.json:179-194).
 * The TRT branch (lbm.py:241-255) is pinned only by its invariants (mass, momentum, rest state): the reference
   publishes no TRT result. Parity of TRT against the reference is therefore "parity unpinned" beyond invariants.
 """
