@@ -8,6 +8,7 @@
 * The simulate.py runs of the healthy case (4 heartbeats at 0.1 mm / 5 us, inlet calibrated on the run) and of the
   occluded case (3 heartbeats on the healthy calibration) against reports/step4_report.md:7-21: mean inflow, MCA and
   ACA flows, peak speed and the probe pressures over the last beat.
+* The healthy run again with FP16 population storage (the precision form on this caller) against the FP32 run.
 """
 import json
 import os
@@ -57,14 +58,14 @@ def test_artery_lattice_and_speed_match_the_published_run(ref):
     assert beat["healthy"]["steps_per_beat"] == pytest.approx(200000)
 
 
-def _run_case(case, tmp, cal=None):
+def _run_case(case, tmp, cal=None, storage="fp32"):
     from cerebrosim_amd import simulate
     from cerebrosim_amd import vessel as V
     lat, ends, probes = V.artery_case(case, segment=SEGMENT)
     dx, dt = V.GRIDS[case]
     beats = {"healthy": 4, "occluded": 3}[case]                  # the report's runs (step4_report.md:7-9)
     ser, meta = simulate.run(lat, ends, probes, dx, dt, str(tmp), case=case, beats=beats, cal=cal,
-                             log=lambda m: print(m, flush=True))
+                             log=lambda m: print(m, flush=True), storage=storage)
     return lat, probes, ser, meta
 
 
@@ -105,3 +106,22 @@ def test_occluded_heartbeats_match_the_step4_report(ref, healthy_run, tmp_path):
     lat, probes, ser, meta = _run_case("occluded", tmp_path, cal=healthy_run[3]["calibration"])
     assert os.path.exists(tmp_path / "fields.npz")
     _check_last_beat(ser, probes, ref["step4_occluded"])
+
+
+def test_healthy_heartbeats_in_fp16_storage(healthy_run, tmp_path):
+    """The precision form on the simulate.py caller: the same 4 calibrated heartbeats with the populations stored
+    as FP16 equilibrium deviations (FP32 arithmetic), against the FP32 run. Series relative L2 over the last beat."""
+    _, probes, ser32, meta32 = healthy_run
+    _, _, ser16, meta16 = _run_case("healthy", tmp_path, storage="fp16")
+    last = ser32["t"] > ser32["t"][-1] - 1.0
+    assert np.array_equal(ser16["t"], ser32["t"])
+
+    def rel(a, b):
+        return float(np.sqrt(np.nansum((a - b) ** 2) / np.nansum(b ** 2)))
+
+    errs = {"Q": rel(ser16["Q"][last], ser32["Q"][last]),
+            "p_probe": rel(ser16["p_probe"][last], ser32["p_probe"][last]),
+            "umax": rel(ser16["umax"][last], ser32["umax"][last])}
+    print(f"FP16 vs FP32 storage, last heartbeat, relative L2: {errs}; inlet calibration R_seg "
+          f"{meta16['calibration']['R_seg']:.4g} vs {meta32['calibration']['R_seg']:.4g}", flush=True)
+    assert errs["Q"] < 1e-3 and errs["p_probe"] < 1e-3 and errs["umax"] < 1e-3, errs
