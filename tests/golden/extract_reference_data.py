@@ -7,7 +7,9 @@ reference code is run) into tests/golden/reference_data.json:
 * geometry/solver.json:"artery": the healthy artery at 0.1 mm (validate.py:311-340): cells, link counts, openings,
   steps, speed and the maximum speed after them;
 * reports/step4_report.md:7-21: each case's grid, beats, mean inflow, MCA and ACA flows, peak speed and probe
-  pressures (None where the report says there is no lumen).
+  pressures (None where the report says there is no lumen);
+* reports/step5_report.md:40-44: each case's maximum speed in the stored fields (40 frames of the last beat, 0.1 mm
+  bins).
 
     python tests/golden/extract_reference_data.py [/path/to/reference]
 """
@@ -49,6 +51,10 @@ def main(ref):
                              "inflow_mL_min": float(row.group(3)), "mca_mL_min": float(row.group(4)),
                              "aca_mL_min": float(row.group(5)), "peak_speed_m_s": float(row.group(6)),
                              "probe_mmHg": probes[c], "source": "reports/step4_report.md:7-21"}
+    step5 = open(os.path.join(ref, "reports", "step5_report.md")).read()
+    for c in cases:
+        row = re.search(r"\| " + c + r" \| ([\d.]+) m/s \| ([\d.]+) m/s \| ([\d.]+) m/s \|", step5)
+        out["step4_" + c]["fields_max_speed_m_s"] = float(row.group(2))
     json.dump(out, open(os.path.join(HERE, "reference_data.json"), "w"), indent=1)
     print("wrote reference_data.json")
 

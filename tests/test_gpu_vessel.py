@@ -7,7 +7,8 @@
 * Test 5 of validate.py (validate.py:311-340): the maximum speed after 21000 steps against the published value.
 * The simulate.py runs of the healthy case (4 heartbeats at 0.1 mm / 5 us, inlet calibrated on the run) and of the
   occluded case (3 heartbeats on the healthy calibration) against reports/step4_report.md:7-21: mean inflow, MCA and
-  ACA flows, peak speed and the probe pressures over the last beat.
+  ACA flows, peak speed and the probe pressures over the last beat, and the largest speed in the stored frames
+  against reports/step5_report.md:40-44.
 * The healthy run again with FP16 population storage (the precision form on this caller) against the FP32 run.
 """
 import json
@@ -66,12 +67,22 @@ def _run_case(case, tmp, cal=None, storage="fp32"):
     beats = {"healthy": 4, "occluded": 3}[case]                  # the report's runs (step4_report.md:7-9)
     ser, meta = simulate.run(lat, ends, probes, dx, dt, str(tmp), case=case, beats=beats, cal=cal,
                              log=lambda m: print(m, flush=True), storage=storage)
-    return lat, probes, ser, meta
+    return lat, probes, ser, meta, tmp
 
 
 @pytest.fixture(scope="module")
 def healthy_run(tmp_path_factory):
     return _run_case("healthy", tmp_path_factory.mktemp("healthy"))
+
+
+def _check_fields(out_dir, pub, tol=0.02):
+    """reports/step5_report.md:40-44: the largest speed in the stored frames (fields.npz, 40 frames of the last
+    beat on 0.1 mm bins, float16), the figure the reference's renderer reads."""
+    f = np.load(os.path.join(str(out_dir), "fields.npz"))
+    assert f["u_m_s"].shape[0] == 40 and f["u_m_s"].dtype == np.float16
+    vmax = float(np.linalg.norm(f["u_m_s"].astype(np.float32), axis=2).max())
+    print(f"stored fields: max speed {vmax:.3f} m/s (published {pub['fields_max_speed_m_s']})", flush=True)
+    assert abs(vmax - pub["fields_max_speed_m_s"]) < tol
 
 
 def _check_last_beat(ser, probes, pub, tol_q=1.0, tol_p=0.15, tol_u=0.01):
@@ -95,24 +106,25 @@ def _check_last_beat(ser, probes, pub, tol_q=1.0, tol_p=0.15, tol_u=0.01):
 
 
 def test_healthy_heartbeats_match_the_step4_report(ref, healthy_run):
-    lat, probes, ser, meta = healthy_run
+    lat, probes, ser, meta, out = healthy_run
     assert lat["nbr"].shape[1] == ref["artery"]["cells"] and meta["cells"] == ref["artery"]["cells"]
     _check_last_beat(ser, probes, ref["step4_healthy"])
+    _check_fields(out, ref["step4_healthy"])
 
 
 def test_occluded_heartbeats_match_the_step4_report(ref, healthy_run, tmp_path):
     """The full occlusion with the healthy run's calibrated inlet pressure and resistances (simulate.py:323-332):
     the lumen splits at the clot, the MCA carries nothing and the ACA takes the inflow."""
-    lat, probes, ser, meta = _run_case("occluded", tmp_path, cal=healthy_run[3]["calibration"])
-    assert os.path.exists(tmp_path / "fields.npz")
+    lat, probes, ser, meta, out = _run_case("occluded", tmp_path, cal=healthy_run[3]["calibration"])
     _check_last_beat(ser, probes, ref["step4_occluded"])
+    _check_fields(out, ref["step4_occluded"])
 
 
 def test_healthy_heartbeats_in_fp16_storage(healthy_run, tmp_path):
     """The precision form on the simulate.py caller: the same 4 calibrated heartbeats with the populations stored
     as FP16 equilibrium deviations (FP32 arithmetic), against the FP32 run. Series relative L2 over the last beat."""
-    _, probes, ser32, meta32 = healthy_run
-    _, _, ser16, meta16 = _run_case("healthy", tmp_path, storage="fp16")
+    _, probes, ser32, meta32, _ = healthy_run
+    _, _, ser16, meta16, _ = _run_case("healthy", tmp_path, storage="fp16")
     last = ser32["t"] > ser32["t"][-1] - 1.0
     assert np.array_equal(ser16["t"], ser32["t"])
 
