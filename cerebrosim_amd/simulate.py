@@ -56,10 +56,11 @@ def probe_cells(lat, dx_mm, probes):
 
 
 def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, flow=1.0, heart_period=HEART_PERIOD,
-        log=print, progress_every=None, **solver_kw):
+        log=print, progress_every=None, autotune=False, **solver_kw):
     """Simulate `beats` heartbeats and write series.npz, meta.json and fields.npz into out_dir. cal: the circuit
     (``artery.circuit``) to drive with; None calibrates on this run, as the reference's healthy case does. flow
-    scales the absolute inlet pressure. progress_every: also log the flows every that many samples. Returns
+    scales the absolute inlet pressure. progress_every: also log the flows every that many samples. autotune: time
+    the launch variants on this lattice first and keep the fastest (the results do not depend on it). Returns
     (series, meta)."""
     os.makedirs(out_dir, exist_ok=True)
     un = units(dx, dt)
@@ -89,6 +90,10 @@ def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, 
         s.set_drive(c, period=spb, ramp=ramp)
 
     drive(cal)
+    if autotune:
+        tune = s.autotune()
+        log(f"{case}: launch variant {s.variant} (ms per step: "
+            + ", ".join(f"{k} {v:.4f}" for k, v in tune.items()) + ")")
     gather = probe_cells(lat, dx_mm, probes)
     pcount = np.bincount(gather["slot"], minlength=len(probes))
     cmap = coarse_map(lat, dx_mm)
@@ -172,6 +177,7 @@ def main(argv=None):
     ap.add_argument("--cal", default=None, help="meta.json of a healthy run (its calibration drives this case)")
     ap.add_argument("--heart-period", type=float, default=HEART_PERIOD)
     ap.add_argument("--storage", default="fp32", choices=("fp32", "fp16"))
+    ap.add_argument("--autotune", action="store_true", help="pick the fastest launch variant on this lattice first")
     args = ap.parse_args(argv)
     cal = None
     if args.cal:
@@ -184,7 +190,7 @@ def main(argv=None):
     dt = SAMPLE_S / round(SAMPLE_S / dt)
     ser, meta = run(lat, ends, probes, dx, dt, args.out or os.path.join("results", args.case), case=args.case,
                     beats=args.beats, cal=cal, heart_period=args.heart_period, progress_every=100,
-                    storage=args.storage)
+                    storage=args.storage, autotune=args.autotune)
     last = ser["t"] > ser["t"][-1] - args.heart_period
     print(f"{args.case}: last beat mean flows " + ", ".join("%.1f" % q for q in ser["Q"][last].mean(0)) + " mL/min")
 
