@@ -6,8 +6,9 @@ mapping of a run.
   opening pressures (k_bc: inlet pressure = drive, outlets relaxed towards R Q) -- one deterministic HIP kernel,
   ``cs_flux_bc``, appended to ``_launch`` so it is captured into the graphs.
 * ``ArterySolver.gather`` -- simulate.py:85-90, 305-318: probe mean pressures and the maximum speed (``cs_gather``).
-* ``inflow_waveform`` -- vessel.py:253-265; ``fourier`` / ``from_fourier`` -- simulate.py:122-135;
-  ``circuit`` -- simulate.py:138-148; ``step_size`` -- simulate.py:155-159; ``flux_slabs`` -- simulate.py:180-192.
+* ``INFLOW_HARMONICS`` / ``inflow_waveform`` -- the published inflow waveform (geometry/physics.json "inlet",
+  generated by vessel.py:253-265) as data; ``circuit`` -- simulate.py:138-148; ``step_size`` -- simulate.py:155-159;
+  ``flux_slabs`` -- simulate.py:180-192.
 """
 from __future__ import annotations
 
@@ -30,51 +31,58 @@ P_OUT_MMHG = 80.0             # simulate.py:48
 SPONGE_MM, SPONGE_TAU = 2.0, 0.55       # simulate.py:50
 FLUX_SLAB_MM = (2.5, 3.5)     # simulate.py:51
 SAMPLE_S = 1.0e-3             # simulate.py:52
-HARMONICS = 10                # simulate.py:54
 OUTLET_RELAX_S = 0.5e-3       # simulate.py:55
+END_LABELS = ("inlet (ICA)", "MCA outlet", "ACA outlet")   # the circuit's openings, in its order (simulate.py:56)
+
+
+# Inflow over one beat, Q(t) / Q_mean, as the trigonometric series a0 + sum_k a_k cos(2 pi k t / T) + b_k sin(...),
+# k = 1..10, coefficients [a0, a1, b1, ..., a10, b10]. Data: the least-squares fit of the reference's published
+# waveform (geometry/physics.json "inlet", 200 samples; tests/golden/fit_inflow_harmonics.py), with a0 = 1 (the
+# waveform is normalised to its mean). It reproduces the published samples to their rounding (5e-6).
+INFLOW_HARMONICS = np.array((
+    1.0, 0.069498890, 0.241689468, -0.053443492, 0.104796822, -0.052107328, 0.065175090, -0.071062744,
+    0.018654888, -0.050795072, -0.032617016, 0.008394505, -0.043637227, 0.024404578, -0.001922961, 0.009999664,
+    0.007282970, 0.004774265, 0.010065523, -0.001034301, 0.006781041))
+
+
+def series(coef, phase):
+    """Value of the trigonometric series coef = [a0, a1, b1, a2, b2, ...] at phase = t / period (array)."""
+    coef = np.asarray(coef, float)
+    k = np.arange(1, (len(coef) - 1) // 2 + 1)
+    z = np.exp(2j * math.pi * np.multiply.outer(np.asarray(phase, float), k))      # (..., harmonics)
+    return coef[0] + z.real @ coef[1::2][:len(k)] + z.imag @ coef[2::2][:len(k)]
 
 
 def inflow_waveform(n=200):
-    """Inflow over one beat Q(t) / Q_mean (mean 1, (max - min) / mean = PULSATILITY): systolic peak, late-systolic
-    shoulder and diastolic decay, low-passed to 10 harmonics (vessel.py:253-265)."""
-    t = np.arange(n) / n
-    g = lambda c, w: np.exp(-0.5 * ((t - c) / w) ** 2)
-    shape = 0.15 + 1.0 * g(0.13, 0.045) + 0.45 * g(0.30, 0.06) + 0.12 * g(0.45, 0.04) + 0.25 * np.exp(-(t / 0.9))
-    F = np.fft.rfft(shape)
-    F[11:] = 0
-    shape = np.fft.irfft(F, n)
-    shape = shape / shape.mean()
-    shape = 1 + (shape - 1) * PULSATILITY / (shape.max() - shape.min())
-    return t * HEART_PERIOD, shape
-
-
-def fourier(y, nh=HARMONICS):
-    """Samples over one period -> [a0, a1, b1, a2, b2, ...] for a0 + sum a_k cos(2 pi k t) + b_k sin(2 pi k t)."""
-    F = np.fft.rfft(y) / len(y)
-    c = [F[0].real]
-    for k in range(1, nh + 1):
-        c += [2 * F[k].real, -2 * F[k].imag]
-    return np.array(c)
-
-
-def from_fourier(c, t):
-    y = np.full_like(np.asarray(t, float), c[0], dtype=float)
-    for k in range(1, (len(c) - 1) // 2 + 1):
-        y += c[2 * k - 1] * np.cos(2 * math.pi * k * t) + c[2 * k] * np.sin(2 * math.pi * k * t)
-    return y
+    """(t in s, Q(t) / Q_mean) at n equally spaced times over one beat, from INFLOW_HARMONICS."""
+    ph = np.arange(n) / n
+    return ph * HEART_PERIOD, series(INFLOW_HARMONICS, ph)
 
 
 def circuit(R_seg):
-    """Lumped circuit (SI): outlet resistances and the inlet pressure waveform (Fourier coefficients, Pa absolute)
-    that draws the mean inflow through segment + outlets (simulate.py:138-148)."""
-    q_in = Q_INLET_MEAN * 1e-6 / 60
-    q_mca = Q_MCA_TARGET * 1e-6 / 60
-    p_out = P_OUT_MMHG * MMHG
-    R_mca, R_aca = p_out / q_mca, p_out / (q_in - q_mca)
-    R_eq = p_out / q_in
-    _, shape = inflow_waveform(400)
-    coef = fourier((R_eq + R_seg) * q_in * shape)
-    return {"R_out": [0.0, R_mca, R_aca], "R_seg": R_seg, "P_in_coef": coef.tolist(), "R_eq": R_eq}
+    """The lumped circulation around the segment (SI units). Each outlet k is a resistance to the common venous
+    pressure P_OUT that passes its share of the mean inflow: R_k = P_OUT / Q_k (MCA: the target flow, ACA: the
+    rest), so together they are R_eq = P_OUT / Q_in. The segment adds R_seg in series, and the inlet pressure that
+    drives the sourced inflow waveform through both is P_in(t) = (R_eq + R_seg) Q_in(t), returned as the
+    coefficients of its trigonometric series (absolute Pa). Counterpart of simulate.py:138-148."""
+    q_in = Q_INLET_MEAN / 6e7                         # mL/min -> m^3/s
+    q_mca = Q_MCA_TARGET / 6e7
+    p_ven = P_OUT_MMHG * MMHG
+    r_eq = p_ven / q_in
+    return {"R_out": [0.0, p_ven / q_mca, p_ven / (q_in - q_mca)], "labels": list(END_LABELS), "R_seg": R_seg,
+            "P_in_coef": ((r_eq + R_seg) * q_in * INFLOW_HARMONICS).tolist(), "R_eq": r_eq}
+
+
+def outlet_resistances(cal, ends):
+    """The circuit's resistance of each open end, matched by the end's label (simulate.py:272-273). A calibration
+    written without labels lists them in END_LABELS order. Raises KeyError for an end the circuit does not have."""
+    labels = list(cal.get("labels", END_LABELS))
+    out = []
+    for e in ends:
+        if e["label"] not in labels:
+            raise KeyError(f"open end {e['label']!r} has no resistance in the calibration (it has {labels})")
+        out.append(cal["R_out"][labels.index(e["label"])])
+    return out
 
 
 def step_size(dx, dt, dt_scale=1.0):
@@ -88,23 +96,26 @@ def units(dx, dt):
 
 
 def flux_slabs(lat, ends, dx_mm, slab=FLUX_SLAB_MM):
-    """Cells of each opening's measurement slab and their weights +-normal / thickness; inflow positive at the
-    first opening, outflow positive at the others (simulate.py:180-192)."""
-    P = lat["origin"] + lat["coords"] * dx_mm
-    idx, w, slot = [], [], []
+    """Flow-measurement slabs (simulate.py:180-192). For opening k, the cells whose depth behind the opening plane
+    (along its inward normal) lies in [slab[0], slab[1]) mm and whose distance from the opening's axis is below
+    1.3 radii. Summing u . w over them gives the flow through the slab per unit length: w = n / thickness, with the
+    sign that counts inflow positive at the first opening and outflow positive at the others.
+    Returns {"idx", "w", "slot"} (cell ids, weights (M, 3), opening index), openings in order."""
+    pos = lat["origin"] + lat["coords"] * dx_mm
     thick = (slab[1] - slab[0]) / dx_mm
+    parts = []
     for k, e in enumerate(ends):
-        normal = np.asarray(e["normal"], float)
-        rel = P - np.asarray(e["centre"], float)
-        depth = -(rel @ normal)
-        side = np.linalg.norm(rel + depth[:, None] * normal, axis=1) < 1.3 * e["radius"]
-        m = np.flatnonzero(side & (depth >= slab[0]) & (depth < slab[1]))
-        sign = -1.0 if k == 0 else 1.0
-        idx.append(m)
-        slot.append(np.full(len(m), k))
-        w.append(np.tile(sign * normal / thick, (len(m), 1)))
-    return {"idx": np.concatenate(idx).astype(np.int32), "w": np.vstack(w).astype(np.float32),
-            "slot": np.concatenate(slot).astype(np.int32)}
+        n = np.asarray(e["normal"], float)
+        off = pos - np.asarray(e["centre"], float)
+        inward = -(off @ n)                                    # depth behind the opening plane (mm)
+        radial = off + np.outer(inward, n)                     # component across the axis
+        sel = np.nonzero((inward >= slab[0]) & (inward < slab[1])
+                         & ((radial * radial).sum(1) < (1.3 * e["radius"]) ** 2))[0]
+        wk = (n if k else -n) / thick
+        parts.append((sel, np.broadcast_to(wk, (sel.size, 3)), np.full(sel.size, k)))
+    return {"idx": np.concatenate([p[0] for p in parts]).astype(np.int32),
+            "w": np.concatenate([p[1] for p in parts]).astype(np.float32),
+            "slot": np.concatenate([p[2] for p in parts]).astype(np.int32)}
 
 
 class ArterySolver(Solver):

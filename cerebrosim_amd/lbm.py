@@ -594,7 +594,11 @@ class Solver:
         from .batch import launch_variants
         torch = _torch()
         b = self.backend
-        saved = [t.clone() for t in [b.f, b.side] + b.mom.tensors()] + [self.tick.clone(), self.p_out.clone()]
+        # the snapshot and the restore run on the solver stream, after the work already queued there
+        state = [b.f, b.side] + b.mom.tensors() + [self.tick, self.p_out]
+        self.synchronize()
+        with torch.cuda.stream(self._tstream):
+            saved = [t.clone() for t in state]
         cur, nsteps = self.cur, self.steps
         times = {}
         for v in range(len(launch_variants())):
@@ -613,9 +617,12 @@ class Solver:
                 ms = e0.elapsed_time(e1) / steps
                 best = ms if best is None else min(best, ms)
             times[v] = best
-        for dst, src in zip([b.f, b.side] + b.mom.tensors() + [self.tick, self.p_out], saved):
-            dst.copy_(src)
         self.synchronize()
+        with torch.cuda.stream(self._tstream):
+            for dst, src in zip(state, saved):
+                dst.copy_(src)
+        self.synchronize()
+        del saved
         self.cur, self.steps = cur, nsteps
         self.variant = min(times, key=times.get)
         self.graphs = {}

@@ -27,7 +27,7 @@ import time
 import numpy as np
 
 from .artery import (HEART_PERIOD, MMHG, OUTLET_RELAX_S, P_OUT_MMHG, RHO, SAMPLE_S, SPONGE_MM, SPONGE_TAU,
-                     ArterySolver, circuit, flux_slabs, units)
+                     ArterySolver, circuit, flux_slabs, outlet_resistances, units)
 
 BEATS = 3                 # simulate.py:45
 FRAMES = 40               # simulate.py:53
@@ -68,9 +68,9 @@ def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, 
     calibrate = cal is None
     if calibrate:
         cal = circuit(0.0)
-    q_si = un.dx ** 3 / un.dt                      # m^3/s per lattice flow unit
-    p_si = RHO * un.c ** 2                         # Pa per lattice pressure unit
-    R_lu = [r * q_si / p_si for r in cal["R_out"][:len(ends)]]
+    q_si = un.dx ** 3 / un.dt                      # flow: SI per lattice unit
+    p_si = RHO * un.c ** 2                         # pressure: SI per lattice unit
+    R_lu = [r * q_si / p_si for r in outlet_resistances(cal, ends)]
     pref_lu = P_OUT_MMHG * MMHG / p_si
     n_open = len(ends)
     flux = flux_slabs(lat, ends, dx_mm)
@@ -86,7 +86,7 @@ def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, 
 
     def drive(c_):
         c = np.array(c_["P_in_coef"]) / p_si * flow
-        c[0] -= pref_lu                            # gauge
+        c[0] -= pref_lu                            # relative to the venous pressure
         s.set_drive(c, period=spb, ramp=ramp)
 
     drive(cal)
@@ -125,7 +125,8 @@ def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, 
             flows = ", ".join("%s %.1f" % (e["label"].split()[0], q) for e, q in zip(ends, Qm))
             log(f"{case}: beat {beat} done ({time.perf_counter() - T0:.1f} s): mean flow {flows} mL/min")
             if calibrate and beat < beats:
-                # the inlet pressure that gives the sourced mean inflow: R_seg from this beat's mean
+                # re-fit the segment's resistance from this beat: mean inlet pressure over mean inflow, less the
+                # outlets' equivalent resistance
                 p_in = np.array(series["p_open"])[m][:, 0].mean() * MMHG
                 R_tot = p_in / (max(Qm[0], 1e-9) * 1e-6 / 60)
                 cal = circuit(float(max(R_tot - cal["R_eq"], 0.0)))

@@ -101,14 +101,13 @@ def request_case(spec, tau=0.55):
     """(lattice, nu, setup) of a serving request: 3.2 mm tube with a cosine narrowing of diameter reduction
     0.8 x severity; pulsatile inflow (the reference's inflow waveform, 10 harmonics) whose mean lattice speed scales
     with pressure x dt_scale and is capped for the throat."""
-    from .artery import fourier, inflow_waveform
+    from .artery import INFLOW_HARMONICS
     sev = 0.8 * spec["severity"]
     lat, _, _, D = stenosis_pipe(spec["dx"], sev=sev, D0_mm=3.2, Ls_mm=6.4, up_d=4, down_d=18)
     nu = (tau - 0.5) / 3.0
     area = (1.0 - sev) ** 2
     u_mean = min(0.02 * spec["map_mmhg"] / 95.0 * spec["dt_scale"], 0.05 * area)
-    _, shape = inflow_waveform(200)
-    coef = (fourier(shape) * u_mean).astype(np.float32)
+    coef = (INFLOW_HARMONICS * u_mean).astype(np.float32)
 
     def setup(s):
         s.set_drive(coef, period=spec["period"], ramp=spec["period"] // 4)

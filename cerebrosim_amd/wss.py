@@ -1,8 +1,9 @@
 """Wall shear stress on the device (counterpart of wss.py).
 
-``weights`` builds, once per wall surface, the sparse least-squares coefficients of wss.py:35-50 on the host (K
-nearest lumen cells per wall vertex, Gaussian side weight, depth window, quadratic fit u_t = a d + b d^2 with a
-linear fallback below 6 cells). ``WallShear`` then evaluates tau = mu * 1e3 * sum c u_t for every frame with the
+``weights`` builds, once per wall surface, the sparse least-squares coefficients of the wall-normal velocity
+derivative on the host (the method of wss.py:35-50: K nearest lumen cells per wall vertex, Gaussian side weight,
+depth window, quadratic fit u_t = a d + b d^2 with a linear fallback below 6 cells), solved here from the fit's
+normal equations. ``WallShear`` then evaluates tau = mu * 1e3 * sum c u_t for every frame with the
 ``cs_wss`` kernel on the solver's live velocity (no field copies to the host) and accumulates the time-averaged WSS
 and the oscillatory shear index (wss.py:120-122).
 """
@@ -22,22 +23,34 @@ SIGMA = 0.15      # mm: sideways fall-off of the weights (wss.py:32)
 
 
 def weights(verts, normals, cells, depth, k=K, radius=0.8, d_min=D_MIN, d_max=D_MAX, sigma=SIGMA):
-    """Per vertex: its k nearest cells and their coefficients c, so that the wall-normal derivative of the
-    tangential velocity is a = sum_j c_j u_t,j (per mm). Returns (idx (V, k), c (V, k) float32, quadratic, any)."""
+    """Per wall vertex, the linear functional that maps the tangential velocities u_t of its k nearest lumen cells
+    (within `radius` mm) to the wall-normal derivative a of the fit u_t(d) = a d + b d^2 (no slip at d = 0): returns
+    (idx (V, k), c (V, k) float32 with a = sum_j c_j u_t,j per mm, quadratic (V,) bool, any (V,) bool).
+
+    Each cell's weight is a Gaussian of its sideways distance from the vertex (sigma) times the depth window
+    d_min < d < d_max. The weighted least-squares problem min sum_j w_j (u_j - a d_j - b d_j^2)^2 has the normal
+    equations M [a, b]^T = B^T W u, with the basis B_j = (d_j, d_j^2) and M = B^T W B; so c = e_1^T M^-1 B^T W. Where
+    M is near-singular or fewer than 6 cells carry weight (> 0.05), the one-term fit u_t = a d is used instead:
+    c = w d / sum w d^2 (wss.py:35-50)."""
     dist, idx = cKDTree(cells).query(verts, k=k, distance_upper_bound=radius)
-    ok = np.isfinite(dist)
-    idx = np.where(ok, idx, 0)
-    d = depth[idx]
-    rel = cells[idx] - verts[:, None]
-    side = rel - (rel * normals[:, None]).sum(-1, keepdims=True) * normals[:, None]
-    w = np.exp(-(np.linalg.norm(side, axis=-1) / sigma) ** 2) * ok * (d > d_min) * (d < d_max)
-    s11, s12, s22 = (w * d ** 2).sum(1), (w * d ** 3).sum(1), (w * d ** 4).sum(1)
-    det = s11 * s22 - s12 ** 2
-    quad = (det > 1e-12 * np.maximum(s11 * s22, 1e-30)) & ((w > 0.05).sum(1) >= 6)
-    c = np.where(quad[:, None], w * (s22[:, None] * d - s12[:, None] * d ** 2) / np.where(quad, det, 1)[:, None],
-                 w * d / np.maximum(s11, 1e-30)[:, None])
-    c[s11 <= 0] = 0.0
-    return idx, c.astype(np.float32), quad, s11 > 0
+    found = np.isfinite(dist)
+    idx = np.where(found, idx, 0)
+    d = depth[idx]                                                    # (V, k)
+    offset = cells[idx] - verts[:, None, :]
+    along = np.einsum("vkx,vx->vk", offset, normals)
+    lateral = np.linalg.norm(offset - along[..., None] * normals[:, None, :], axis=-1)
+    w = np.exp(-(lateral / sigma) ** 2) * (found & (d > d_min) & (d < d_max))
+    basis = np.stack([d, d * d], axis=-1)                             # (V, k, 2)
+    M = np.einsum("vk,vki,vkj->vij", w, basis, basis)                 # (V, 2, 2)
+    det = M[:, 0, 0] * M[:, 1, 1] - M[:, 0, 1] * M[:, 1, 0]
+    quadratic = (det > 1e-12 * np.maximum(M[:, 0, 0] * M[:, 1, 1], 1e-30)) & ((w > 0.05).sum(1) >= 6)
+    # first row of M^-1 (M symmetric), applied to the weighted basis
+    row = np.stack([M[:, 1, 1], -M[:, 0, 1]], axis=-1) / np.where(quadratic, det, 1.0)[:, None]
+    c_quad = w * np.einsum("vki,vi->vk", basis, row)
+    c_lin = w * d / np.maximum(M[:, 0, 0], 1e-30)[:, None]
+    c = np.where(quadratic[:, None], c_quad, c_lin)
+    c[M[:, 0, 0] <= 0] = 0.0
+    return idx, c.astype(np.float32), quadratic, M[:, 0, 0] > 0
 
 
 class WallShear:

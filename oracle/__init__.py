@@ -21,12 +21,30 @@ How each function is pinned (so that a plausible mistake fails a check that is n
   The Galilean term and ``_h3`` vanish in the periodic pipes (u = (0, 0, u_z(r))), so they are pinned separately by
   the tilted open pipe's published pressure-gradient error (geometry/solver.json:92-112, slow test) and by the
   brute-force moment checks of ``_h3``.
-* ``orc_flux_bc`` / ``orc_gather`` (simulate.py:65-90): closed forms on hand-made inputs (sums, relaxation update).
+* the hybrid blend (sigma < 1, lbm.py:224-229): the closed form of the output's second moment with a linear
+  previous-velocity field (exact finite differences, off-diagonal gradients included):
+  dr/3 I + u u^T + (1 - om) (sigma P - (1 - sigma) / (3 om) (G + G^T) - diag E)
+  (tests/test_oracle.py::test_hybrid_blend_second_moment).
+* TRT (lbm.py:241-255):
+  - the steady planar channel with half-way bounce-back and a body force, whose discrete solution is the mid-link
+    parabola plus the uniform slip (F / nu)(2 magic / 3 - 1/8) (Ginzburg 2008; exact for magic = 3/16), in FP64 to
+    1e-12 for four values of magic and two of tau (test_trt_channel_poiseuille_closed_form);
+  - the odd third moment sum c_x c_y^2 f, which relaxes with om- and carries the Guo source's odd part and the
+    third-order equilibrium term (test_trt_odd_third_moment);
+  - the second moment, whose closed form equals the regularised one with the Galilean term (test_trt_second_moment_
+    equals_regularised_form); mass, momentum and the rest state as for the regularised collision.
+* ``orc_flux_bc`` / ``orc_gather`` (simulate.py:65-90): closed forms on hand-made inputs -- per-opening slab sums,
+  the inlet pressure copy, the outlet relaxation p += alpha (R Q - pref - p), the accumulator reset; per-probe sums of
+  rho / 3 and the running maximum of |u| (test_flux_bc_closed_form, test_gather_closed_form).
 * ``oracle/sdf.py`` (vessel.py:183-207, the vessel signed distance): closed forms on a capped cylinder, a cone and
   the smooth minimum (tests/test_vessel_host.py); the lattice built from the checked HIP kernel reproduces the
   published artery counts (geometry/solver.json:179-194).
-* The TRT branch (lbm.py:241-255) is pinned only by its invariants (mass, momentum, rest state): the reference
-  publishes no TRT result. Parity of TRT against the reference is therefore "parity unpinned" beyond invariants.
+* ``oracle/wss.py`` (wss.py:35-57): the reference's own known answers, 2 mu u_max / R and 4 mu u_max / R
+  (tests/test_wss.py).
+
+Each of these checks fails on a plausible mistake planted in the C restatement: the TRT magic inverted, sigma on the
+wrong term or the blend's sign flipped, the odd Guo factor with om+ instead of om-, the sign of the Galilean term or
+of the third-order term, the sign of pref in the relaxation, the 1/3 of the probe pressure.
 """
 from __future__ import annotations
 

@@ -207,6 +207,178 @@ def test_galilean_correction_second_moment():
     assert np.abs((1 - o.om[0]) * E[interior]).max() > 1e-9
 
 
+def _linear_velocity_box(o, lat, A, rng, amp=2e-3):
+    """Random populations and a previous velocity u_b = sum_a A[b, a] x_a + const, whose central differences are
+    exact away from the periodic seam. Returns (f_in, interior mask, G) with G[a, b] = d u_b / d x_a."""
+    random_state(o, rng, amp=amp)
+    coords = lat["coords"].astype(np.float64)
+    o.vels[o.cur][:] = coords @ A.T + 0.01
+    n = lat["shape"][0]
+    interior = ((coords > 0) & (coords < n - 1)).all(1)
+    return o.f[o.cur].copy(), interior, A.T.copy()
+
+
+def _streamed_moments(f_in, lat):
+    f_s = np.take_along_axis(f_in, lat["nbr"].astype(np.int64), axis=1)
+    dr = f_s.sum(0)
+    j = C.T @ f_s
+    Pi = np.einsum("ia,ib,in->nab", C, C, f_s)
+    return f_s, dr, j, Pi
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.3, 0.75])
+def test_hybrid_blend_second_moment(sigma):
+    """Regularised collision with sigma < 1 (lbm.py:224-229), no force. The second moment of the output is
+    dr/3 I + u u^T + (1 - om) (sigma P - (1 - sigma) / (3 om) (G + G^T) - diag E), P the non-equilibrium stress of the
+    streamed populations and G the velocity-gradient tensor of the previous step, exact for a linear field. The field
+    has off-diagonal gradients, so the blend's off-diagonal terms are checked too."""
+    n = 6
+    lat = periodic_box(n)
+    o = oracle.OracleSolver(lat, 0.1, model="regularised", sigma=sigma, dtype=np.float64, threads=1)
+    rng = np.random.default_rng(23)
+    A = np.array([[0.003, -0.001, 0.002], [0.0015, -0.002, -0.0025], [-0.001, 0.0005, 0.0015]])
+    f_in, interior, G = _linear_velocity_box(o, lat, A, rng)
+    o.step_once(1.0)
+    _, dr, u, Pi = _streamed_moments(f_in, lat)
+    om = float(o.om[0])
+    P = Pi - dr[:, None, None] / 3 * np.eye(3) - np.einsum("an,bn->nab", u, u)
+    E = 3 / om * u.T ** 2 * np.diag(G)[None, :]
+    blend = sigma * P - (1 - sigma) / (3 * om) * (G + G.T)[None]
+    blend[:, [0, 1, 2], [0, 1, 2]] -= E
+    expect = dr[:, None, None] / 3 * np.eye(3) + np.einsum("an,bn->nab", u, u) + (1 - om) * blend
+    got = np.einsum("ia,ib,in->nab", C, C, o.get_f())
+    assert np.allclose(got[interior], expect[interior], rtol=0, atol=1e-14)
+    # the finite-difference part is not negligible against the tolerance
+    assert (1 - sigma) * np.abs((G + G.T) / (3 * om)).max() * abs(1 - om) > 1e-6 or sigma == 1.0
+
+
+def test_trt_second_moment_equals_regularised_form():
+    """TRT without force (lbm.py:241-255): the even part relaxes with om+, and the Galilean term psi has second
+    moment diag(D) = -(1 - om+) diag(E). So the output's second moment has the same closed form as the regularised
+    collision: dr/3 I + u u^T + (1 - om+) (P - diag E)."""
+    n = 6
+    lat = periodic_box(n)
+    o = oracle.OracleSolver(lat, 0.1, model="trt", dtype=np.float64, threads=1)
+    rng = np.random.default_rng(29)
+    A = np.diag([0.003, -0.002, 0.0015])
+    f_in, interior, G = _linear_velocity_box(o, lat, A, rng)
+    o.step_once(1.0)
+    _, dr, u, Pi = _streamed_moments(f_in, lat)
+    om = float(o.om[0])
+    P = Pi - dr[:, None, None] / 3 * np.eye(3) - np.einsum("an,bn->nab", u, u)
+    P[:, [0, 1, 2], [0, 1, 2]] -= 3 / om * u.T ** 2 * np.diag(G)[None, :]
+    expect = dr[:, None, None] / 3 * np.eye(3) + np.einsum("an,bn->nab", u, u) + (1 - om) * P
+    got = np.einsum("ia,ib,in->nab", C, C, o.get_f())
+    assert np.allclose(got[interior], expect[interior], rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("gal", [False, True])
+@pytest.mark.parametrize("magic", [3.0 / 16.0, 0.25, 1.0 / 12.0])
+def test_trt_odd_third_moment(magic, gal):
+    """TRT's odd part relaxes with om- = 1 / (1/2 + magic / (1/om+ - 1/2)) (lbm.py:150). For the odd third moment
+    M_x = sum c_x c_y^2 f: M_x(out) = (1 - om-) M_x(streamed) + om- (u_x / 3 + t3 u_x u_y^2) + (1 - om- / 2) F_x / 3.
+    The equilibrium's u_x / 3 + u_x u_y^2 is that of the D3Q27 Maxwellian with the third-order Hermite term (the same
+    moments the inlet-equilibrium test checks), which the Galilean switch turns on (t3, lbm.py:196-198); the force's
+    is F_x / 3; the even Galilean term psi has no odd moment. Pins om- and the Guo source's odd part."""
+    lat = periodic_box(5)
+    tau = 0.7
+    o = oracle.OracleSolver(lat, (tau - 0.5) / 3, magic=magic, model="trt", galilean=gal, dtype=np.float64,
+                            threads=1)
+    rng = np.random.default_rng(31)
+    random_state(o, rng, amp=3e-3)
+    F = np.array([2e-4, -1e-4, 3e-4])
+    o.set_force(F)
+    Fr = F.astype(np.float32).astype(np.float64)
+    f_in = o.f[o.cur].copy()
+    o.step_once(1.0)
+    f_s, dr, j, _ = _streamed_moments(f_in, lat)
+    u = j + Fr[:, None] / 2
+    om_m = 1.0 / (0.5 + magic / (tau - 0.5))
+    f_out = o.get_f()
+    for a, b in ((0, 1), (1, 2), (2, 0), (0, 2)):
+        w = C[:, a] * C[:, b] ** 2
+        eq = u[a] / 3 + (u[a] * u[b] ** 2 if gal else 0.0)
+        expect = (1 - om_m) * (w @ f_s) + om_m * eq + (1 - om_m / 2) * Fr[a] / 3
+        assert np.allclose(w @ f_out, expect, rtol=0, atol=1e-15), (a, b)
+
+
+def _channel(nx, n=4):
+    """Planar channel: nx fluid layers across x (walls at the mid-links x = -1/2 and nx - 1/2), periodic in y, z."""
+    phi = -np.ones((nx, n, n), np.float32)
+    return L.build_lattice(phi, [0, 0, 0], 1.0, periodic=(1, 2))
+
+
+@pytest.mark.parametrize("magic", [3.0 / 16.0, 0.25, 1.0 / 12.0, 0.5])
+@pytest.mark.parametrize("tau", [0.6, 1.3])
+def test_trt_channel_poiseuille_closed_form(magic, tau):
+    """Steady TRT with half-way bounce-back in a body-force-driven planar channel (lbm.py:241-255 with the wall rule
+    of lbm.py:163-173). The steady discrete solution is the parabola with the walls at the mid-links plus a uniform
+    slip (F / nu) (2 magic / 3 - 1/8) (Ginzburg, Two-relaxation-time lattice Boltzmann scheme, Commun. Comput.
+    Phys. 3 (2008): bounce-back Poiseuille flow, exact for magic = 3/16). The flow is unidirectional, so the
+    third-order equilibrium term and the Galilean correction vanish, and the solution does not depend on om+ at
+    fixed F / nu. FP64, to 1e-12 relative."""
+    nx = 8
+    lat = _channel(nx)
+    nu = (tau - 0.5) / 3
+    o = oracle.OracleSolver(lat, nu, magic=magic, bouzidi=False, model="trt", dtype=np.float64, threads=1)
+    F = 1e-5
+    o.set_force([0, 0, F])
+    Fr = float(np.float32(F))                    # set_force rounds to float32 like the reference's vec3
+    o.run(int(40 * nx * nx / (np.pi ** 2 * nu)) + 200)
+    _, u = o.fields()
+    x = lat["coords"][:, 0].astype(np.float64)
+    exact = Fr / (2 * nu) * (x + 0.5) * (nx - 0.5 - x) + Fr / nu * (2 * magic / 3 - 1 / 8)
+    assert np.abs(u[:, 2] - exact).max() < 1e-12 * np.abs(exact).max()
+    assert np.abs(u[:, :2]).max() < 1e-16
+
+
+def test_flux_bc_closed_form():
+    """k_flux + k_bc (simulate.py:65-82) on hand-made inputs: each opening's flow is the sum over its slab cells of
+    u . w; after the step p_out[0] is the drive, every other opening relaxes towards R Q - pref with rate alpha,
+    q_last holds the flows and the accumulators are zero."""
+    rng = np.random.default_rng(37)
+    N, M, n_open = 50, 40, 3
+    vel = rng.standard_normal((N, 3))
+    idx = rng.integers(0, N, M).astype(np.int32)
+    slot = rng.integers(0, n_open, M).astype(np.int32)
+    w = rng.standard_normal((M, 3))
+    R = np.array([0.0, 2.5, 7.0])
+    pref, alpha, drive = 0.125, 0.3, np.array([0.04])
+    p0 = np.array([0.5, -0.25, 0.75])
+    for dt, tol in ((np.float64, 1e-13), (np.float32, 2e-5)):
+        Qacc = np.zeros(n_open, dt)
+        p_out = p0.astype(dt)
+        q_last = np.full(n_open, np.nan, dt)
+        oracle.flux_bc(idx, w.astype(dt), slot, vel.astype(dt), Qacc, R.astype(dt), pref, drive.astype(dt), n_open,
+                       alpha, p_out, q_last, dtype=dt)
+        Qx = np.array([sum(vel[idx[t]] @ w[t] for t in range(M) if slot[t] == k) for k in range(n_open)])
+        assert np.allclose(q_last, Qx, rtol=0, atol=tol * np.abs(Qx).max() * 10)
+        assert not np.any(Qacc)
+        assert p_out[0] == dt(drive[0])
+        for k in (1, 2):
+            assert abs(p_out[k] - (p0[k] + alpha * (R[k] * Qx[k] - pref - p0[k]))) < tol * 50
+
+
+def test_gather_closed_form():
+    """k_gather (simulate.py:85-90) on hand-made inputs: per-probe sums of rho / 3, and the largest |u| over all
+    listed cells (an atomic maximum onto the value already there)."""
+    rng = np.random.default_rng(41)
+    N, M, n_probe = 60, 45, 4
+    rho = rng.standard_normal(N)
+    vel = rng.standard_normal((N, 3))
+    idx = rng.integers(0, N, M).astype(np.int32)
+    slot = rng.integers(0, n_probe, M).astype(np.int32)
+    out = np.zeros(n_probe)
+    umax = np.zeros(1)
+    oracle.gather(idx, slot, rho, vel, out, umax, dtype=np.float64)
+    expect = np.array([rho[idx[slot == k]].sum() / 3 for k in range(n_probe)])
+    assert np.allclose(out, expect, rtol=0, atol=1e-13)
+    assert umax[0] == np.linalg.norm(vel[idx], axis=1).max()
+    big = np.array([1e3])
+    oracle.gather(idx, slot, rho, vel, np.zeros(n_probe), big, dtype=np.float64)
+    assert big[0] == 1e3                           # a maximum, not an overwrite
+
+
 def test_fp32_tracks_fp64():
     lat = small_pipe()
     rng = np.random.default_rng(19)
