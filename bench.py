@@ -25,8 +25,10 @@ artery (healthy, 0.1 mm, 620,028 cells, 1000 + 20000 steps), beside its publishe
 
 Prints one JSON line. ``roofline.achieved`` = algorithmic bytes of the step kernel per launch (244 B per fluid cell:
 27 populations read + 27 written x 4 B, previous velocity read 12 B, velocity + density written 16 B) / the kernel's
-mean duration measured with HIP events on the solver's stream; ``copy_gbs`` is a 1 GiB device copy's (read + write)
-bandwidth on the same GPU, the achievable ceiling beside the 8 TB/s peak. ``cpu_baseline`` times the oracle (the C restatement
+mean duration measured with HIP events on the solver's stream; ``step_frac`` is the whole step's algorithmic rate
+(MLUPS x 244 B) over the 8 TB/s peak; ``copy_gbs`` is the library's own streaming copy kernel (cs_stream_copy, 1 GiB,
+read + write) on the same GPU, the achievable ceiling beside the peak. The timed steps are graph replays, as the
+product runs them (the graph is captured before the timed region). ``cpu_baseline`` times the oracle (the C restatement
 of the reference step, FP32, all host threads) on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -86,40 +88,78 @@ def kernel_times_ms(s, steps=40):
 
 
 def copy_bandwidth_gbs(nbytes=1 << 30, reps=10):
-    """Achievable HBM bandwidth on this device: a device-to-device copy of nbytes, (read + write) bytes over the
-    mean duration (HIP events), for comparison with the 8 TB/s peak."""
+    """Achievable HBM bandwidth on this device: the library's streaming copy kernel (cs_stream_copy, 16-byte loads
+    and stores) over nbytes, (read + write) bytes over its mean duration (HIP events on the stream it runs on)."""
+    import ctypes
+
     import torch
+    from cerebrosim_amd import _native
     x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     x.fill_(1.0)
-    y.copy_(x)
+    st = torch.cuda.current_stream()
+    h = ctypes.c_void_p(st.cuda_stream)
+    cp = lambda: _native.call("cs_stream_copy", ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                              nbytes, h)
+    cp()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
+    e0.record(st)
     for _ in range(reps):
-        y.copy_(x)
-    e1.record()
+        cp()
+    e1.record(st)
     torch.cuda.synchronize()
     secs = e0.elapsed_time(e1) * 1e-3 / reps
+    ok = bool(torch.equal(x, y))
     del x, y
     torch.cuda.empty_cache()
+    if not ok:
+        raise RuntimeError("cs_stream_copy: copy differs from its source")
     return 2.0 * nbytes / secs / 1e9
 
 
-def cpu_baseline(lat, steps):
+def timed_run(s, steps, warmup):
+    """`warmup` untimed steps, then `steps` steps timed on the wall clock as the product runs them: one captured
+    graph of the even part of `steps` (captured before the timed region; capture launches nothing) and an odd last
+    step launched eagerly. Returns seconds."""
+    import torch
+    if warmup:
+        s.run(warmup, chunk=max(2, warmup - warmup % 2))
+    chunk = steps - steps % 2
+    if chunk:
+        s.prepare(chunk)
+    s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(steps, chunk=max(2, chunk))
+    s.synchronize()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(lat, steps, steps_1thread=4):
+    """The oracle (C restatement of lbm.py k_step, FP32) on `steps` steps of the same pipe with every host thread,
+    and on `steps_1thread` steps with one thread (a serial launch loop)."""
     import oracle
     tau = 0.56
+
+    def rate(threads, n):
+        o = oracle.OracleSolver(lat, (tau - 0.5) / 3, model="regularised", threads=threads)
+        o.set_drive([0.02, 0.0, 0.01], period=4000)
+        o.set_outlet_pressure([0.0])
+        o.run(2)
+        t0 = time.perf_counter()
+        o.run(n)
+        return o.N * n / (time.perf_counter() - t0) / 1e6, o.N
+
     threads = oracle.default_threads()
-    o = oracle.OracleSolver(lat, (tau - 0.5) / 3, model="regularised", threads=threads)
-    o.set_drive([0.02, 0.0, 0.01], period=4000)
-    o.set_outlet_pressure([0.0])
-    o.run(2)
-    t0 = time.perf_counter()
-    o.run(steps)
-    dt = time.perf_counter() - t0
-    return {"value": round(o.N * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps of the same {o.N}-cell pipe (oracle: C restatement of lbm.py k_step, FP32, "
-                      f"{threads} threads; the reference's Warp CPU path cannot run here)"}
+    v, N = rate(threads, steps)
+    v1, _ = rate(1, steps_1thread)
+    return {"value": round(v, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "value_1thread": round(v1, 3),
+            "sample": f"{steps} steps ({threads} threads) and {steps_1thread} steps (1 thread) of the same {N}-cell "
+                      f"pipe (oracle: C restatement of lbm.py k_step, FP32; the reference's Warp CPU path cannot "
+                      f"run here)"}
 
 
 def form_precision(lat, steps, warmup):
@@ -127,12 +167,7 @@ def form_precision(lat, steps, warmup):
     import torch
     s = make_solver(lat, "bricks", "fp16")
     s.autotune()
-    s.run(warmup)
-    s.synchronize()
-    t0 = time.perf_counter()
-    s.run(steps)
-    s.synchronize()
-    secs = time.perf_counter() - t0
+    secs = timed_run(s, steps, warmup)
     ok = s.ok()
     kt = kernel_times_ms(s)
     name, (k_ms, k_cells) = max(kt.items(), key=lambda kv: kv[1][0])
@@ -140,7 +175,9 @@ def form_precision(lat, steps, warmup):
     variant = s.variant
     del s
     torch.cuda.empty_cache()
-    return {"storage": "fp16", "mlups": round(lat["nbr"].shape[1] * steps / secs / 1e6, 1), "ok": ok,
+    mlups = lat["nbr"].shape[1] * steps / secs / 1e6
+    return {"storage": "fp16", "mlups": round(mlups, 1), "ok": ok,
+            "step_frac": round(mlups * BYTES_PER_LUP["fp16"] / 1e3 / HBM_PEAK_GBS, 4),
             "variant": variant,
             "kernel": name, "kernel_ms": round(k_ms, 5), "achieved_gbs": round(ach, 1),
             "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_cell": BYTES_PER_LUP["fp16"],
@@ -283,20 +320,32 @@ def form_serving(full, rate=None):
             "steady_speedup_vs_sequential": round(steady / seq["cases_per_hour"], 2)}
 
 
-def load_traffic(layout, storage="fp32"):
+def load_traffic(layout, storage="fp32", variant=0):
+    """HBM bytes per launch of the dominant kernel from the committed PMC summary (FETCH_SIZE + WRITE_SIZE, scaled
+    by the copy kernel's readings), for the launch variant that was timed; None when that variant was not
+    profiled."""
     tag = layout if storage == "fp32" else f"{layout}_{storage}"
     path = os.path.join(ROOT, "profiles", f"traffic_step_{tag}_womersley.json")
-    if os.path.exists(path):
-        with open(path) as fh:
-            return json.load(fh).get("bytes_per_launch")
-    return None
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh)
+    by_variant = rec.get("bytes_per_launch_by_variant")
+    if by_variant is not None:
+        return by_variant.get(str(variant))
+    return rec.get("bytes_per_launch") if rec.get("variant", 0) == variant else None
 
 
-def pmc_probe(layout, storage):
+def pmc_probe(layout, storage, variant=0):
     """Short run for hardware-counter passes: the bench workload, 4 eager steps (2 even, 2 odd) with the interior and
-    boundary launches in sequence, and a 1 GiB device copy whose counter readings calibrate FETCH_SIZE / WRITE_SIZE."""
+    boundary launches in sequence, in launch variant `variant`, and a 1 GiB cs_stream_copy whose counter readings
+    calibrate FETCH_SIZE / WRITE_SIZE."""
+    import ctypes
+
     import torch
+    from cerebrosim_amd import _native
     s = make_solver(build_workload(), layout, storage)
+    s.variant = int(variant)
     # the step's launches one after the other on the solver stream (in a run they overlap on two streams, and a
     # counter read around one dispatch would then include the other's traffic)
     for _ in range(4):
@@ -310,7 +359,8 @@ def pmc_probe(layout, storage):
     y = torch.empty_like(x)
     x.fill_(1.0)
     torch.cuda.synchronize()
-    y.copy_(x)
+    _native.call("cs_stream_copy", ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), 1 << 30,
+                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
     print("pmc probe done", flush=True)
 
@@ -348,7 +398,8 @@ def main():
     ap.add_argument("--forms", default="precision,batching,serving",
                     help="comma-separated chosen forms to measure after the headline ('none' to skip)")
     ap.add_argument("--full", action="store_true", help="full-size batching / serving workloads")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch variant")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the launch variant given by --variant")
+    ap.add_argument("--variant", type=int, default=0, help="launch variant without autotuning (and of --pmc-probe)")
     ap.add_argument("--no-artery", action="store_true", help="skip the story-artery speed comparison")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -356,7 +407,7 @@ def main():
               f"runs on one device; independent cases shard across GPUs without exchange)", file=sys.stderr)
         sys.exit(2)
     if args.pmc_probe:
-        pmc_probe(args.layout, args.storage)
+        pmc_probe(args.layout, args.storage, args.variant)
         return
     import torch
     t0 = time.perf_counter()
@@ -364,15 +415,9 @@ def main():
     N = lat["nbr"].shape[1]
     print(f"bench: lattice {N} cells built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     s = make_solver(lat, args.layout, args.storage)
+    s.variant = args.variant
     tune = s.autotune() if not args.no_autotune else {}
-    s.run(args.warmup)
-    s.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s.run(args.steps)
-    s.synchronize()
-    torch.cuda.synchronize()
-    secs = time.perf_counter() - t0
+    secs = timed_run(s, args.steps, args.warmup)
     if not s.ok():
         raise RuntimeError("solver diverged during the benchmark")
     mlups = N * args.steps / secs / 1e6
@@ -380,7 +425,7 @@ def main():
     name, (k_ms, k_cells) = max(kt.items(), key=lambda kv: kv[1][0])   # dominant launch
     bpl = BYTES_PER_LUP[args.storage]
     achieved = bpl * k_cells / (k_ms * 1e-3) / 1e9
-    traffic = load_traffic(s.layout, args.storage)
+    traffic = load_traffic(s.layout, args.storage, s.variant)
     out = {
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(secs / args.steps * 1e3, 5), "higher_is_better": True,
@@ -390,6 +435,8 @@ def main():
                    "autotune_ms_per_step": {str(k): round(v, 4) for k, v in tune.items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "step_achieved": round(mlups * bpl / 1e3, 1),
+                     "step_frac": round(mlups * bpl / 1e3 / HBM_PEAK_GBS, 4),
                      "kernel": name, "kernel_ms": round(k_ms, 5), "kernel_cells": k_cells,
                      "bytes_per_cell": bpl,
                      "launches_ms": {k: round(v[0], 5) for k, v in kt.items()}},

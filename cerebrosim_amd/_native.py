@@ -144,6 +144,7 @@ SIGNATURES = {
     "cs_bricks_variant_info": [c_i32, c_void_p, c_void_p],
     "cs_gather": [ctypes.POINTER(GatherArgs), c_void_p],
     "cs_reduce_finite": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
+    "cs_stream_copy": [c_void_p, c_void_p, c_i64, c_void_p],
     "cs_wss": [ctypes.POINTER(WssArgs), c_void_p],
     "cs_sdf_tubes": [ctypes.POINTER(SdfTubesArgs), c_void_p],
     "cs_lattice_build": [ctypes.POINTER(GridIn), ctypes.POINTER(c_void_p)],

@@ -112,10 +112,11 @@ def flux_slabs(lat, ends, dx_mm, slab=FLUX_SLAB_MM):
         sel = np.nonzero((inward >= slab[0]) & (inward < slab[1])
                          & ((radial * radial).sum(1) < (1.3 * e["radius"]) ** 2))[0]
         wk = (n if k else -n) / thick
-        parts.append((sel, np.broadcast_to(wk, (sel.size, 3)), np.full(sel.size, k)))
-    return {"idx": np.concatenate([p[0] for p in parts]).astype(np.int32),
-            "w": np.concatenate([p[1] for p in parts]).astype(np.float32),
-            "slot": np.concatenate([p[2] for p in parts]).astype(np.int32)}
+        parts.append((sel, np.tile(wk, (sel.size, 1)), np.full(sel.size, k)))
+    # C order throughout: the arrays go to C and device code as flat buffers
+    return {"idx": np.ascontiguousarray(np.concatenate([p[0] for p in parts]), np.int32),
+            "w": np.ascontiguousarray(np.concatenate([p[1] for p in parts]), np.float32),
+            "slot": np.ascontiguousarray(np.concatenate([p[2] for p in parts]), np.int32)}
 
 
 class ArterySolver(Solver):

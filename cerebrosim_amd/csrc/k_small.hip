@@ -51,6 +51,22 @@ __global__ void k_clock_cases(cs_clock_cases_args a) {
     clock_tick(a.tick + c, a.coef + (int64_t)c * a.coef_stride, a.nh[c], a.period[c], a.ramp[c], a.drive + c);
 }
 
+// ---- bandwidth reference: 16-byte streaming copy -----------------------------------------------------------
+constexpr int kCopyGroups = 2048;
+
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    // two independent 16-byte loads in flight per thread and iteration
+    for (; i + stride < n; i += 2 * stride) {
+        const uint4 a = src[i], b = src[i + stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+    }
+    if (i < n) dst[i] = src[i];
+}
+
 // ---- block-wide sum in a fixed order (1024 threads = 16 waves of 64) ---------------------------------------
 constexpr int kRedThreads = 1024;
 
@@ -356,6 +372,19 @@ extern "C" int cs_gather(const cs_gather_args* a, void* stream) {
     CS_REQUIRE(a->m == 0 || (a->idx && a->slot && a->rho && a->vel), "cs_gather: null cell list");
     cs::k_gather<<<1, cs::kRedThreads, 0, cs::as_stream(stream)>>>(*a);
     CS_LAUNCHED("k_gather");
+    return CS_OK;
+}
+
+extern "C" int cs_stream_copy(const void* src, void* dst, int64_t bytes, void* stream) {
+    CS_REQUIRE(src && dst && bytes >= 0 && bytes % 16 == 0, "cs_stream_copy: null pointer or %lld bytes not a "
+               "multiple of 16", (long long)bytes);
+    if (bytes == 0) return CS_OK;
+    const int64_t n = bytes / 16;
+    const int64_t want = (n + 255) / 256;
+    const unsigned groups = (unsigned)(want < cs::kCopyGroups ? want : cs::kCopyGroups);
+    cs::k_stream_copy<<<groups, 256, 0, cs::as_stream(stream)>>>(reinterpret_cast<const uint4*>(src),
+                                                                 reinterpret_cast<uint4*>(dst), n);
+    CS_LAUNCHED("k_stream_copy");
     return CS_OK;
 }
 

@@ -557,22 +557,7 @@ class Solver:
         chunk += chunk % 2
         full, rest = divmod(int(steps), chunk)
         if full:
-            if chunk > self._dtab.numel():          # grow the table outside the capture
-                self._invalidate()
-                torch = _torch()
-                with torch.cuda.stream(self._tstream):
-                    self._dtab = torch.zeros(chunk, dtype=torch.float32, device=self.dev)
-            key = (self.cur, chunk)
-            if key not in self.graphs:
-                with _native.capture(self.stream_handle) as cap:
-                    self._clock_table(chunk)
-                    a = self.cur
-                    for k in range(chunk):
-                        self._k = k
-                        self._launch(a)
-                        a = 1 - a
-                self.graphs[key] = cap.graph
-            g = self.graphs[key]
+            g = self.prepare(chunk)
             for _ in range(full):
                 g.launch(self.stream_handle)
             self._k = chunk - 1
@@ -581,6 +566,29 @@ class Solver:
             self._launch(self.cur)
             self.cur = 1 - self.cur
         self.steps += int(steps)
+
+    def prepare(self, chunk):
+        """The graph of `chunk` (even) steps from the current parity, captured now if it is not cached (capture
+        launches nothing); run() replays it. Returns the graph."""
+        chunk = int(chunk)
+        if chunk <= 0 or chunk % 2:
+            raise ValueError(f"graph chunk {chunk}: must be even and positive")
+        if chunk > self._dtab.numel():          # grow the table outside the capture
+            self._invalidate()
+            torch = _torch()
+            with torch.cuda.stream(self._tstream):
+                self._dtab = torch.zeros(chunk, dtype=torch.float32, device=self.dev)
+        key = (self.cur, chunk)
+        if key not in self.graphs:
+            with _native.capture(self.stream_handle) as cap:
+                self._clock_table(chunk)
+                a = self.cur
+                for k in range(chunk):
+                    self._k = k
+                    self._launch(a)
+                    a = 1 - a
+            self.graphs[key] = cap.graph
+        return self.graphs[key]
 
     def synchronize(self):
         self._tstream.synchronize()

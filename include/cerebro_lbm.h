@@ -427,6 +427,13 @@ int cs_wss(const cs_wss_args* a, void* stream);
 int cs_reduce_finite(const float* x, int64_t n, double* work, double* result, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Bandwidth reference (no counterpart in the reference): copies `bytes` (a multiple of 16) from src to dst with
+ * 16-byte loads and stores, a grid-stride loop over 2048 workgroups of 256 threads. The benchmark times it beside
+ * the step kernels as the achievable read + write bandwidth of the device.
+ * ------------------------------------------------------------------------------------------------------------- */
+int cs_stream_copy(const void* src, void* dst, int64_t bytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Vessel signed distance -- replaces k_sdf (vessel.py:183-207) launched by sdf_grid (vessel.py:234-245): at every
  * voxel centre p = origin + (i, j, k) dx of an (n[0], n[1], n[2]) grid (k fastest), the distance to each branch's
  * chain of round cones (stations pts[br_start[b] ..][x, y, z, radius], radius interpolated along each segment),

@@ -103,6 +103,8 @@ def tables():
 
 
 def _p(a):
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError(f"oracle: array of shape {a.shape} and strides {a.strides} is not C-contiguous")
     return a.ctypes.data_as(ctypes.POINTER(np.ctypeslib.as_ctypes_type(a.dtype)))
 
 

@@ -61,6 +61,7 @@ def test_flux_slabs_select_cells_inside_each_end():
     lat, sc, rc, ax = open_pipe((0, 0, 1), 40, lambda s: np.full_like(np.asarray(s, float), 5.0))
     ends = lat["openings"]
     fl = artery.flux_slabs(lat, ends, 1.0, slab=(3.0, 5.0))
+    assert all(a.flags["C_CONTIGUOUS"] for a in fl.values())    # passed to C as flat buffers
     for k, e in enumerate(ends):
         cells = fl["idx"][fl["slot"] == k]
         depth = -((lat["coords"][cells] - e["centre"]) @ np.asarray(e["normal"]))
