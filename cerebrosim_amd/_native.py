@@ -75,6 +75,12 @@ def flux_chunks(m):
     return -(-m // (1024 * per))
 
 
+class BinArgs(ctypes.Structure):
+    _fields_ = [("start", c_void_p), ("slots", c_void_p), ("n_bins", c_i64), ("rho", c_void_p), ("rho_elem", c_i64),
+                ("vel", c_void_p), ("vel_elem", c_i64), ("vel_comp", c_i64), ("p_scale", ctypes.c_double),
+                ("p_offset", ctypes.c_double), ("u_scale", ctypes.c_double), ("p_out", c_void_p), ("u_out", c_void_p)]
+
+
 class GatherArgs(ctypes.Structure):
     _fields_ = [("idx", c_void_p), ("slot", c_void_p), ("m", c_i64), ("rho", c_void_p), ("vel", c_void_p),
                 ("vel_elem", c_i64), ("vel_comp", c_i64), ("out", c_void_p), ("umax", c_void_p),
@@ -145,6 +151,7 @@ SIGNATURES = {
     "cs_gather": [ctypes.POINTER(GatherArgs), c_void_p],
     "cs_reduce_finite": [c_void_p, c_i64, c_void_p, c_void_p, c_void_p],
     "cs_stream_copy": [c_void_p, c_void_p, c_i64, c_void_p],
+    "cs_bin_fields": [ctypes.POINTER(BinArgs), c_void_p],
     "cs_wss": [ctypes.POINTER(WssArgs), c_void_p],
     "cs_sdf_tubes": [ctypes.POINTER(SdfTubesArgs), c_void_p],
     "cs_lattice_build": [ctypes.POINTER(GridIn), ctypes.POINTER(c_void_p)],

@@ -138,7 +138,7 @@ class ArterySolver(Solver):
             self.Qlast = torch.zeros(self.n_open, dtype=torch.float32, device=dev)
             self.R = torch.from_numpy(np.asarray(R_lu, np.float32)).to(dev)
             self.f_chunks = max(1, _native.flux_chunks(int(self.f_idx.numel())))
-            self.f_work = torch.zeros(self.n_open * self.f_chunks, dtype=torch.float32, device=dev)
+            self.f_work = torch.zeros(self.n_open * self.f_chunks, dtype=torch.float64, device=dev)
         self.synchronize()
         self.pref = float(pref_lu)
         self.alpha = float(alpha)

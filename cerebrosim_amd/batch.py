@@ -248,7 +248,7 @@ class Arena:
             if fx.get("work") is None or fx["chunks"] < need:
                 fx["chunks"] = min(256, _pow2(need, lo=1))
                 fx["work"] = torch.zeros(self.batch.max_cases * self.batch.max_open * fx["chunks"],
-                                         dtype=torch.float32, device=dev)
+                                         dtype=torch.float64, device=dev)
                 grew = True
             self.batch._upload(fx["start"], start)
             self.flux = fx

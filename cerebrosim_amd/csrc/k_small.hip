@@ -52,19 +52,37 @@ __global__ void k_clock_cases(cs_clock_cases_args a) {
 }
 
 // ---- bandwidth reference: 16-byte streaming copy -----------------------------------------------------------
-constexpr int kCopyGroups = 2048;
+// One 16-byte element per thread (a grid of n / 256 workgroups, capped at 2^24; a larger copy loops). On gfx950 this
+// measured faster than grid-stride loops over fewer workgroups with several loads in flight per thread (6.2 against
+// 4.6-5.5 TB/s for 1 GiB).
+constexpr int64_t kCopyMaxGroups = (int64_t)1 << 24;
 
 __global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                      int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * 256;
-    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    // two independent 16-byte loads in flight per thread and iteration
-    for (; i + stride < n; i += 2 * stride) {
-        const uint4 a = src[i], b = src[i + stride];
-        dst[i] = a;
-        dst[i + stride] = b;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// ---- frame binning: one thread per bin, FP64 means in cell order, float16 outputs -----------------------------
+__global__ __launch_bounds__(256) void k_bin_fields(cs_bin_args a) {
+    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (b >= a.n_bins) return;
+    const int32_t s0 = a.start[b], s1 = a.start[b + 1];
+    double r = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    for (int32_t k = s0; k < s1; ++k) {
+        const int64_t s = a.slots[k];
+        r += (double)a.rho[s * a.rho_elem];
+        const float* v = a.vel + s * a.vel_elem;
+        u0 += (double)v[0];
+        u1 += (double)v[a.vel_comp];
+        u2 += (double)v[2 * a.vel_comp];
     }
-    if (i < n) dst[i] = src[i];
+    const double inv = 1.0 / (double)(s1 - s0);
+    reinterpret_cast<_Float16*>(a.p_out)[b] = (_Float16)(a.p_scale * (r * inv) + a.p_offset);
+    _Float16* u = reinterpret_cast<_Float16*>(a.u_out) + 3 * b;
+    u[0] = (_Float16)(a.u_scale * (u0 * inv));
+    u[1] = (_Float16)(a.u_scale * (u1 * inv));
+    u[2] = (_Float16)(a.u_scale * (u2 * inv));
 }
 
 // ---- block-wide sum in a fixed order (1024 threads = 16 waves of 64) ---------------------------------------
@@ -112,68 +130,73 @@ __device__ __forceinline__ int64_t flux_chunks(int64_t m) {
 }
 
 // Phase 1: chunk b of one lattice's slab cells; work[k * wc + b] = sum over the chunk's cells of opening k of
-// vel . w (per thread in cell order, then the fixed block tree).
+// vel . w (the product in FP32 from the FP32 inputs, accumulated in FP64: per thread in cell order, then the fixed
+// block tree).
 __device__ __forceinline__ void flux_partial(const int32_t* idx, const float* w, const int32_t* slot, int64_t m,
                                              const float* vel, int64_t vel_elem, int64_t vel_comp, int n_open,
-                                             float* work, int wc, int64_t b, float* lds) {
+                                             double* work, int wc, int64_t b, double* lds) {
     const int64_t per = flux_per_thread(m), chunk = (int64_t)kRedThreads * per;
     for (int k0 = 0; k0 < n_open; k0 += kFluxGroup) {
-        float acc[kFluxGroup];
+        double acc[kFluxGroup];
 #pragma unroll
-        for (int q = 0; q < kFluxGroup; ++q) acc[q] = 0.0f;
+        for (int q = 0; q < kFluxGroup; ++q) acc[q] = 0.0;
         for (int64_t j = 0; j < per; ++j) {
             const int64_t t = b * chunk + j * kRedThreads + threadIdx.x;
             if (t >= m) break;
             const int kk = slot[t] - k0;
             if (kk < 0 || kk >= kFluxGroup) continue;
             const float* v = vel + (int64_t)idx[t] * vel_elem;
-            const float val = v[0] * w[3 * t] + v[vel_comp] * w[3 * t + 1] + v[2 * vel_comp] * w[3 * t + 2];
+            const double val = (double)(v[0] * w[3 * t] + v[vel_comp] * w[3 * t + 1] + v[2 * vel_comp] * w[3 * t + 2]);
 #pragma unroll
-            for (int q = 0; q < kFluxGroup; ++q) acc[q] += (q == kk) ? val : 0.0f;
+            for (int q = 0; q < kFluxGroup; ++q) acc[q] += (q == kk) ? val : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < kFluxGroup; ++q) {
             if (k0 + q < n_open) {   // workgroup-uniform
-                const float tot = block_sum(acc[q], lds);
+                const double tot = block_sum(acc[q], lds);
                 if (threadIdx.x == 0) work[(int64_t)(k0 + q) * wc + b] = tot;
             }
         }
     }
 }
 
-// Phase 2 (one wave): Q[k] = sum of the G chunk sums (per lane in chunk order, then a fixed butterfly), then
-// p_out[0] = drive and p_out[k] += alpha (R[k] Q[k] - pref - p_out[k]) for k >= 1; q_last = Q. valid = false
-// (a work buffer too small for the chunks) writes NaN, which the health check reports.
-__device__ __forceinline__ void flux_final(const float* work, int wc, int64_t G, bool valid, int n_open,
+// Phase 2 (one wave): Q[k] = sum of the G chunk sums in FP64 (per lane in chunk order, then a fixed butterfly),
+// then p_out[0] = drive and p_out[k] += alpha (R[k] Q[k] - pref - p_out[k]) for k >= 1 (in FP64, stored as float);
+// q_last = Q (float). valid = false (a work buffer too small for the chunks) writes NaN, which the health check
+// reports.
+__device__ __forceinline__ void flux_final(const double* work, int wc, int64_t G, bool valid, int n_open,
                                            const float* R, float drive, float* p_out, float* q_last, float pref,
                                            float alpha) {
     const int lane = threadIdx.x;
     if (lane == 0) p_out[0] = drive;
     for (int k0 = 0; k0 < n_open; k0 += kFluxGroup) {
-        float s[kFluxGroup];   // the group's loads are issued together; each sum keeps its order
+        double s[kFluxGroup];   // the group's loads are issued together; each sum keeps its order
 #pragma unroll
         for (int q = 0; q < kFluxGroup; ++q) {
-            s[q] = 0.0f;
+            s[q] = 0.0;
             if (valid && k0 + q < n_open)
                 for (int64_t b = lane; b < G; b += 64) s[q] += work[(int64_t)(k0 + q) * wc + b];
         }
 #pragma unroll
         for (int q = 0; q < kFluxGroup; ++q) {
             if (k0 + q >= n_open) break;   // wave-uniform
-            float t = s[q];
+            double t = s[q];
             for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
             if (lane == 0) {
                 const int k = k0 + q;
-                const float qk = valid ? t : __builtin_nanf("");
-                q_last[k] = qk;
-                if (k > 0) p_out[k] = p_out[k] + alpha * (R[k] * qk - pref - p_out[k]);
+                const double qk = valid ? t : (double)__builtin_nanf("");
+                q_last[k] = (float)qk;
+                if (k > 0) {
+                    const double p = (double)p_out[k];
+                    p_out[k] = (float)(p + (double)alpha * ((double)R[k] * qk - (double)pref - p));
+                }
             }
         }
     }
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_flux_partial(cs_flux_args a) {
-    __shared__ float lds[kRedThreads / 64];
+    __shared__ double lds[kRedThreads / 64];
     flux_partial(a.idx, a.w, a.slot, a.m, a.vel, a.vel_elem, a.vel_comp, a.n_open, a.work, a.work_chunks,
                  blockIdx.x, lds);
 }
@@ -185,13 +208,13 @@ __global__ __launch_bounds__(64) void k_flux_final(cs_flux_args a) {
 
 // packed lattices: grid (work_chunks, n_cases), case c's chunks b < flux_chunks(m_c)
 __global__ __launch_bounds__(kRedThreads) void k_flux_partial_cases(cs_flux_cases_args a) {
-    __shared__ float lds[kRedThreads / 64];
+    __shared__ double lds[kRedThreads / 64];
     const int c = blockIdx.y;
     const int n_open = a.n_open[c];
     if (n_open <= 0) return;  // workgroup-uniform
     const int64_t s0 = a.start[c], m = a.start[c + 1] - s0;
     if ((int64_t)blockIdx.x >= flux_chunks(m)) return;
-    float* work = a.work + (int64_t)c * a.open_stride * a.work_chunks;
+    double* work = a.work + (int64_t)c * a.open_stride * a.work_chunks;
     flux_partial(a.idx + s0, a.w + 3 * s0, a.slot + s0, m, a.vel, a.vel_elem, a.vel_comp, n_open, work,
                  a.work_chunks, blockIdx.x, lds);
 }
@@ -381,10 +404,20 @@ extern "C" int cs_stream_copy(const void* src, void* dst, int64_t bytes, void* s
     if (bytes == 0) return CS_OK;
     const int64_t n = bytes / 16;
     const int64_t want = (n + 255) / 256;
-    const unsigned groups = (unsigned)(want < cs::kCopyGroups ? want : cs::kCopyGroups);
+    const unsigned groups = (unsigned)(want < cs::kCopyMaxGroups ? want : cs::kCopyMaxGroups);
     cs::k_stream_copy<<<groups, 256, 0, cs::as_stream(stream)>>>(reinterpret_cast<const uint4*>(src),
                                                                  reinterpret_cast<uint4*>(dst), n);
     CS_LAUNCHED("k_stream_copy");
+    return CS_OK;
+}
+
+extern "C" int cs_bin_fields(const cs_bin_args* a, void* stream) {
+    CS_REQUIRE(a && a->start && a->slots && a->rho && a->vel && a->p_out && a->u_out, "cs_bin_fields: null argument");
+    CS_REQUIRE(a->n_bins >= 0 && a->rho_elem >= 1 && a->vel_elem >= 1 && a->vel_comp >= 1,
+               "cs_bin_fields: n_bins %lld / strides", (long long)a->n_bins);
+    if (a->n_bins == 0) return CS_OK;
+    cs::k_bin_fields<<<(unsigned)((a->n_bins + 255) / 256), 256, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_bin_fields");
     return CS_OK;
 }
 

@@ -224,6 +224,8 @@ def artery_y(dx_mm=0.1, R_parent=1.6, R_a=1.25, R_b=1.0, L_parent=14.0, L_branch
         ends.append({"kind": "outlet", "label": label, "centre": A + ax * L, "normal": ax, "radius": R})
     lat = build_lattice(phi, origin, dx_mm, openings=ends)
     lat["wall_mm"] = -phi[tuple(lat["coords"].T)]
+    from .wss import wall_surface
+    lat["wall_verts"], lat["wall_normals"] = wall_surface(phi, origin, dx_mm, ends)
     probes = [{"name": "ICA", "point": np.array([0.0, 0.0, 0.4 * L_parent]), "radius": 0.5 * R_parent},
               {"name": "junction", "point": J.copy(), "radius": 1.0},
               {"name": "MCA distal", "point": J + ax_a * 0.7 * L_branch, "radius": 0.5 * R_a},

@@ -9,9 +9,16 @@ the lumped circulation around the 3D segment, sampled every millisecond, written
 * Sampling (simulate.py:305-318): every SAMPLE_S the flows (mL/min), the opening pressures, the probe pressures
   (``cs_gather``) and the maximum speed; a progress line per beat.
 * Frames (simulate.py:345-352): FRAMES snapshots of (p, u) over the last beat, binned to 0.1 mm cells
-  (``coarse_map``, simulate.py:199-205) and stored as float16.
+  (``coarse_map``, simulate.py:199-205) and stored as float16; the binning runs on the device (``cs_bin_fields``)
+  into frame buffers that are copied to the host once, at the end.
+* Wall shear stress (wss.py:112-136): when the lattice carries wall points ("wall_verts", "wall_normals": the
+  zero crossings of its signed distance, ``wss.wall_surface``), the fit weights are built once and ``cs_wss``
+  evaluates tau at every stored frame on the solver's live velocity, accumulating TAWSS and OSI on the device;
+  ``wss.npz`` gets the reference's keys (verts, tau, tawss, osi, wall, t) except the mesh faces, which belong to
+  the renderer.
 * Outputs (simulate.py:356-371): ``series.npz`` (t, Q, p_open, p_probe, umax), ``meta.json`` and ``fields.npz``
-  (coords int16, origin, dx_mm, wall_mm, t, p_mmhg, u_m_s), the files wss.py and the explorer read.
+  (coords int16, origin, dx_mm, wall_mm, t, p_mmhg, u_m_s), the files wss.py and the explorer read, and
+  ``wss.npz``.
 
 The geometry comes from the caller (a lattice with ``wall_mm``, its open ends and probes): ``vessel.artery_case``
 builds the story artery as the reference's setup does (simulate.py:155-200); ``geometry.artery_y`` is a smaller
@@ -20,13 +27,15 @@ synthetic Y artery.
 """
 from __future__ import annotations
 
+import ctypes
 import json
 import os
 import time
 
 import numpy as np
 
-from .artery import (HEART_PERIOD, MMHG, OUTLET_RELAX_S, P_OUT_MMHG, RHO, SAMPLE_S, SPONGE_MM, SPONGE_TAU,
+from . import _native
+from .artery import (HEART_PERIOD, MMHG, MU, OUTLET_RELAX_S, P_OUT_MMHG, RHO, SAMPLE_S, SPONGE_MM, SPONGE_TAU,
                      ArterySolver, circuit, flux_slabs, outlet_resistances, units)
 
 BEATS = 3                 # simulate.py:45
@@ -43,6 +52,59 @@ def coarse_map(lat, dx_mm):
     return uk.astype(np.int16), inv.ravel(), cnt, f
 
 
+class FrameBinner:
+    """Device binning of the newest fields into float16 frame buffers (FRAMES x bins), one launch per frame."""
+
+    def __init__(self, s, cmap, n_frames, p_scale, p_offset, u_scale):
+        import torch
+        uk, inv, cnt, _ = cmap
+        order = np.argsort(inv, kind="stable")
+        start = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+        self.s = s
+        with torch.cuda.stream(s._tstream):
+            self.start = torch.from_numpy(start).to(s.dev)
+            self.slots = torch.from_numpy(np.ascontiguousarray(s.backend.slots(order), np.int32)).to(s.dev)
+            self.p = torch.zeros((n_frames, len(uk)), dtype=torch.float16, device=s.dev)
+            self.u = torch.zeros((n_frames, len(uk), 3), dtype=torch.float16, device=s.dev)
+        s.synchronize()
+        self.n_bins = len(uk)
+        self.scales = (float(p_scale), float(p_offset), float(u_scale))
+
+    def frame(self, k):
+        s = self.s
+        A = _native.BinArgs()
+        A.start, A.slots, A.n_bins = self.start.data_ptr(), self.slots.data_ptr(), self.n_bins
+        A.rho, A.rho_elem = s.backend.density(s.cur).data_ptr(), s.backend.density_stride()
+        A.vel = s.backend.velocity(s.cur).data_ptr()
+        A.vel_elem, A.vel_comp = s.backend.velocity_addressing()
+        A.p_scale, A.p_offset, A.u_scale = self.scales
+        A.p_out, A.u_out = self.p[k].data_ptr(), self.u[k].data_ptr()
+        _native.call("cs_bin_fields", ctypes.byref(A), s.stream_handle)
+
+    def host(self, n):
+        self.s.synchronize()
+        return self.p[:n].cpu().numpy(), self.u[:n].cpu().numpy()
+
+
+def wall_shear_setup(s, lat, dx_mm, u_scale, n_frames):
+    """Fit weights of the lattice's wall points (``wss.weights`` over the lattice cells, depth = wall distance) and
+    the device evaluator; None without wall points."""
+    import torch
+
+    from . import wss
+    if "wall_verts" not in lat or not len(lat["wall_verts"]):
+        return None
+    cells = lat["origin"] + lat["coords"] * dx_mm
+    idx, c, quad, any_ = wss.weights(lat["wall_verts"], lat["wall_normals"], cells, lat["wall_mm"])
+    ws = wss.WallShear(s, idx, c, lat["wall_normals"], u_scale=u_scale, mu=MU)
+    with torch.cuda.stream(s._tstream):
+        ws.store = torch.zeros((n_frames, len(idx), 3), dtype=torch.float16, device=s.dev)
+    s.synchronize()
+    ws.any = any_
+    ws.quad = quad
+    return ws
+
+
 def probe_cells(lat, dx_mm, probes):
     """Cells within each probe's radius (simulate.py:193-199): ({"idx", "slot"}, probes with their cell counts)."""
     P = lat["origin"] + lat["coords"] * dx_mm
@@ -56,12 +118,14 @@ def probe_cells(lat, dx_mm, probes):
 
 
 def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, flow=1.0, heart_period=HEART_PERIOD,
-        log=print, progress_every=None, autotune=False, **solver_kw):
-    """Simulate `beats` heartbeats and write series.npz, meta.json and fields.npz into out_dir. cal: the circuit
+        log=print, progress_every=None, autotune=False, wall_shear=True, **solver_kw):
+    """Simulate `beats` heartbeats and write series.npz, meta.json, fields.npz (and wss.npz when the lattice has wall
+    points and wall_shear is set) into out_dir. cal: the circuit
     (``artery.circuit``) to drive with; None calibrates on this run, as the reference's healthy case does. flow
     scales the absolute inlet pressure. progress_every: also log the flows every that many samples. autotune: time
     the launch variants on this lattice first and keep the fastest (the results do not depend on it). Returns
     (series, meta)."""
+    import torch
     os.makedirs(out_dir, exist_ok=True)
     un = units(dx, dt)
     dx_mm = dx * 1e3
@@ -97,8 +161,10 @@ def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, 
     gather = probe_cells(lat, dx_mm, probes)
     pcount = np.bincount(gather["slot"], minlength=len(probes))
     cmap = coarse_map(lat, dx_mm)
+    binner = FrameBinner(s, cmap, FRAMES, RHO * un.c ** 2 / (3.0 * MMHG), P_OUT_MMHG, un.c)
+    ws = wall_shear_setup(s, lat, dx_mm, un.c, FRAMES) if wall_shear else None
     series = {"t": [], "Q": [], "p_open": [], "p_probe": [], "umax": []}
-    frames_t, frames_p, frames_u = [], [], []
+    frames_t = []
     T0 = time.perf_counter()
     total = beats * spb
     while s.steps < total:
@@ -132,13 +198,13 @@ def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, 
                 cal = circuit(float(max(R_tot - cal["R_eq"], 0.0)))
                 drive(cal)
         if s.steps > (beats - 1) * spb and (s.steps - (beats - 1) * spb) % frame_every == 0:
-            p, u = s.fields()
-            uk, inv, cnt, _ = cmap
-            pc = np.bincount(inv, weights=p, minlength=len(uk)) / cnt
-            uc = np.stack([np.bincount(inv, weights=u[:, k], minlength=len(uk)) / cnt for k in range(3)], 1)
+            k = len(frames_t)
+            binner.frame(k)
+            if ws is not None:
+                tau = ws.frame(accumulate=True)
+                with torch.cuda.stream(s._tstream):      # ordered before the next frame's kernel rewrites tau
+                    ws.store[k].copy_(tau)
             frames_t.append((s.steps - (beats - 1) * spb) * un.dt)
-            frames_p.append((un.p(pc) / MMHG + P_OUT_MMHG).astype(np.float16))
-            frames_u.append(un.u(uc).astype(np.float16))
     secs = time.perf_counter() - T0
     ser = {k: np.array(v) for k, v in series.items()}
     np.savez_compressed(os.path.join(out_dir, "series.npz"), **ser)
@@ -153,9 +219,24 @@ def run(lat, ends, probes, dx, dt, out_dir, case="case", beats=BEATS, cal=None, 
         json.dump(meta, fh, indent=1, default=float)
     uk, inv, cnt, f = cmap
     wall = np.bincount(inv, weights=lat["wall_mm"], minlength=len(uk)) / cnt
+    frames_p, frames_u = binner.host(len(frames_t))
     np.savez_compressed(os.path.join(out_dir, "fields.npz"), coords=uk, origin=lat["origin"],
                         dx_mm=0.1 if f > 1 else lat["dx"], wall_mm=wall.astype(np.float32), t=np.array(frames_t),
-                        p_mmhg=np.array(frames_p), u_m_s=np.array(frames_u))
+                        p_mmhg=frames_p, u_m_s=frames_u)
+    if ws is not None:
+        # vertices with no lumen cell in reach carry no wall (wss.py:129-134; the open ends' faces were dropped
+        # with the points)
+        on = np.asarray(ws.any, bool)
+        tau = ws.store[:len(frames_t)].cpu().numpy()
+        tawss, osi = ws.tawss(), ws.osi()
+        tau[:, ~on], tawss[~on], osi[~on] = 0, 0, 0
+        np.savez_compressed(os.path.join(out_dir, "wss.npz"), verts=lat["wall_verts"].astype(np.float32),
+                            tau=tau.astype(np.float16), tawss=tawss.astype(np.float32),
+                            osi=osi.astype(np.float32), wall=on, t=np.array(frames_t))
+        meta["wss"] = {"vertices": int(on.sum()), "quadratic_fit_pct": round(100 * float(ws.quad[on].mean()), 1)
+                       if on.any() else 0.0, "tawss_mean_pa": float(tawss[on].mean()) if on.any() else 0.0}
+        with open(os.path.join(out_dir, "meta.json"), "w") as fh:
+            json.dump(meta, fh, indent=1, default=float)
     log(f"{case}: done in {secs:.1f} s ({int(lat['nbr'].shape[1])} cells, {s.steps} steps)")
     return ser, meta
 

@@ -276,8 +276,10 @@ def _cap_opening(b, key, kind, label=None):
 
 def artery_case(case, dx_mm=None, severity=None, segment=None):
     """simulate.py:155-200 for the story artery: the case's lumen on the healthy case's box, every end a pressure
-    opening (inlet first), the wall distance per cell (mm) and the probes. Returns (lattice, ends, probes)."""
+    opening (inlet first), the wall distance per cell (mm), the wall points with their normals (the signed distance's
+    zero crossings, "wall_verts" / "wall_normals") and the probes. Returns (lattice, ends, probes)."""
     from .lattice import build_lattice
+    from .wss import wall_surface
     if dx_mm is None:
         dx_mm = GRIDS[case][0] * 1e3
     gh = build("healthy", segment=segment)
@@ -287,6 +289,7 @@ def artery_case(case, dx_mm=None, severity=None, segment=None):
     ends = [_cap_opening(g["branches"][name], key, "outlet", label) for name, key, label in OPEN_ENDS]
     lat = build_lattice(phi, origin, dx_mm, openings=ends)
     lat["wall_mm"] = -phi[tuple(lat["coords"].T)]
+    lat["wall_verts"], lat["wall_normals"] = wall_surface(phi, origin, dx_mm, ends)
     return lat, ends, probe_points(gh, g)
 
 

@@ -53,6 +53,60 @@ def weights(verts, normals, cells, depth, k=K, radius=0.8, d_min=D_MIN, d_max=D_
     return idx, c.astype(np.float32), quadratic, M[:, 0, 0] > 0
 
 
+def wall_surface(phi, origin, dx, ends=(), clearance=0.15):
+    """Points on the lumen wall of a signed-distance grid (mm, negative inside, node (i, j, k) at origin + (i, j, k)
+    dx): where the distance changes sign along an axis edge between two nodes, at the linear-interpolation zero (the
+    vertices marching cubes places on this grid, which the reference's wall mesh has). Normals point into the lumen:
+    minus the distance's gradient, by central differences at the edge's two nodes, interpolated like the point.
+    Points on an open end's cut face (within `clearance` mm of its plane and 1.2 radii + 0.2 mm of its centre,
+    wss.py:130-133) are dropped. Returns (verts (V, 3) float64 mm, normals (V, 3) float64)."""
+    phi = np.asarray(phi, np.float32)
+    shape = np.array(phi.shape)
+    flat = phi.ravel()
+    strides = np.array([shape[1] * shape[2], shape[2], 1])
+    inside = phi < 0
+    verts, normals = [], []
+
+    def grad_at(node):
+        """(n, 3) central-difference gradient at flat node indices (one-sided at the grid's faces)."""
+        ijk = np.stack(np.unravel_index(node, phi.shape), 1)
+        g = np.empty((len(node), 3))
+        for a in range(3):
+            lo = np.maximum(ijk[:, a] - 1, 0)
+            hi = np.minimum(ijk[:, a] + 1, shape[a] - 1)
+            span = np.maximum(hi - lo, 1) * dx
+            g[:, a] = (flat[node + (hi - ijk[:, a]) * strides[a]].astype(np.float64)
+                       - flat[node - (ijk[:, a] - lo) * strides[a]]) / span
+        return g
+
+    for a in range(3):
+        sl0 = [slice(None)] * 3
+        sl1 = [slice(None)] * 3
+        sl0[a] = slice(0, shape[a] - 1)
+        sl1[a] = slice(1, shape[a])
+        cross = inside[tuple(sl0)] != inside[tuple(sl1)]
+        ijk = np.stack(np.nonzero(cross), 1)
+        if not len(ijk):
+            continue
+        n0 = ijk @ strides
+        n1 = n0 + strides[a]
+        p0, p1 = flat[n0].astype(np.float64), flat[n1].astype(np.float64)
+        t = p0 / (p0 - p1)
+        pos = ijk.astype(np.float64)
+        pos[:, a] += t
+        verts.append(np.asarray(origin, float) + pos * dx)
+        g = (1.0 - t)[:, None] * grad_at(n0) + t[:, None] * grad_at(n1)
+        normals.append(-g / np.maximum(np.linalg.norm(g, axis=1, keepdims=True), 1e-12))
+    verts = np.concatenate(verts) if verts else np.zeros((0, 3))
+    normals = np.concatenate(normals) if normals else np.zeros((0, 3))
+    keep = np.ones(len(verts), bool)
+    for e in ends:
+        off = verts - np.asarray(e["centre"], float)
+        keep &= ~((np.abs(off @ np.asarray(e["normal"], float)) < clearance)
+                  & (np.linalg.norm(off, axis=1) < 1.2 * e["radius"] + 0.2))
+    return verts[keep], normals[keep]
+
+
 class WallShear:
     """Device WSS over a fixed wall surface for one solver (or any velocity storage with the same addressing).
 

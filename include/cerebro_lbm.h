@@ -320,10 +320,11 @@ int cs_bricks_set_f(const cs_bricks_convert_args* a, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Per-step flow measurement and boundary update -- replaces k_flux + k_bc (simulate.py:65-82) launched after each
- * step at simulate.py:111-114. Q[slot] = sum over the M slab cells of vel[idx] . w (deterministic order), then
- * p_out[0] = drive[0]; for k in 1..n_open-1: p_out[k] += alpha (R[k] Q[k] - pref - p_out[k]); q_last = Q.
+ * step at simulate.py:111-114. Q[slot] = sum over the M slab cells of vel[idx] . w (deterministic order, FP64
+ * accumulation of the FP32 products), then p_out[0] = drive[0]; for k in 1..n_open-1:
+ * p_out[k] += alpha (R[k] Q[k] - pref - p_out[k]) (FP64, stored as float); q_last = Q.
  * The M cells are summed in G chunks, G a function of M alone: P = max(1, ceil(M / 262144)) cells per thread,
- * chunks of 1024 P cells, G = ceil(M / (1024 P)) <= 256. work holds n_open x work_chunks floats (chunk sums,
+ * chunks of 1024 P cells, G = ceil(M / (1024 P)) <= 256. work holds n_open x work_chunks doubles (chunk sums,
  * overwritten each call), work_chunks >= max(G, 1).
  * Cells are given as storage slots of the layout in use; component c of slot s is vel[s * vel_elem + c * vel_comp]
  * ((N, 3) AoS: vel_elem = 3, vel_comp = 1; component planes of S slots: vel_elem = 1, vel_comp = S).
@@ -343,14 +344,14 @@ typedef struct cs_flux_args {
     int32_t n_open;          /* at most 64 */
     float pref;
     float alpha;
-    float* work;             /* device, [n_open][work_chunks] */
+    double* work;            /* device, [n_open][work_chunks] */
     int32_t work_chunks;
 } cs_flux_args;
 int cs_flux_bc(const cs_flux_args* a, void* stream);
 
 /* k_flux + k_bc for packed lattices: every case c with n_open[c] > 0, over the slab cells start[c] .. start[c+1]-1
  * of the concatenated lists; per-case arrays at c * open_stride. Same chunks, arithmetic and summation order as
- * cs_flux_bc on the case alone. work holds n_cases x open_stride x work_chunks floats; work_chunks (at most 256)
+ * cs_flux_bc on the case alone. work holds n_cases x open_stride x work_chunks doubles; work_chunks (at most 256)
  * must cover every case's G, else that case's flows and pressures are written as NaN. */
 typedef struct cs_flux_cases_args {
     const int32_t* idx;
@@ -369,7 +370,7 @@ typedef struct cs_flux_cases_args {
     const float* alpha;         /* [n_cases] */
     int32_t n_cases;
     int32_t open_stride;        /* at most 64 */
-    float* work;                /* device, [n_cases][open_stride][work_chunks] */
+    double* work;               /* device, [n_cases][open_stride][work_chunks] */
     int32_t work_chunks;
 } cs_flux_cases_args;
 int cs_flux_cases(const cs_flux_cases_args* a, void* stream);
@@ -417,6 +418,30 @@ typedef struct cs_wss_args {
 int cs_wss(const cs_wss_args* a, void* stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Stored frames -- replaces the host binning of a frame in simulate.py's run loop (simulate.py:345-352, with
+ * coarse_map of simulate.py:199-205): each bin b averages its cells slots[start[b] .. start[b + 1]) (at least one)
+ * in FP64, in order, and writes float16 p_out[b] = p_scale * mean(rho) + p_offset and u_out[b][c] = u_scale *
+ * mean(u_c) (rho = the density deviation; pressure = rho / 3 in lattice units, so p_scale carries the 1/3 and the
+ * unit conversion). Component c of slot s: vel[s * vel_elem + c * vel_comp]; density: rho[s * rho_elem].
+ * ------------------------------------------------------------------------------------------------------------- */
+typedef struct cs_bin_args {
+    const int32_t* start;       /* device, [n_bins + 1] */
+    const int32_t* slots;       /* device, storage slots of the cells, bin by bin */
+    int64_t n_bins;
+    const float* rho;
+    int64_t rho_elem;
+    const float* vel;
+    int64_t vel_elem;
+    int64_t vel_comp;
+    double p_scale;
+    double p_offset;
+    double u_scale;
+    void* p_out;                /* device, float16 [n_bins] */
+    void* u_out;                /* device, float16 [n_bins][3] */
+} cs_bin_args;
+int cs_bin_fields(const cs_bin_args* a, void* stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Health check -- replaces Solver.ok() (lbm.py:454-455), which copies rho to the host and tests isfinite. A
  * deterministic two-pass device reduction over x[0..n) writes result[0] = sum of the finite values (FP64) and
  * result[1] = number of non-finite values, so ok() reads back 16 bytes instead of 4 N. `work` is caller-provided
@@ -428,8 +453,8 @@ int cs_reduce_finite(const float* x, int64_t n, double* work, double* result, vo
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Bandwidth reference (no counterpart in the reference): copies `bytes` (a multiple of 16) from src to dst with
- * 16-byte loads and stores, a grid-stride loop over 2048 workgroups of 256 threads. The benchmark times it beside
- * the step kernels as the achievable read + write bandwidth of the device.
+ * 16-byte loads and stores, one per thread (workgroups of 256 threads). The benchmark times it beside the step
+ * kernels as the achievable read + write bandwidth of the device.
  * ------------------------------------------------------------------------------------------------------------- */
 int cs_stream_copy(const void* src, void* dst, int64_t bytes, void* stream);
 

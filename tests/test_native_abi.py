@@ -52,7 +52,7 @@ STRUCTS = {"cs_clock_args": _native.ClockArgs, "cs_step_args": _native.StepArgs,
            "cs_bricks_sizes": _native.BricksSizes, "cs_bricks_step_args": _native.BricksStepArgs,
            "cs_bricks_convert_args": _native.BricksConvertArgs, "cs_clock_cases_args": _native.ClockCasesArgs,
            "cs_flux_cases_args": _native.FluxCasesArgs, "cs_clock_table_args": _native.ClockTableArgs,
-           "cs_sdf_tubes_args": _native.SdfTubesArgs,
+           "cs_sdf_tubes_args": _native.SdfTubesArgs, "cs_bin_args": _native.BinArgs,
            "cs_opening": _native.Opening, "cs_grid_in": _native.GridIn}
 
 
