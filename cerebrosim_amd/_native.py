@@ -144,6 +144,7 @@ SIGNATURES = {
     "cs_clock": [ctypes.POINTER(ClockArgs), c_void_p],
     "cs_step_indexed": [ctypes.POINTER(StepArgs), c_void_p],
     "cs_flux_bc": [ctypes.POINTER(FluxArgs), c_void_p],
+    "cs_flux_bc_static": [ctypes.POINTER(FluxArgs), c_void_p],
     "cs_clock_cases": [ctypes.POINTER(ClockCasesArgs), c_void_p],
     "cs_clock_table": [ctypes.POINTER(ClockTableArgs), c_void_p],
     "cs_flux_cases": [ctypes.POINTER(FluxCasesArgs), c_void_p],

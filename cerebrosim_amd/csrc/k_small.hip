@@ -3,6 +3,7 @@
 // (lbm.py:454-455). The reductions are deterministic: each thread sums a fixed strided subset in order, then a
 // fixed-shape tree in LDS combines the partial sums (the reference's float atomics have no defined order).
 #include <cmath>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -347,6 +348,54 @@ extern "C" int cs_flux_bc(const cs_flux_args* a, void* stream) {
         CS_LAUNCHED("k_flux_partial");
     }
     cs::k_flux_final<<<1, 64, 0, cs::as_stream(stream)>>>(*a);
+    CS_LAUNCHED("k_flux_final");
+    return CS_OK;
+}
+
+// ---- cs_flux_bc_static: the same launches with the library's own chunk-sum scratch ---------------------------
+// One array per device, for as many openings as cs_flux_bc accepts (64) x the most chunks a cell list can need (256),
+// in FP64: 128 KiB, held by each device the library runs on until the process ends. In host code `scratch` names a
+// host-side stand-in, not the device array: kernels get the device address from hipGetSymbolAddress.
+constexpr int kStaticChunks = 256;
+static __device__ __align__(256) double scratch[cs::kMaxOpen * kStaticChunks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each device, looked up once per device
+}  // namespace
+
+extern "C" int cs_flux_bc_static(const cs_flux_args* a, void* stream) {
+    CS_REQUIRE(a && a->R && a->drive && a->p_out && a->q_last, "cs_flux_bc_static: null argument");
+    CS_REQUIRE(a->n_open >= 1 && a->n_open <= cs::kMaxOpen,
+               "cs_flux_bc_static: n_open %d not in [1, %d] (the static scratch holds %d openings)", a->n_open,
+               cs::kMaxOpen, cs::kMaxOpen);
+    CS_REQUIRE(a->m >= 0 && (a->m == 0 || (a->idx && a->w && a->slot && a->vel)), "cs_flux_bc_static: null cell list");
+    int dev = 0;
+    CS_HIP(hipGetDevice(&dev));
+    CS_REQUIRE(dev >= 0 && dev < kMaxDevices, "cs_flux_bc_static: device %d beyond %d", dev, kMaxDevices);
+    double* work = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(scratch_mutex);
+        if (!scratch_ptr[dev]) {
+            void* ptr = nullptr;
+            CS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            scratch_ptr[dev] = static_cast<double*>(ptr);
+        }
+        work = scratch_ptr[dev];
+    }
+    cs_flux_args b = *a;
+    b.work = work;
+    b.work_chunks = kStaticChunks;
+    const int64_t per = (b.m + (int64_t)cs::kRedThreads * 256 - 1) / ((int64_t)cs::kRedThreads * 256);
+    const int64_t chunk = (int64_t)cs::kRedThreads * (per < 1 ? 1 : per);
+    const int64_t G = (b.m + chunk - 1) / chunk;
+    CS_REQUIRE(G <= kStaticChunks, "cs_flux_bc_static: %lld chunks exceed %d", (long long)G, kStaticChunks);
+    if (G > 0) {
+        cs::k_flux_partial<<<(unsigned)G, cs::kRedThreads, 0, cs::as_stream(stream)>>>(b);
+        CS_LAUNCHED("k_flux_partial");
+    }
+    cs::k_flux_final<<<1, 64, 0, cs::as_stream(stream)>>>(b);
     CS_LAUNCHED("k_flux_final");
     return CS_OK;
 }

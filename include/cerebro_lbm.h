@@ -348,6 +348,13 @@ typedef struct cs_flux_args {
     int32_t work_chunks;
 } cs_flux_args;
 int cs_flux_bc(const cs_flux_args* a, void* stream);
+/* No counterpart in the reference: cs_flux_bc for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It ignores a->work / a->work_chunks and uses a
+ * library-owned static device array of 64 x 256 doubles (128 KiB per device, held until the process ends): up to 64
+ * openings, any number of slab cells. It calls no allocating routine; n_open above 64 is rejected. All calls on one
+ * device share that array: calls whose launches overlap on one device (different streams, no ordering between
+ * them) race. Runs on the current device; `stream` must belong to it. Results equal cs_flux_bc's bit for bit. */
+int cs_flux_bc_static(const cs_flux_args* a, void* stream);
 
 /* k_flux + k_bc for packed lattices: every case c with n_open[c] > 0, over the slab cells start[c] .. start[c+1]-1
  * of the concatenated lists; per-case arrays at c * open_stride. Same chunks, arithmetic and summation order as
