@@ -118,6 +118,65 @@ def copy_bandwidth_gbs(nbytes=1 << 30, reps=10):
     return 2.0 * nbytes / secs / 1e9
 
 
+def oracle_checks():
+    """The CPU-baseline leg's checks, with the oracle as the checker (FP32 and FP64 restatements of the step):
+
+    * the steady Poiseuille box (24 x 24 x 64, R 10, 20,160 cells, tau 0.8, force 1e-6, 10,000 steps): the oracle
+      on all host threads (timed: the full CPU run of this workload) against the GPU's FP32 and FP16 storage;
+    * a served sample of the seed-2 request stream (4 requests, 0.2 mm grid, cycles of 60 steps): each request
+      served in FP16 storage against the FP64 oracle.
+    Returns {"poiseuille": {...}, "served_fp16": {...}}."""
+    import oracle
+    from cerebrosim_amd import lbm, serving, workloads
+    from cerebrosim_amd.geometry import poiseuille_box
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    out = {}
+    lat, r, R = poiseuille_box()
+    nu, steps = 0.1, 10000
+    refs = {}
+    for dt in (np.float32, np.float64):
+        o = oracle.OracleSolver(lat, nu, model="regularised", dtype=dt)
+        o.set_force([0, 0, 1e-6])
+        t0 = time.perf_counter()
+        o.run(steps)
+        secs = time.perf_counter() - t0
+        refs[np.dtype(dt).name] = (o.get_f(), o.fields())
+        if dt is np.float32:
+            out_cpu = {"mlups": round(o.N * steps / secs / 1e6, 2), "s": round(secs, 2), "cores": o.threads}
+    res = {"workload": "poiseuille_box_R10_24x24x64", "cells": int(lat["nbr"].shape[1]), "steps": steps,
+           "oracle_fp32_cpu": out_cpu}
+    for storage in ("fp32", "fp16"):
+        s = lbm.Solver(lat, nu, model="regularised", layout="bricks", storage=storage)
+        s.set_force([0, 0, 1e-6])
+        s.run(steps)
+        f, (p, u) = s.get_f(), s.fields()
+        f32, (p32, u32) = refs["float32"]
+        _, (p64, u64) = refs["float64"]
+        res[storage] = {"u_vs_oracle_fp64": rel(u, u64), "u_vs_oracle_fp32": rel(u, u32)}
+        if storage == "fp32":
+            res[storage]["f_vs_oracle_fp32"] = rel(f, f32)
+        del s
+    out["poiseuille"] = res
+    reqs = workloads.request_specs(4, rate_per_s=50.0, seed=2, beats=4, beat_steps=60, grids=(0.2e-3,))
+    srv = serving.Server(storage="fp16", chunk=50, arena_bricks=1 << 15, arena_records=1 << 14, keep="fields")
+    rep = srv.serve(reqs, workloads.request_case)
+    got = {x["name"]: x for x in rep["records"]}
+    errs = []
+    for q in reqs:
+        lat, nu, setup = workloads.request_case(q)
+        o = oracle.OracleSolver(lat, nu, model="regularised", dtype=np.float64)
+        setup(o)
+        o.run(got[q["name"]]["steps"])
+        errs.append(rel(got[q["name"]]["fields"][1], o.fields()[1]))
+    out["served_fp16"] = {"workload": "stenosis_requests_seed2_n4_0.2mm_beat60", "u_vs_oracle_fp64": errs,
+                          "max": max(errs)}
+    return out
+
+
 def timed_run(s, steps, warmup):
     """`warmup` untimed steps, then `steps` steps timed on the wall clock as the product runs them: one captured
     graph of the even part of `steps` (captured before the timed region; capture launches nothing) and an odd last
@@ -181,7 +240,7 @@ def form_precision(lat, steps, warmup):
             "variant": variant,
             "kernel": name, "kernel_ms": round(k_ms, 5), "achieved_gbs": round(ach, 1),
             "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_cell": BYTES_PER_LUP["fp16"],
-            "error_vs_fp64": "tests/test_gpu_fp16.py: relative L2 of u within 1e-3 of the FP64 oracle"}
+            "error_vs_fp64": None}
 
 
 def form_precision_bifurcation(full, steps=None):
@@ -445,6 +504,8 @@ def main():
     out["roofline"].update({"copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)})
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(lat, args.cpu_steps)
+        out["cpu_baseline"]["checks"] = oracle_checks()
+        print("bench: oracle checks done", file=sys.stderr, flush=True)
     forms = [f for f in args.forms.split(",") if f and f != "none"]
     if forms:
         del s
@@ -460,6 +521,15 @@ def main():
         if "serving" in forms:
             out["forms"]["serving"] = form_serving(args.full)
             print(f"bench: serving form done", file=sys.stderr, flush=True)
+    checks = out.get("cpu_baseline", {}).get("checks")
+    if checks and "forms" in out:
+        if "precision" in out["forms"]:
+            out["forms"]["precision"]["error_vs_fp64"] = {
+                "workload": checks["poiseuille"]["workload"], "steps": checks["poiseuille"]["steps"],
+                "u_rel_l2_fp16": checks["poiseuille"]["fp16"]["u_vs_oracle_fp64"],
+                "u_rel_l2_fp32": checks["poiseuille"]["fp32"]["u_vs_oracle_fp64"]}
+        if "serving" in out["forms"]:
+            out["forms"]["serving"]["fp16_u_rel_l2_vs_fp64"] = checks["served_fp16"]
     if not args.no_artery:
         torch.cuda.empty_cache()
         out["artery"] = artery_speed()

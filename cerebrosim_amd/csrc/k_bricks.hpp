@@ -179,9 +179,64 @@ constexpr RecordTable make_records() {
     return t;
 }
 
+// The same entries as runs: the 26 records in staged order, record r covering staged positions [start[r],
+// start[r + 1]) with entry = base[r] + p (base[r] = (o << 11) + block offset of the record's first slot - start[r]).
+// The odd step computes its record addresses from these compile-time runs instead of loading a table.
+struct RecordRuns {
+    int start[26];
+    int base[26];
+};
+
+constexpr RecordRuns make_runs() {
+    RecordRuns t{};
+    int p = 0, r = 0;
+    for (int oi = 0; oi < 27; ++oi) {
+        const int h = stage_at(oi);
+        if (h == 13) continue;
+        const int g = 26 - h;
+        t.start[r] = p;
+        t.base[r] = (h << 11) + group_start(g) - p;
+        p += group_size(g);
+        ++r;
+    }
+    return t;
+}
+
+constexpr RecordRuns kRuns = make_runs();
+
+constexpr bool runs_match_records() {
+    constexpr RecordRuns t = make_runs();
+    constexpr RecordTable rt = make_records();
+    for (int p = 0; p < kRecs; ++p) {
+        int r = 0;
+        while (r + 1 < 26 && t.start[r + 1] <= p) ++r;
+        if ((uint32_t)(t.base[r] + p) != rt.e[p]) return false;
+    }
+    return true;
+}
+
 static_assert(group_start(13) == 0 && group_size(13) == kOwn, "own group first");
 static_assert(stage_start(stage_at(1)) == kOwn, "records after the own group");
 __constant__ RecordTable kRecords = make_records();
+static_assert(runs_match_records(), "record runs give the record table");
+
+// Entry (o << 11) | block offset of staged record position p in [LO, HI] (the positions one load instruction
+// covers): the last record starting at or before p (a select chain over the records the range touches).
+template <int LO, int HI>
+__device__ __forceinline__ uint32_t record_entry(int p) {
+    constexpr int lo = LO, hi = HI;
+    uint32_t e = 0u;
+    static_for<0, 26>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        constexpr int s = kRuns.start[r];
+        constexpr int end = r + 1 < 26 ? kRuns.start[r + 1] : kRecs;
+        if constexpr (s <= hi && end > lo) {
+            if constexpr (s <= lo) e = (uint32_t)(kRuns.base[r] + p);
+            else if (p >= s) e = (uint32_t)(kRuns.base[r] + p);
+        }
+    });
+    return e;
+}
 __constant__ SlotTable kOddPos = make_oddpos();
 
 // Float16 odd step: the face and edge records as 4-byte pairs (staged positions kOwn + 2p, 2p + 1; entry (o << 11)
@@ -281,7 +336,7 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
                 constexpr int j = decltype(jc)::value;
                 const int p = j * BC + L;
                 if ((j + 1) * BC <= kNPairs || p < kNPairs) {
-                    const uint32_t e = kPairs.e[p];
+                    const uint32_t e = record_entry<2 * j * BC, 2 * (j * BC + BC - 1)>(2 * p);
                     uint32_t* g = reinterpret_cast<uint32_t*>(F + brick_base(tbl[e >> 11]) + (e & 2047u));
                     if constexpr (STORE) *g = lp[p];
                     else if constexpr (DMA) load_to_lds<4>(g, lp + j * BC);
@@ -289,7 +344,7 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
                 }
             });
             if (L < kNCorners) {
-                const uint32_t e = kPairs.c[L];
+                const uint32_t e = record_entry<2 * kNPairs, 2 * kNPairs + kNCorners - 1>(2 * kNPairs + L);
                 const uint32_t at = brick_base(tbl[e >> 11]) + (e & 2047u);
                 if constexpr (STORE) F[at] = stage[kOwn + 2 * kNPairs + L];
                 else stage[kOwn + 2 * kNPairs + L] = F[at];
@@ -299,7 +354,7 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
                 constexpr int j = decltype(jc)::value;
                 const int p = j * BC + L;
                 if ((j + 1) * BC <= kRecs || p < kRecs) {
-                    const uint32_t e = rec[p];
+                    const uint32_t e = record_entry<j * BC, j * BC + BC - 1>(p);
                     const uint32_t at = brick_base(tbl[e >> 11]) + (e & 2047u);
                     if constexpr (STORE) F[at] = stage[kOwn + p];
                     else if constexpr (DMA) load_to_lds<4>(F + at, stage + kOwn + j * BC);

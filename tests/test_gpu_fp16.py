@@ -133,15 +133,17 @@ def test_fp16_needs_bricks():
     assert s.layout == "bricks" and s.f.dtype.itemsize == 2
 
 
-def test_precision_bifurcation_u_p_wss_vs_fp64():
-    """The precision-form workload at 0.15 scale (48,593 cells): stenosed bifurcation, pulsatile 8-harmonic inlet
-    (peak 0.08, throat speed ~0.22), tau 0.52, 800 steps. FP16 and FP32 storage against the FP64 oracle on velocity,
-    pressure and wall shear stress (device cs_wss vs the numpy restatement on the FP64 field)."""
+@pytest.mark.parametrize("scale,steps", [(0.15, 800), (0.3, 1600)])
+def test_precision_bifurcation_u_p_wss_vs_fp64(scale, steps):
+    """The precision-form workload at 0.15 scale (48,593 cells, 800 steps) and at 0.3 scale (one inlet period of
+    1600 steps): stenosed bifurcation, pulsatile 8-harmonic inlet (peak 0.08, throat speed ~0.22), tau 0.52. FP16 and
+    FP32 storage against the FP64 oracle on velocity, pressure and wall shear stress (device cs_wss vs the numpy
+    restatement on the FP64 field)."""
     from oracle.wss import shear
     from cerebrosim_amd import wss, workloads
-    lat, nu, setup, info = workloads.bifurcation_case(0.15, period=800)
+    lat, nu, setup, info = workloads.bifurcation_case(scale, period=steps)
     idx, c, normals, _ = workloads.wall_shear_weights(lat, info)
-    steps = 800
+    print(f"bifurcation scale {scale}: {lat['nbr'].shape[1]} cells, {steps} steps", flush=True)
     o = oracle.OracleSolver(lat, nu, model="regularised", dtype=np.float64)
     setup(o)
     o.run(steps)
@@ -157,6 +159,6 @@ def test_precision_bifurcation_u_p_wss_vs_fp64():
         ws = wss.WallShear(s, idx, c, normals)
         tau = ws.frame(accumulate=False).cpu().numpy()
         errs[st] = {"u": rel(u, uo), "p": rel(p, po), "wss": rel(tau, to)}
-    print(f"bifurcation vs FP64: {errs}; |u|max {np.abs(uo).max():.3f}", flush=True)
+    print(f"bifurcation scale {scale} vs FP64: {errs}; |u|max {np.abs(uo).max():.3f}", flush=True)
     assert errs["fp32"]["u"] <= 1e-5 and errs["fp32"]["p"] <= 1e-5 and errs["fp32"]["wss"] <= 1e-5
     assert errs["fp16"]["u"] <= TOL_U and errs["fp16"]["p"] <= TOL_U and errs["fp16"]["wss"] <= TOL_U
