@@ -302,6 +302,16 @@ def form_batching(full, steps=400):
           flush=True)
     out = {"workload": f"tubes_seed1_{'n96' if full else 'n32_half_size'}", "cases": len(cases), "cells": cells,
            "steps": steps, "host_build_s": round(host_s, 1)}
+    # brick-size census of the same tubes: storage slots per fluid cell and the share of cells in boundary bricks
+    # for the 4^3 bricks the kernels use and for 8^3 bricks
+    cen = {}
+    for edge in (4, 8):
+        cs_ = [bricks.census(c[0], edge) for c in cases]
+        tot_cells = sum(c["cells"] for c in cs_)
+        cen[f"{edge}^3"] = {"slots_per_cell": round(sum(c["slots"] for c in cs_) / tot_cells, 4),
+                            "boundary_cell_fraction": round(sum(c["boundary_cell_fraction"] * c["cells"]
+                                                                for c in cs_) / tot_cells, 4)}
+    out["brick_census"] = cen
     # sequential plain solves (same host layouts; set-up and stepping timed apart)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -158,3 +158,26 @@ def record_table():
 def fill_fraction(layout):
     """Fluid cells / storage slots."""
     return layout["n_cells"] / max(1, layout["n_slots"])
+
+
+def census(lat, edge=BRICK):
+    """Brick statistics of a lattice for a brick edge (4 is the layout the kernels use; 8 sizes the alternative):
+    bricks holding fluid, storage slots (edge^3 per brick), the fraction of fluid cells in boundary bricks (a brick
+    with a solid slot or a missing link, which the slower boundary kernel steps) and storage slots per fluid cell
+    (population traffic scales with slots, not with cells). Host-side numpy over the reference layout."""
+    coords = np.asarray(lat["coords"], np.int64)
+    N = len(coords)
+    bc = coords // edge
+    nb = bc.max(0) + 1
+    key = (bc[:, 0] * nb[1] + bc[:, 1]) * nb[2] + bc[:, 2]
+    ukey, inv = np.unique(key, return_inverse=True)
+    inv = inv.ravel()
+    cells = np.bincount(inv, minlength=len(ukey))
+    complete = (np.asarray(lat["nbr"]) >= 0).all(0)               # every link of the cell fluid
+    complete_cells = np.bincount(inv, weights=complete.astype(np.float64), minlength=len(ukey))
+    interior = (cells == edge ** 3) & (complete_cells == cells)
+    slots = len(ukey) * edge ** 3
+    return {"edge": int(edge), "cells": int(N), "bricks": int(len(ukey)), "interior_bricks": int(interior.sum()),
+            "slots": int(slots), "slots_per_cell": slots / max(N, 1),
+            "boundary_cell_fraction": float(cells[~interior].sum()) / max(N, 1),
+            "boundary_brick_fill": float(cells[~interior].sum()) / max(1, int((~interior).sum()) * edge ** 3)}

@@ -278,3 +278,19 @@ def test_aa_fp16_form_matches_fp64_oracle():
     assert e <= 1e-3, e
     bad = run_fp16_model(120, bounced_dir=False)
     assert bad > 10 * max(e, 1e-4), (e, bad)
+
+
+def test_census_counts_bricks_and_boundary_cells():
+    """bricks.census on a hand-checkable lattice: a fully periodic 8^3 box is all interior at edge 4 and 8; a 6^3
+    non-periodic box has every cell in a boundary brick (missing links at the faces) and 8 bricks of 4^3."""
+    from cerebrosim_amd import bricks
+    from cerebrosim_amd import lattice as L
+    box = L.build_lattice(-np.ones((8, 8, 8), np.float32), [0, 0, 0], 1.0, periodic=(0, 1, 2))
+    for edge in (4, 8):
+        c = bricks.census(box, edge)
+        assert c["bricks"] == (8 // edge) ** 3 and c["interior_bricks"] == c["bricks"]
+        assert c["slots_per_cell"] == 1.0 and c["boundary_cell_fraction"] == 0.0
+    small = L.build_lattice(-np.ones((6, 6, 6), np.float32), [0, 0, 0], 1.0)
+    c = bricks.census(small, 4)
+    assert c["bricks"] == 8 and c["interior_bricks"] == 0 and c["boundary_cell_fraction"] == 1.0
+    assert c["slots"] == 512 and abs(c["slots_per_cell"] - 512 / 216) < 1e-12
