@@ -6,7 +6,8 @@
 //
 // Arithmetic follows the numpy builder of cerebrosim_amd/lattice.py step for step: opening tests in double
 // (along >= -1e-6, lateral < radius + slack dx, inlets before outlets, first match wins); q = -phi_here /
-// (phi_solid - phi_here) in float, clipped to [1e-3, 1], 0.5 when not finite (solid node off the grid).
+// (phi_solid - phi_here) computed in FP64 from the FP32 distances (the reference's arithmetic is FP32; FP64 keeps q
+// correctly rounded), clipped to [1e-3, 1], 0.5 when not finite (solid node off the grid), stored as FP32.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -170,9 +171,11 @@ extern "C" int cs_lattice_build(const cs_grid_in* in, void** handle) {
                         ++c[2 + k];
                     } else {
                         L->nbr[(size_t)i * N + n] = CS_WALL;
-                        const float phs = on_grid ? in->phi[g] : NAN;
-                        const float qv = -ph / (phs - ph);
-                        L->q[(size_t)i * N + n] = std::isfinite(qv) ? std::min(std::max(qv, 1e-3f), 1.0f) : 0.5f;
+                        // the wall fraction in FP64 from the FP32 distances, clipped, then stored as FP32
+                        const double phs = on_grid ? (double)in->phi[g] : NAN;
+                        const double qv = -(double)ph / (phs - (double)ph);
+                        L->q[(size_t)i * N + n] =
+                            std::isfinite(qv) ? (float)std::min(std::max(qv, 1e-3), 1.0) : 0.5f;
                         ++c[0];
                     }
                 }

@@ -193,11 +193,11 @@ def build_lattice_numpy(phi, origin, dx, openings=(), periodic=(), opening_slack
             code = _classify_openings(origin + node * dx, np.full(len(miss), WALL, np.int32), inlets, outlets, dx,
                                       opening_slack)
             walls = code == WALL
-            phi_solid = phi_p[tuple(s[miss[walls]].T)]          # NaN when the solid node is off the grid
-            ph = phi_here[miss[walls]]
+            phi_solid = phi_p[tuple(s[miss[walls]].T)].astype(np.float64)   # NaN when the solid node is off the grid
+            ph = phi_here[miss[walls]].astype(np.float64)
             with np.errstate(divide="ignore", invalid="ignore"):
-                q = -ph / (phi_solid - ph)
-            qw[i, miss[walls]] = np.where(np.isfinite(q), np.clip(q, 1e-3, 1.0), 0.5)
+                q = -ph / (phi_solid - ph)                      # FP64 from the FP32 distances, stored as FP32
+            qw[i, miss[walls]] = np.where(np.isfinite(q), np.clip(q, 1e-3, 1.0), 0.5).astype(np.float32)
             src[miss] = code
             counts["wall"] += int(walls.sum())
             counts["inlet"] += int((code == INLET).sum())
