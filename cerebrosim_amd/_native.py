@@ -158,6 +158,7 @@ SIGNATURES = {
     "cs_lattice_build": [ctypes.POINTER(GridIn), ctypes.POINTER(c_void_p)],
     "cs_lattice_sizes": [c_void_p, ctypes.POINTER(c_i64), c_void_p, c_i32],
     "cs_lattice_copy": [c_void_p, ctypes.c_char_p, c_void_p],
+    "cs_lattice_view": [c_void_p, ctypes.c_char_p, ctypes.POINTER(c_void_p)],
     "cs_lattice_free": [c_void_p],
     "cs_bricks_build": [ctypes.POINTER(LatticeIn), ctypes.POINTER(c_void_p)],
     "cs_bricks_sizes_of": [c_void_p, ctypes.POINTER(BricksSizes)],

@@ -21,6 +21,8 @@ constexpr int B = 4;
 constexpr int BC = B * B * B;
 static_assert((B & (B - 1)) == 0 && BC == 64, "brick edge a power of two (bit masks), 64 cells (shift by 6)");
 constexpr uint32_t kInletBit = 1u << 27;
+constexpr uint32_t kInletLinks = 1u << 27;   // ldir[2 bb]: some cell of the brick has an inlet link
+constexpr uint32_t kOutletLinks = 1u << 28;  // ... an outlet link
 
 struct BrickHost {
     int64_t N = 0, nb = 0, nbb = 0, nbi = 0;
@@ -39,8 +41,9 @@ struct BrickHost {
 // lane reads its entry for every direction of its brick's union, also where its own link is not missing);
 // qstart / sstart are each brick's run start in qlink / in the Bouzidi side copies (links with an smask bit, same
 // order); loff is each lane's run offset inside its brick's runs (qlink offset | side offset << 16), so link i of a
-// lane is at its offset + the number of its links below i; ldir is the per-brick union of the missing links and of
-// the side links.
+// lane is at its offset + the number of its links below i; ldir is the per-brick union of the missing links (bits
+// 1..26, plus kInletLinks / kOutletLinks when some cell of the brick has an inlet / outlet link, so that bricks with
+// wall links only take the kernel's wall path) and of the side links.
 int compact_links(BrickHost& h) {
     constexpr uint32_t kLinks = (1u << Q) - 2u;  // directions 1..26
     const int64_t nbb = h.nbb;
@@ -66,13 +69,22 @@ int compact_links(BrickHost& h) {
             const uint32_t sb = h.smask[bb * BC + l] & kLinks;
             h.loff[bb * BC + l] = (uint32_t)(q - h.qstart[bb]) | ((uint32_t)(s - h.sstart[bb]) << 16);
             for (int i = 1; i < Q; ++i)
-                if ((miss >> i) & 1u) h.qlink[q++] = h.qc[(bb * Q + i) * BC + l];
+                if ((miss >> i) & 1u) {
+                    const float c = h.qc[(bb * Q + i) * BC + l];
+                    h.qlink[q++] = c;
+                    if (c < 0.0f) wq |= c > -1.5f ? kInletLinks : kOutletLinks;
+                }
             s += __builtin_popcount(sb);
             wq |= miss;
             ws |= sb;
         }
         h.ldir[2 * bb] = wq;
         h.ldir[2 * bb + 1] = ws;
+        if ((q - h.qstart[bb]) + (s - h.sstart[bb]) > CS_BRICK_LINK_RUN_MAX) {
+            set_error("cs_bricks_build: boundary brick %lld has %lld wall / open links and side copies (at most %d)",
+                      (long long)bb, (long long)((q - h.qstart[bb]) + (s - h.sstart[bb])), CS_BRICK_LINK_RUN_MAX);
+            return CS_E_LAYOUT;
+        }
     }
     CS_REQUIRE(q < (int64_t(1) << 31) && s < (int64_t(1) << 31), "cs_bricks_build: more than 2^31 boundary links");
     h.qstart[nbb] = (int32_t)q;

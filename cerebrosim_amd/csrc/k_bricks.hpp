@@ -38,7 +38,13 @@ namespace {
 
 constexpr int BC = 64;
 constexpr int kLinkBatch = 8;  // boundary links evaluated per batch of loads (an even number: opposite pairs)
+// LDS floats for a boundary brick's link run and side run, each with the one entry after it that lanes may read
+// (the builder keeps the two runs within CS_BRICK_LINK_RUN_MAX); an FP16 stage holds exactly this many floats
+constexpr int kLinkLds = CS_BRICK_LINK_RUN_MAX + 2;
+static_assert(kLinkLds * 4 == 27 * 64 * 2, "the float16 even step's stage holds the link runs");
 constexpr uint32_t kInletBit = 1u << 27;
+constexpr uint32_t kInletLinks = 1u << 27;   // ldir[2 bb]: some cell of the brick has an inlet link
+constexpr uint32_t kOutletLinks = 1u << 28;  // ... an outlet link
 constexpr uint32_t kAllLinks = (1u << Q) - 1;
 
 struct SlotTable {
@@ -308,13 +314,16 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
         static_for<0, (NV + BC - 1) / BC>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
             const int k = j * BC + L;
-            if (NV % BC == 0 || (j + 1) * BC <= NV || k < NV) {
+            if constexpr (!STORE && !DMA && NV % BC != 0 && (j + 1) * BC > NV) {
+                // the partial last row goes straight to LDS, so that it is in flight with the full rows
+                if (k < NV) load_to_lds<16>(g + k, l + j * BC);
+            } else if (NV % BC == 0 || (j + 1) * BC <= NV || k < NV) {
                 if constexpr (STORE) g[k] = l[k];
                 else if constexpr (DMA) load_to_lds<16>(g + k, l + j * BC);
                 else l[k] = g[k];
             }
         });
-        if constexpr (!STORE && DMA) __builtin_amdgcn_s_waitcnt(0);
+        if constexpr (!STORE && (DMA || NV % BC != 0)) __builtin_amdgcn_s_waitcnt(0);
     } else {
         static_assert(kOwn * sizeof(T) % 16 == 0, "own group in 16-byte vectors");
         constexpr int NV = kOwn * (int)sizeof(T) / 16;
@@ -473,6 +482,15 @@ __device__ __forceinline__ float link_value(float code, float fo, bool opp_fluid
     return -fo + 2.0f * wI * (3.0f * p_out[k] + 4.5f * cu * cu - 1.5f * upsq);
 }
 
+// Bouzidi value of a wall link (the code >= 0 case of link_value) without branches: both interpolations are
+// computed and one is selected (for q < 1/2, rcp(2q) may be infinite; that branch is then discarded).
+__device__ __forceinline__ float wall_value(float q, float fo, bool opp_fluid, float f_opp, float p_self) {
+    const float t = 2.0f * q;
+    const float lo = opp_fluid ? t * fo + (1.0f - t) * f_opp : fo;
+    const float hi = (fo + (t - 1.0f) * p_self) * __builtin_amdgcn_rcpf(t);
+    return q < 0.5f ? lo : hi;
+}
+
 // One wavefront per brick, W bricks per workgroup. xcd: remap workgroup ids so that each XCD (which takes
 // workgroups round-robin) processes a contiguous run of the list.
 template <int W, int MODEL, bool GAL, bool BND, int PAR, bool H16>
@@ -498,6 +516,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     // LDS of a compute unit; FP16 moves the records as pairs (kPairs, also constant memory)
     __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
     __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
+    // odd boundary steps: each wave's copy of its brick's link and side runs (even steps use the free stage)
+    __shared__ float lbuf_all[BND && !STAGED ? WAVES * kLinkLds : 1];
     const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
     // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
     const int per = gridDim.x >> 3;
@@ -538,15 +558,59 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
 
     uint32_t mask = kAllLinks;
     uint32_t sm = 0u, lo = 0u;  // boundary bricks: Bouzidi side bits and the lane's link-run offsets
-    int bb = 0;
+    uint32_t wq = 0u, ws = 0u;  // boundary bricks: the brick's link unions (ldir)
+    int bb = 0, qs = 0, ss = 0;  // boundary bricks: record, starts of its link and side runs
+    bool own_om = false;         // the brick has per-cell omegas
     if constexpr (BND) {
         bb = a.brec ? a.brec[li] : (int)li;  // the entry's boundary record: bidx[brick] == bb
-        // the per-cell boundary words are loaded up front, so the link loads later wait for one round trip only
+        // the per-cell boundary words and the brick's run starts are loaded up front, so the link loads later wait
+        // for one round trip only
         mask = a.lmask[(uint32_t)bb * BC + L];
         sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
         lo = a.loff[(uint32_t)bb * BC + L];
+        wq = a.ldir[2 * bb];
+        ws = a.bouzidi ? a.ldir[2 * bb + 1] : 0u;
+        qs = a.qstart[bb];
+        ss = a.sstart[bb];
+        own_om = a.om_brick[b] != 0;
     }
     const bool live = mask != 0u;  // solid slots of boundary bricks only join the staging copies
+    // Boundary bricks wait on memory for most of their life: the inputs that do not depend on the populations (the
+    // cell's omega, its previous velocity and, in FP32, the neighbour velocities of the gradient stencil, read at
+    // addresses that always exist) are loaded before the streaming copy, so that their round trips overlap it.
+    // Every lane loads them (solid lanes too: all the addresses exist), so that no divergent branch makes the wave
+    // wait for them before the copy; the omega of the cell is selected where it is used.
+    float om_c = 0.0f, om_k = 0.0f, up[3] = {0.0f, 0.0f, 0.0f};
+    // FP32 + GAL: velocity ax at x - e_ax, x + e_ax. Loaded here in even steps; odd steps load them at their use
+    // (their direct gathers hold more registers: holding these too would lower the occupancy)
+    constexpr bool HOIST_GRAD = PAR == 0;
+    float gnb[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (BND) {
+        om_c = a.om_cell[slot];
+        om_k = a.case_omega ? a.case_omega[cs] : a.omega;
+        if constexpr (H16) {
+            const float4 mo = reinterpret_cast<const float4*>(a.vel_old)[slot];
+            up[0] = mo.y;
+            up[1] = mo.z;
+            up[2] = mo.w;
+        } else {
+            up[0] = a.vel_old[slot];
+            up[1] = a.vel_old[S + slot];
+            up[2] = a.vel_old[2 * S + slot];
+            if constexpr (GAL && HOIST_GRAD)
+                static_for<0, 3>([&](auto ac) {
+                    constexpr int ax = decltype(ac)::value;
+                    int d1, l1, d2, l2;
+                    shift_pos<ax == 0 ? -1 : 0, ax == 1 ? -1 : 0, ax == 2 ? -1 : 0>(x, y, z, d1, l1);
+                    shift_pos<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z, d2, l2);
+                    const float* V = a.vel_old + ax * S;
+                    // an absent neighbour brick (-1) reads brick 0 instead: a valid address whose value the link
+                    // rules discard (the link is then not fluid)
+                    gnb[2 * ax] = V[(uint32_t)max(tbl[wave][d1], 0) * BC + l1];
+                    gnb[2 * ax + 1] = V[(uint32_t)max(tbl[wave][d2], 0) * BC + l2];
+                });
+        }
+    }
 
     // ---- streaming: raw values ---------------------------------------------------------------------------
     float f[Q];
@@ -574,43 +638,65 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                 constexpr int i = decltype(ic)::value;
                 f[i] = value(ic, stage[slt[i * BC + L]]);
             });
-    } else if (live) {
-        static_for<0, Q>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            constexpr int io = Lat::opp[i];
-            if constexpr (i == 0) {
-                f[i] = value(ic, F[base + slt[L]]);
-            } else {
-                int dd, l;
-                shift_pos<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z, dd, l);
-                if (!BND || ((mask >> i) & 1u)) {
-                    f[i] = value(ic, F[brick_base(tbl[wave][dd]) + slt[io * BC + l]]);
+    } else {
+        if (live) {
+            static_for<0, Q>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                constexpr int io = Lat::opp[i];
+                if constexpr (i == 0) {
+                    f[i] = value(ic, F[base + slt[L]]);
                 } else {
-                    f[i] = value(ic, F[base + slt[i * BC + L]]);
+                    int dd, l;
+                    shift_pos<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z, dd, l);
+                    if (!BND || ((mask >> i) & 1u)) {
+                        f[i] = value(ic, F[brick_base(tbl[wave][dd]) + slt[io * BC + l]]);
+                    } else {
+                        f[i] = value(ic, F[base + slt[i * BC + L]]);
+                    }
                 }
-            }
-        });
+            });
+        }
     }
-
+    // Boundary bricks: the brick's link run (qlink, with the entry after it that lanes may read ahead) and side run
+    // are copied into LDS in one round trip (direct loads to LDS; the lengths from the last lane's offsets), so that
+    // the per-direction link reads below are LDS reads. Even steps put them in the stage, free once the populations
+    // are in registers (the wait makes sure of that). The builder keeps every brick's runs within the buffer (the
+    // lengths are clamped to it all the same, so a bad layout cannot write past it).
+    float* lb = nullptr;
+    int nq = 0;
+    if constexpr (BND) {
+        const uint32_t cnt = mask ? (uint32_t)__popc(~mask & (kAllLinks - 1u)) : 0u;
+        nq = min(__builtin_amdgcn_readlane((int)((lo & 0xFFFFu) + cnt), 63), kLinkLds - 2);
+        const int ns = min(__builtin_amdgcn_readlane((int)((lo >> 16) + (uint32_t)__popc(sm & (kAllLinks - 1u))), 63),
+                           kLinkLds - 2 - nq);
+        if constexpr (STAGED) {
+            lb = reinterpret_cast<float*>(stage);
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the stage reads above have returned
+        } else {
+            lb = lbuf_all + wave * kLinkLds;
+        }
+        for (int j = 0; j * BC < nq + 1; ++j)
+            if (j * BC + L < nq + 1) load_to_lds<4>(a.qlink + qs + j * BC + L, lb + j * BC);
+        for (int j = 0; j * BC < ns + 1; ++j)
+            if (j * BC + L < ns + 1) load_to_lds<4>(a.side + ss + j * BC + L, lb + nq + 1 + j * BC);
+        __builtin_amdgcn_s_waitcnt(0);
+        wave_sync();
+    }
     if (live) {
-        const float om = a.om_brick[b] ? a.om_cell[slot] : (a.case_omega ? a.case_omega[cs] : a.omega);
+        const float om = BND ? (own_om ? om_c : om_k)
+                             : (a.om_brick[b] ? a.om_cell[slot] : (a.case_omega ? a.case_omega[cs] : a.omega));
         const float d = a.drive[cs];
         const float* p_out = a.p_out + (int64_t)cs * a.p_out_stride;
         // ---- boundary links and the previous velocity --------------------------------------------------------
-        float up[3] = {0.0f, 0.0f, 0.0f};
         float q16[6];  // BND + GAL: qc of links 1..6 (read by the gradient stencil), loaded with the link batch
         if constexpr (BND) {
-            if constexpr (H16) {
-                const float4 mo = reinterpret_cast<const float4*>(a.vel_old)[slot];
-                up[0] = mo.y;
-                up[1] = mo.z;
-                up[2] = mo.w;
-            } else {
-                up[0] = a.vel_old[slot];
-                up[1] = a.vel_old[S + slot];
-                up[2] = a.vel_old[2 * S + slot];
-            }
             if ((mask & kAllLinks) != kAllLinks) {
+                // compact link data: each lane's links in direction order from its offset in the brick's run, so
+                // link i sits at the offset + the number of the lane's links below i. Directions that no cell of
+                // the brick has (ldir, wave-uniform) are skipped; a lane that has not this link reads a neighbouring
+                // entry (at most the end of its run: the arrays carry one element of padding) and ignores it.
+                const uint32_t miss = ~mask & (kAllLinks - 1u);
+                const uint32_t qo = lo & 0xFFFFu, so = lo >> 16;
                 const float upsq = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
                 float uind[3] = {0.0f, 0.0f, 0.0f};
                 if (mask & kInletBit) {
@@ -618,15 +704,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                     uind[1] = a.uin[((uint32_t)bb * 3 + 1) * BC + L] * d;
                     uind[2] = a.uin[((uint32_t)bb * 3 + 2) * BC + L] * d;
                 }
-                // compact link data: each lane's links in direction order from its offset in the brick's run, so
-                // link i sits at the offset + the number of the lane's links below i. Directions that no cell of
-                // the brick has (ldir, wave-uniform) are skipped; a lane that has not this link reads a neighbouring
-                // entry (at most the end of its run: the arrays carry one element of padding) and ignores it.
-                const uint32_t wq = a.ldir[2 * bb], ws = a.bouzidi ? a.ldir[2 * bb + 1] : 0u;
-                const uint32_t miss = ~mask & (kAllLinks - 1u);
-                const float* ql = a.qlink + a.qstart[bb];
-                const float* sd = a.side + a.sstart[bb];
-                const uint32_t qo = lo & 0xFFFFu, so = lo >> 16;
+                const float* ql = lb;           // the brick's link run in LDS
+                const float* sd = lb + nq + 1;  // its side run
                 // links in batches of kLinkBatch (opposite pairs): first issue the batch's loads, then evaluate
                 static_for<0, (Q - 1 + kLinkBatch - 1) / kLinkBatch>([&](auto bc) {
                     constexpr int h0 = 1 + kLinkBatch * decltype(bc)::value;
@@ -701,25 +780,26 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                     const float v2 = nbv(false);
                     p.g[ax] = (v2 - v1) * 0.5f;
                 } else {
-                    float h1 = 1.0f, v1 = up[ax], h2 = 1.0f, v2 = up[ax];
-                    if ((mask >> i1) & 1u) {
-                        v1 = nbv(true);
+                    // both neighbours are read whatever the links (the own slot stands in for a missing one: an
+                    // address that always exists), so the loads issue together; then the wall and open-link rules
+                    // select: a wall link at distance max(q, 0.1) with zero velocity, an open link the cell's own
+                    const bool fl1 = (mask >> i1) & 1u, fl2 = (mask >> i2) & 1u;
+                    float n1, n2;
+                    if constexpr (H16) {
+                        n1 = nbv(true);
+                        n2 = nbv(false);
+                    } else if constexpr (HOIST_GRAD) {
+                        n1 = gnb[2 * ax];
+                        n2 = gnb[2 * ax + 1];
                     } else {
-                        const float c = q16[i1 - 1];
-                        if (c >= 0.0f) {
-                            h1 = fmaxf(c, 0.1f);
-                            v1 = 0.0f;
-                        }
+                        n1 = V[(uint32_t)max(tbl[wave][d1], 0) * BC + l1];
+                        n2 = V[(uint32_t)max(tbl[wave][d2], 0) * BC + l2];
                     }
-                    if ((mask >> i2) & 1u) {
-                        v2 = nbv(false);
-                    } else {
-                        const float c = q16[i2 - 1];
-                        if (c >= 0.0f) {
-                            h2 = fmaxf(c, 0.1f);
-                            v2 = 0.0f;
-                        }
-                    }
+                    const float c1 = q16[i1 - 1], c2 = q16[i2 - 1];
+                    const bool w1 = !fl1 && c1 >= 0.0f, w2 = !fl2 && c2 >= 0.0f;
+                    const float h1 = w1 ? fmaxf(c1, 0.1f) : 1.0f, h2 = w2 ? fmaxf(c2, 0.1f) : 1.0f;
+                    const float v1 = fl1 ? n1 : (w1 ? 0.0f : up[ax]);
+                    const float v2 = fl2 ? n2 : (w2 ? 0.0f : up[ax]);
                     p.g[ax] = (h1 * h1 * (v2 - up[ax]) + h2 * h2 * (up[ax] - v1)) *
                               __builtin_amdgcn_rcpf(h1 * h2 * (h1 + h2));
                 }
@@ -766,8 +846,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         }
         if constexpr (BND) {
             if (sm) {
-                float* sd = a.side + a.sstart[bb];
-                const uint32_t so = lo >> 16, ws = a.ldir[2 * bb + 1];
+                float* sd = a.side + ss;
+                const uint32_t so = lo >> 16;
                 // FUSED16: f holds the stored form; P_i = w_i (f_i / scale + rho' + c.u (3 + 4.5 c.u))
                 const float rp = dr - 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
                 static_for<1, Q>([&](auto ic) {

@@ -136,26 +136,45 @@ extern "C" int cs_lattice_build(const cs_grid_in* in, void** handle) {
         std::vector<std::vector<int64_t>> cnt(threads, std::vector<int64_t>(2 + n_out, 0));
         std::vector<uint8_t> has_inlet(N, 0);
         const int64_t shp[3] = {nx, ny, nz};
+        const bool any_periodic = in->periodic[0] || in->periodic[1] || in->periodic[2];
+        int64_t goff[cs::Q];  // grid-index offset of the source node x - c_i
+        for (int i = 0; i < cs::Q; ++i) goff[i] = (cs::Lat::cx[i] * ny + cs::Lat::cy[i]) * nz + cs::Lat::cz[i];
         cs::parallel_for(N, threads, [&](int64_t lo, int64_t hi, int t) {
             std::vector<int64_t>& c = cnt[t];
             for (int64_t n = lo; n < hi; ++n) {
                 const int64_t x[3] = {L->coords[3 * n], L->coords[3 * n + 1], L->coords[3 * n + 2]};
-                const float ph = in->phi[(x[0] * ny + x[1]) * nz + x[2]];
+                const int64_t g0 = (x[0] * ny + x[1]) * nz + x[2];
+                const float ph = in->phi[g0];
+                // a node whose 27 sources all lie on the grid without wrapping: sources by a fixed offset
+                const bool inner = !any_periodic && x[0] > 0 && x[0] < nx - 1 && x[1] > 0 && x[1] < ny - 1 &&
+                                   x[2] > 0 && x[2] < nz - 1;
                 for (int i = 0; i < cs::Q; ++i) {
                     int64_t s[3];
                     bool on_grid = true;
+                    int64_t g;
                     const int ci[3] = {cs::Lat::cx[i], cs::Lat::cy[i], cs::Lat::cz[i]};
-                    for (int a = 0; a < 3; ++a) {
-                        s[a] = x[a] - ci[a];
-                        if (in->periodic[a]) s[a] = ((s[a] % shp[a]) + shp[a]) % shp[a];
-                        else if (s[a] < 0 || s[a] >= shp[a]) on_grid = false;
-                    }
-                    const int64_t g = on_grid ? (s[0] * ny + s[1]) * nz + s[2] : -1;
-                    const int32_t src = on_grid ? ids[g] : -1;
-                    if (src >= 0) {
-                        L->nbr[(size_t)i * N + n] = src;
-                        L->q[(size_t)i * N + n] = 0.5f;
-                        continue;
+                    if (inner) {
+                        g = g0 - goff[i];
+                        const int32_t src = ids[g];
+                        if (src >= 0) {
+                            L->nbr[(size_t)i * N + n] = src;
+                            L->q[(size_t)i * N + n] = 0.5f;
+                            continue;
+                        }
+                        for (int a = 0; a < 3; ++a) s[a] = x[a] - ci[a];
+                    } else {
+                        for (int a = 0; a < 3; ++a) {
+                            s[a] = x[a] - ci[a];
+                            if (in->periodic[a]) s[a] = ((s[a] % shp[a]) + shp[a]) % shp[a];
+                            else if (s[a] < 0 || s[a] >= shp[a]) on_grid = false;
+                        }
+                        g = on_grid ? (s[0] * ny + s[1]) * nz + s[2] : -1;
+                        const int32_t src = on_grid ? ids[g] : -1;
+                        if (src >= 0) {
+                            L->nbr[(size_t)i * N + n] = src;
+                            L->q[(size_t)i * N + n] = 0.5f;
+                            continue;
+                        }
                     }
                     const double p[3] = {in->origin[0] + (double)s[0] * in->dx, in->origin[1] + (double)s[1] * in->dx,
                                          in->origin[2] + (double)s[2] * in->dx};
@@ -233,6 +252,21 @@ extern "C" int cs_lattice_copy(const void* handle, const char* name, void* dst) 
     else if (s == "uin") std::memcpy(dst, L->uin.data(), L->uin.size() * 4);
     else {
         cs::set_error("cs_lattice_copy: unknown array '%s'", name);
+        return CS_E_ARG;
+    }
+    return CS_OK;
+}
+
+extern "C" int cs_lattice_view(const void* handle, const char* name, void** data) {
+    CS_REQUIRE(handle && name && data, "cs_lattice_view: null argument");
+    auto* L = static_cast<cs::LatticeHost*>(const_cast<void*>(handle));
+    const std::string s(name);
+    if (s == "coords") *data = L->coords.data();
+    else if (s == "nbr") *data = L->nbr.get();
+    else if (s == "q") *data = L->q.get();
+    else if (s == "uin") *data = L->uin.data();
+    else {
+        cs::set_error("cs_lattice_view: unknown array '%s'", name);
         return CS_E_ARG;
     }
     return CS_OK;

@@ -104,6 +104,29 @@ def builder_threads(n):
         _local.threads = old
 
 
+class _NativeLattice:
+    """Owner of a native lattice handle: the arrays of build_lattice are views of its buffers (no copy of the
+    hundreds of megabytes a large lattice holds), and the handle is freed when the last of them is released."""
+
+    def __init__(self, lib, handle):
+        self.lib, self.handle = lib, handle
+
+    def view(self, name, shape, dtype):
+        import ctypes
+        from . import _native
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        if nbytes == 0:
+            return np.empty(shape, dtype)
+        ptr = ctypes.c_void_p()
+        _native.call("cs_lattice_view", self.handle, name.encode(), ctypes.byref(ptr))
+        buf = (ctypes.c_char * nbytes).from_address(ptr.value)
+        buf._owner = self                      # the buffer keeps the handle alive; numpy keeps the buffer
+        return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def __del__(self):
+        self.lib.cs_lattice_free(self.handle)
+
+
 def build_lattice(phi, origin, dx, openings=(), periodic=(), opening_slack=1.0, native=True, threads=None):
     """Sparse fluid-cell lattice from a signed distance sampled on grid nodes (negative = fluid), lbm.py:264-330.
 
@@ -137,18 +160,14 @@ def build_lattice(phi, origin, dx, openings=(), periodic=(), opening_slack=1.0, 
     g.slack, g.threads = float(opening_slack), int(threads)
     h = ctypes.c_void_p()
     _native.call("cs_lattice_build", ctypes.byref(g), ctypes.byref(h))
-    try:
-        n = ctypes.c_int64()
-        cnt = np.zeros(2 + len(outlets), np.int64)
-        lib.cs_lattice_sizes(h, ctypes.byref(n), ctypes.c_void_p(cnt.ctypes.data), len(cnt))
-        N = int(n.value)
-        out = {"coords": np.empty((N, 3), np.int32), "nbr": np.empty((Q, N), np.int32),
-               "q": np.empty((Q, N), np.float32), "uin": np.empty((N, 3), np.float32)}
-        for name, arr in out.items():
-            if arr.size:
-                _native.call("cs_lattice_copy", h, name.encode(), ctypes.c_void_p(arr.ctypes.data))
-    finally:
-        lib.cs_lattice_free(h)
+    owner = _NativeLattice(lib, h)
+    n = ctypes.c_int64()
+    cnt = np.zeros(2 + len(outlets), np.int64)
+    lib.cs_lattice_sizes(h, ctypes.byref(n), ctypes.c_void_p(cnt.ctypes.data), len(cnt))
+    N = int(n.value)
+    shapes = {"coords": ((N, 3), np.int32), "nbr": ((Q, N), np.int32), "q": ((Q, N), np.float32),
+              "uin": ((N, 3), np.float32)}
+    out = {name: owner.view(name, shape, dt) for name, (shape, dt) in shapes.items()}
     counts = {"wall": int(cnt[0]), "inlet": int(cnt[1]), "outlet": [int(v) for v in cnt[2:]]}
     return {"nbr": out["nbr"], "q": out["q"], "uin": out["uin"], "coords": out["coords"], "origin": origin, "dx": dx,
             "shape": phi.shape, "n_outlets": len(outlets), "counts": counts, "openings": inlets + outlets}

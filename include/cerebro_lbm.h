@@ -156,6 +156,9 @@ int cs_lattice_build(const cs_grid_in* in, void** handle);
 /* N into *n_cells and up to max_counts link counts (wall, inlet, outlet 0, ...); returns the number of counts. */
 int cs_lattice_sizes(const void* handle, int64_t* n_cells, int64_t* counts, int32_t max_counts);
 int cs_lattice_copy(const void* handle, const char* name, void* dst);
+/* The handle's own buffer of an output array (same names and shapes as cs_lattice_copy), valid until
+ * cs_lattice_free: callers that keep the handle alive while they use the arrays avoid the copy. */
+int cs_lattice_view(const void* handle, const char* name, void** data);
 int cs_lattice_free(void* handle);
 
 /* ---------------------------------------------------------------------------------------------------------------
@@ -208,7 +211,11 @@ int cs_bricks_sizes_of(const void* handle, cs_bricks_sizes* out);
  *  "sstart"        int32 [n_boundary + 1]  start of each boundary brick's run of side copies (smask links, same order)
  *  "loff"          uint32 [n_boundary][64] lane's run offset in its brick: qlink offset | side offset << 16 (link i
  *                                          of a lane: offset + number of the lane's links below i)
- *  "ldir"          uint32 [n_boundary][2]  per brick: union of the missing links, union of the smask bits */
+ *  "ldir"          uint32 [n_boundary][2]  per brick: union of the missing links (bit 27 / 28 also set when a cell
+ *                                          of the brick has an inlet / outlet link), union of the smask bits
+ * The step copies a boundary brick's link run and side run into on-chip memory: the builder fails with CS_E_LAYOUT
+ * for a lattice in which some brick has more than CS_BRICK_LINK_RUN_MAX of them together. */
+#define CS_BRICK_LINK_RUN_MAX 862
 int cs_bricks_copy(const void* handle, const char* name, void* dst);
 int cs_bricks_free(void* handle);
 
