@@ -101,9 +101,7 @@ def compact_links(lay):
       (n_boundary + 1); ``n_side`` entries in all;
     * ``loff`` (n_boundary, 64) uint32: the start of each lane's run inside its brick's runs, qlink offset | side
       offset << 16; link i of a lane sits at its offset + the number of its links below i (a popcount);
-    * ``ldir`` (n_boundary, 2) uint32: the union over the brick's cells of the missing links and of the smask bits;
-      the first word also has bit 27 set when some cell of the brick has an inlet link and bit 28 when some cell has
-      an outlet link (bricks with neither take the kernels' wall-only link path).
+    * ``ldir`` (n_boundary, 2) uint32: the union over the brick's cells of the missing links and of the smask bits.
 
     ``qlink`` carries one element of padding after the last run: a lane reads its entry for every direction in
     ``ldir``, also where its own link is not missing, and such an index is at most the end of its run."""
@@ -128,10 +126,7 @@ def compact_links(lay):
     if qstart[-1] >= 2 ** 31 or sstart[-1] >= 2 ** 31:
         raise LayoutError("more than 2^31 boundary links")
     loff = (qrel | (srel << 16)).astype(np.uint32)
-    code = np.where(mT, qc, 0.0)
-    open_flags = (np.where((code < 0) & (code > -1.5), 1 << 27, 0) | np.where(code <= -1.5, 1 << 28, 0))
-    miss_flags = miss | np.bitwise_or.reduce(open_flags.astype(np.uint32), axis=2)
-    ldir = np.stack([np.bitwise_or.reduce(miss_flags, axis=1) if nb else np.zeros(0, np.uint32),
+    ldir = np.stack([np.bitwise_or.reduce(miss, axis=1) if nb else np.zeros(0, np.uint32),
                      np.bitwise_or.reduce(sbit, axis=1) if nb else np.zeros(0, np.uint32)], 1).astype(np.uint32)
     return {"qlink": qlink, "qstart": qstart.astype(np.int32), "sstart": sstart.astype(np.int32),
             "loff": np.ascontiguousarray(loff), "ldir": np.ascontiguousarray(ldir.reshape(nb, 2)),

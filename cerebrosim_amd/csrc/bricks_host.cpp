@@ -21,8 +21,6 @@ constexpr int B = 4;
 constexpr int BC = B * B * B;
 static_assert((B & (B - 1)) == 0 && BC == 64, "brick edge a power of two (bit masks), 64 cells (shift by 6)");
 constexpr uint32_t kInletBit = 1u << 27;
-constexpr uint32_t kInletLinks = 1u << 27;   // ldir[2 bb]: some cell of the brick has an inlet link
-constexpr uint32_t kOutletLinks = 1u << 28;  // ... an outlet link
 
 struct BrickHost {
     int64_t N = 0, nb = 0, nbb = 0, nbi = 0;
@@ -41,9 +39,8 @@ struct BrickHost {
 // lane reads its entry for every direction of its brick's union, also where its own link is not missing);
 // qstart / sstart are each brick's run start in qlink / in the Bouzidi side copies (links with an smask bit, same
 // order); loff is each lane's run offset inside its brick's runs (qlink offset | side offset << 16), so link i of a
-// lane is at its offset + the number of its links below i; ldir is the per-brick union of the missing links (bits
-// 1..26, plus kInletLinks / kOutletLinks when some cell of the brick has an inlet / outlet link, so that bricks with
-// wall links only take the kernel's wall path) and of the side links.
+// lane is at its offset + the number of its links below i; ldir is the per-brick union of the missing links and of
+// the side links.
 int compact_links(BrickHost& h) {
     constexpr uint32_t kLinks = (1u << Q) - 2u;  // directions 1..26
     const int64_t nbb = h.nbb;
@@ -69,11 +66,7 @@ int compact_links(BrickHost& h) {
             const uint32_t sb = h.smask[bb * BC + l] & kLinks;
             h.loff[bb * BC + l] = (uint32_t)(q - h.qstart[bb]) | ((uint32_t)(s - h.sstart[bb]) << 16);
             for (int i = 1; i < Q; ++i)
-                if ((miss >> i) & 1u) {
-                    const float c = h.qc[(bb * Q + i) * BC + l];
-                    h.qlink[q++] = c;
-                    if (c < 0.0f) wq |= c > -1.5f ? kInletLinks : kOutletLinks;
-                }
+                if ((miss >> i) & 1u) h.qlink[q++] = h.qc[(bb * Q + i) * BC + l];
             s += __builtin_popcount(sb);
             wq |= miss;
             ws |= sb;

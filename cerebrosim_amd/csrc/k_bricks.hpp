@@ -43,8 +43,6 @@ constexpr int kLinkBatch = 8;  // boundary links evaluated per batch of loads (a
 constexpr int kLinkLds = CS_BRICK_LINK_RUN_MAX + 2;
 static_assert(kLinkLds * 4 == 27 * 64 * 2, "the float16 even step's stage holds the link runs");
 constexpr uint32_t kInletBit = 1u << 27;
-constexpr uint32_t kInletLinks = 1u << 27;   // ldir[2 bb]: some cell of the brick has an inlet link
-constexpr uint32_t kOutletLinks = 1u << 28;  // ... an outlet link
 constexpr uint32_t kAllLinks = (1u << Q) - 1;
 
 struct SlotTable {
@@ -480,15 +478,6 @@ __device__ __forceinline__ float link_value(float code, float fo, bool opp_fluid
     const int k = (int)(-2.0f - code);  // outlet k: anti-bounce-back at the prescribed pressure
     const float cu = cdot<I>(up[0], up[1], up[2]);
     return -fo + 2.0f * wI * (3.0f * p_out[k] + 4.5f * cu * cu - 1.5f * upsq);
-}
-
-// Bouzidi value of a wall link (the code >= 0 case of link_value) without branches: both interpolations are
-// computed and one is selected (for q < 1/2, rcp(2q) may be infinite; that branch is then discarded).
-__device__ __forceinline__ float wall_value(float q, float fo, bool opp_fluid, float f_opp, float p_self) {
-    const float t = 2.0f * q;
-    const float lo = opp_fluid ? t * fo + (1.0f - t) * f_opp : fo;
-    const float hi = (fo + (t - 1.0f) * p_self) * __builtin_amdgcn_rcpf(t);
-    return q < 0.5f ? lo : hi;
 }
 
 // One wavefront per brick, W bricks per workgroup. xcd: remap workgroup ids so that each XCD (which takes

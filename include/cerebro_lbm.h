@@ -211,8 +211,7 @@ int cs_bricks_sizes_of(const void* handle, cs_bricks_sizes* out);
  *  "sstart"        int32 [n_boundary + 1]  start of each boundary brick's run of side copies (smask links, same order)
  *  "loff"          uint32 [n_boundary][64] lane's run offset in its brick: qlink offset | side offset << 16 (link i
  *                                          of a lane: offset + number of the lane's links below i)
- *  "ldir"          uint32 [n_boundary][2]  per brick: union of the missing links (bit 27 / 28 also set when a cell
- *                                          of the brick has an inlet / outlet link), union of the smask bits
+ *  "ldir"          uint32 [n_boundary][2]  per brick: union of the missing links, union of the smask bits
  * The step copies a boundary brick's link run and side run into on-chip memory: the builder fails with CS_E_LAYOUT
  * for a lattice in which some brick has more than CS_BRICK_LINK_RUN_MAX of them together. */
 #define CS_BRICK_LINK_RUN_MAX 862

@@ -210,8 +210,7 @@ def test_odd_step_staging_maps_to_direct_slots():
 
 def test_compact_link_arrays():
     """qlink / side order: brick by brick, lane by lane, direction 1..26; qstart / sstart the brick runs, loff the lane
-    runs, ldir the per-brick unions (with bit 27 / 28 set when a cell of the brick has an inlet / outlet link); and the
-    kernels' index (lane offset + popcount of the lane's links below i) finds
+    runs, ldir the per-brick unions; and the kernels' index (lane offset + popcount of the lane's links below i) finds
     every entry. Checked against the dense qc, lmask and smask with plain loops."""
     lat = tilted_pipe()
     lay = bricks.build(lat, dense=True)
@@ -237,8 +236,6 @@ def test_compact_link_arrays():
                     expect.append(qc[bb, i, l])
                     nq += 1
                     wq |= 1 << i
-                    if qc[bb, i, l] < 0:              # inlet (-1) or outlet (-2 - k) link: the brick's open flags
-                        wq |= 1 << 27 if qc[bb, i, l] > -1.5 else 1 << 28
                 if (sm >> i) & 1:
                     assert (off >> 16) + ns == len(sides) - sstarts[-1]
                     sides.append((bb, i, l))
