@@ -6,6 +6,7 @@ if the library is missing or fails to load, every solver entry point raises.
 from __future__ import annotations
 
 import ctypes
+import gc
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -233,16 +234,29 @@ class capture:
         self.graph = None
 
     def __enter__(self):
-        call("cs_graph_begin", self.stream)
+        # no cyclic garbage collection while capturing: finalizers of unreachable objects (graphs, events, device
+        # memory of dropped solvers and batches) would make runtime calls that a stream capture does not allow
+        self._gc = gc.isenabled()
+        gc.disable()
+        try:
+            call("cs_graph_begin", self.stream)
+        except BaseException:
+            if self._gc:
+                gc.enable()
+            raise
         return self
 
     def __exit__(self, exc_type, exc, tb):
-        h = c_void_p()
-        if exc_type is not None:
-            load().cs_graph_end(self.stream, ctypes.byref(h))   # end the capture, discard the graph
-            if h.value:
-                load().cs_graph_destroy(h)
+        try:
+            h = c_void_p()
+            if exc_type is not None:
+                load().cs_graph_end(self.stream, ctypes.byref(h))   # end the capture, discard the graph
+                if h.value:
+                    load().cs_graph_destroy(h)
+                return False
+            call("cs_graph_end", self.stream, ctypes.byref(h))
+            self.graph = Graph(h.value)
             return False
-        call("cs_graph_end", self.stream, ctypes.byref(h))
-        self.graph = Graph(h.value)
-        return False
+        finally:
+            if self._gc:
+                gc.enable()

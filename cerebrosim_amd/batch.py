@@ -24,6 +24,7 @@ resistance-outlet flow measurement of ``artery.ArterySolver`` (``set_outlets``, 
 """
 from __future__ import annotations
 
+import collections
 import ctypes
 import time
 
@@ -269,11 +270,13 @@ class Batch:
     storage "fp32" / "fp16"; model, magic, bouzidi, galilean as in ``lbm.Solver`` (shared by all cases). Arenas
     are created on demand with ``arena_bricks`` / ``arena_records`` capacity while the total stays within
     ``budget_bytes`` (default: the device's free memory less 2 GiB). ``staging_bytes``: the pinned host ring that
-    uploads go through; the host waits only when it wraps onto copies still queued behind the steps in flight."""
+    uploads go through; the host waits only when it wraps onto copies still queued behind the steps in flight.
+    ``result_bytes``: the pinned host pool that ``Case.read_async`` copies results into (a read that does not fit
+    takes a pinned allocation of its own)."""
 
     def __init__(self, storage="fp32", model="regularised", magic=3.0 / 16.0, bouzidi=True, galilean=None,
                  device=None, arena_bricks=65536, arena_records=16384, budget_bytes=None, max_cases=256,
-                 max_open=8, max_harmonics=16, variant=0, staging_bytes=64 << 20):
+                 max_open=8, max_harmonics=16, variant=0, staging_bytes=64 << 20, result_bytes=256 << 20):
         torch = _torch()
         if storage not in STORAGES:
             raise ValueError(f"unknown storage {storage!r}")
@@ -317,7 +320,11 @@ class Batch:
         self._tstream.synchronize()
         self._ring = torch.empty(int(staging_bytes), dtype=torch.uint8, pin_memory=True)   # upload staging
         self._ring_np = self._ring.numpy()
-        self._ring_pos, self._ring_busy, self._ring_pending = 0, [], []
+        self._ring_pos, self._ring_busy, self._ring_pending = 0, collections.deque(), []
+        # pinned pool of result reads, in 256-byte units, allocated at the first read (allocating pinned memory per
+        # read can wait for the device)
+        self._rpool, self._rpool_bytes = None, int(result_bytes)
+        self._rranges = _Ranges(self._rpool_bytes // 256)
         self.arenas = []
         self.topo = 0                      # bumped when the launch structure changes (graphs captured again)
         self.case_by_id = {}
@@ -339,15 +346,23 @@ class Batch:
         if n > cap:
             return torch.from_numpy(x).pin_memory()
         if self._ring_pos + n > cap:
+            self._staged()                 # the regions of this pass get their event before the ring wraps
             self._ring_pos = 0
         lo, hi = self._ring_pos, self._ring_pos + n
-        keep = []
-        for a, b, ev in self._ring_busy:
+        # the busy regions form a queue in the order their copies were queued on the stream, which is the order they
+        # complete in: copies already done leave the front; if a region still in flight overlaps the new one, waiting
+        # for the last such region's event completes it and every region before it
+        busy = self._ring_busy
+        while busy and busy[0][2].query():
+            busy.popleft()
+        last = -1
+        for k, (a, b, _) in enumerate(busy):
             if a < hi and lo < b:
-                ev.synchronize()
-            else:
-                keep.append((a, b, ev))
-        self._ring_busy = keep
+                last = k
+        if last >= 0:
+            busy[last][2].synchronize()
+            for _ in range(last + 1):
+                busy.popleft()
         self._ring_np[lo:lo + x.nbytes] = x.reshape(-1).view(np.uint8)
         self._ring_pos = hi
         self._ring_pending.append((lo, hi))
@@ -358,27 +373,59 @@ class Batch:
         if self._ring_pending:
             ev = _torch().cuda.Event()
             ev.record(self._tstream)
-            self._ring_busy += [(a, b, ev) for a, b in self._ring_pending]
+            self._ring_busy.extend((a, b, ev) for a, b in self._ring_pending)
             self._ring_pending = []
 
-    def _upload(self, dst, x):
+    def _result_buffer(self, shapes):
+        """Pinned host float32 tensors of the given shapes for a read's copies, in one region of the result pool, and
+        a function that returns the region; (fresh pinned tensors, None) when the pool has no room."""
+        torch = _torch()
+        if self._rpool is None and self._rpool_bytes:
+            self._rpool = torch.empty(self._rpool_bytes, dtype=torch.uint8, pin_memory=True)
+        sizes = [(int(np.prod(sh)) * 4 + 255) // 256 for sh in shapes]
+        start = self._rranges.alloc(sum(sizes)) if self._rpool is not None else None
+        if start is None:
+            return [torch.empty(sh, dtype=torch.float32, pin_memory=True) for sh in shapes], None
+        out, pos = [], start * 256
+        for sh, units in zip(shapes, sizes):
+            n = int(np.prod(sh))
+            out.append(self._rpool[pos:pos + 4 * n].view(torch.float32).view(sh))
+            pos += units * 256
+        return out, lambda: self._rranges.release(start, sum(sizes))
+
+    def _upload(self, dst, x, staged=True):
         """Asynchronous host-to-device copy into dst on the batch stream: ordered after the work already queued and
-        before the next launches, without blocking the host."""
+        before the next launches, without blocking the host. staged=False leaves the ring event to a later
+        ``_staged()`` (one event for a group of uploads)."""
         torch = _torch()
         x = np.asarray(x).astype(torch.empty(0, dtype=dst.dtype).numpy().dtype, copy=False).reshape(dst.shape)
         src = self._stage(x)
         with torch.cuda.stream(self._tstream):
             dst.copy_(src, non_blocking=True)
-        self._staged()
+        if staged:
+            self._staged()
 
-    def _to_device(self, x):
+    def _to_device(self, x, staged=True):
         """A new device tensor holding x, copied asynchronously on the batch stream."""
         torch = _torch()
         src = self._stage(x)
         with torch.cuda.stream(self._tstream):
             out = src.to(self.dev, non_blocking=True)
-        self._staged()
+        if staged:
+            self._staged()
         return out
+
+    def _scatter_rows(self, dst, rows, data):
+        """Rows `rows` (host int array) of the 2-D device view dst set to data (len(rows), dst.shape[1]): one upload
+        of the rows and their indices, then an index copy on the batch stream."""
+        torch = _torch()
+        if not len(rows):
+            return
+        idx = self._to_device(np.asarray(rows, np.int64), staged=False)
+        src = self._to_device(np.ascontiguousarray(data, np.float32), staged=False)
+        with torch.cuda.stream(self._tstream):
+            dst.index_copy_(0, idx, src)
+        self._staged()
 
     # --- capacity -------------------------------------------------------------------------------------------
     def nbytes(self):
@@ -770,20 +817,28 @@ class SetupRecorder:
 class PendingFields:
     """(pressure, velocity) of a case on their way to pinned host memory (``Case.read_async``)."""
 
-    def __init__(self, event, rho, u, device_copies):
+    def __init__(self, event, rho, u, device_copies, release=None):
         self.event, self._rho, self._u = event, rho, u
         self._keep = device_copies           # the gathered device tensors stay referenced until the copy is done
+        self._release = release              # returns the pinned region to the batch's result pool
 
     def ready(self):
         return self.event.query()
 
     def result(self):
         """Wait for the copies; (pressure, velocity) as ``Case.fields`` returns them, and whether every density
-        is finite (``Case.ok`` over the fluid cells)."""
+        is finite (``Case.ok`` over the fluid cells). The pinned region goes back to the pool (results copied out)."""
         self.event.synchronize()
         self._keep = None
-        rho = self._rho.numpy()
-        return (rho / 3.0, self._u.numpy()), bool(np.isfinite(rho).all())
+        rho, u = self._rho.numpy(), self._u.numpy()
+        if self._release is not None:
+            u = u.copy()
+            out = (rho / 3.0, u), bool(np.isfinite(rho).all())
+            self._rho = self._u = None
+            self._release()
+            self._release = None
+            return out
+        return (rho / 3.0, u), bool(np.isfinite(rho).all())
 
 
 class Case:
@@ -795,24 +850,27 @@ class Case:
         b0, r0, q0, s0 = spot
         self.batch, self.cid, self.lat, self.lay, self.arena = batch, cid, lat, lay, arena
         self.spot, self.b0, self.r0 = spot, b0, r0
-        self.N = int(lat["nbr"].shape[1])
+        self.N = int(lay["n_cells"])
         self.tau = 3.0 * nu_lu + 0.5
         self.steps = 0
         self.flux = None
         self.n_open = 0
         nb, nr = int(lay["n_bricks"]), int(lay["n_boundary"])
-        dev = batch.dev
-        self.slots = (lay["slot_of_cell"].astype(np.int64) + b0 * BC)
-        # every upload below is asynchronous on the batch stream: queued after the chunks in flight (which may still
-        # step a case that left this range) and before the next launches
-        self.slot_dev = batch._to_device(self.slots)
-        self.slot32 = batch._to_device(self.slots.astype(np.int32))
-        nbr = lay["nbr"].astype(np.int64)
-        batch._upload(arena.nbr[b0 * 27:(b0 + nb) * 27], np.where(nbr >= 0, nbr + b0, -1).astype(np.int32))
-        bidx = lay["bidx"].astype(np.int64)
-        batch._upload(arena.bidx[b0:b0 + nb], np.where(bidx >= 0, bidx + r0, -1).astype(np.int32))
-        up = batch._upload
+        # Every upload below is asynchronous on the batch stream: queued after the chunks in flight (which may still
+        # step a case that left this range) and before the next launches. The layout's arrays go up as they are and
+        # the arena offsets are added on the device (less host work per admission); one ring event covers them all.
+        up = lambda dst, x: batch._upload(dst, x, staged=False)
         with torch.cuda.stream(batch._tstream):
+            # slot of each cell (int32: the gathers index with it too)
+            self.slot32 = batch._to_device(lay["slot_of_cell"], staged=False)
+            self.slot32.add_(b0 * BC)
+            self.slot_dev = self.slot32
+            nbr = arena.nbr[b0 * 27:(b0 + nb) * 27]
+            up(nbr, lay["nbr"].reshape(-1))
+            nbr.add_(nbr.ge(0).int().mul_(b0))                      # neighbour bricks; -1 (absent) stays
+            bidx = arena.bidx[b0:b0 + nb]
+            up(bidx, lay["bidx"])
+            bidx.add_(bidx.ge(0).int().mul_(r0))                    # boundary records; -1 (interior) stays
             arena.om_brick[b0:b0 + nb].zero_()
             arena.om[b0 * BC:(b0 + nb) * BC].zero_()
             arena.f[b0 * BLOCK:(b0 + nb) * BLOCK].zero_()
@@ -820,9 +878,14 @@ class Case:
             if nr:
                 up(arena.lmask[r0 * BC:(r0 + nr) * BC], lay["lmask"].view(np.int32).reshape(-1))
                 up(arena.smask[r0 * BC:(r0 + nr) * BC], lay["smask"].view(np.int32).reshape(-1))
-                up(arena.uin[r0 * 3 * BC:(r0 + nr) * 3 * BC], lay["uin"].reshape(-1))
-                up(arena.qstart[r0:r0 + nr], (lay["qstart"][:-1].astype(np.int64) + q0).astype(np.int32))
-                up(arena.sstart[r0:r0 + nr], (lay["sstart"][:-1].astype(np.int64) + s0).astype(np.int32))
+                # inlet profiles: only the records with inlet cells read theirs
+                inl = np.flatnonzero((lay["lmask"] & np.uint32(1 << 27)).any(axis=1))
+                batch._scatter_rows(arena.uin.view(-1, 3 * BC), inl + r0, lay["uin"].reshape(nr, 3 * BC)[inl])
+                qs, ss = arena.qstart[r0:r0 + nr], arena.sstart[r0:r0 + nr]
+                up(qs, lay["qstart"][:-1])
+                qs.add_(q0)
+                up(ss, lay["sstart"][:-1])
+                ss.add_(s0)
                 up(arena.loff[r0 * BC:(r0 + nr) * BC], lay["loff"].view(np.int32).reshape(-1))
                 up(arena.ldir[r0 * 2:(r0 + nr) * 2], lay["ldir"].view(np.int32).reshape(-1))
                 if lay["n_links"]:
@@ -838,6 +901,7 @@ class Case:
             batch.force[c].zero_()
             batch.p_out[c].zero_()
             batch.q_last[c].zero_()
+        batch._staged()
         self.set_drive([1.0])
 
     # --- reference surface ----------------------------------------------------------------------------------
@@ -872,9 +936,11 @@ class Case:
         self.set_omega_slots(*sponge_slots(self.lat, self.lay, self.tau, length, tau_max, kinds))
 
     def set_omega_slots(self, slots, own):
-        """Per-slot omega of the case's bricks and the bricks that need it (from sponge_slots)."""
+        """Per-slot omega of the case's bricks and the bricks that need it (from sponge_slots). Only the flagged
+        bricks' slots are uploaded: the step reads a brick's per-slot omegas only when its flag is set."""
         nb, ar, b0 = int(self.lay["n_bricks"]), self.arena, self.b0
-        self.batch._upload(ar.om[b0 * BC:(b0 + nb) * BC], slots)
+        rows = np.flatnonzero(own)
+        self.batch._scatter_rows(ar.om.view(-1, BC), rows + b0, np.asarray(slots, np.float32).reshape(nb, BC)[rows])
         self.batch._upload(ar.om_brick[b0:b0 + nb], own)
 
     def set_outlet_pressure(self, p_lu):
@@ -897,7 +963,8 @@ class Case:
         self._set(self.batch.alpha[self.cid:self.cid + 1], [alpha])
         self.flux = {"w": np.ascontiguousarray(flux["w"], np.float32),
                      "slot": np.ascontiguousarray(flux["slot"], np.int32)}
-        self.flux_slots = self.slots[np.asarray(flux["idx"])].astype(np.int32)
+        self.flux_slots = (self.lay["slot_of_cell"][np.asarray(flux["idx"])].astype(np.int64) +
+                           self.b0 * BC).astype(np.int32)
         self.n_open = n
         self.batch._topology_changed(self.arena)
 
@@ -918,13 +985,12 @@ class Case:
         bt = self.batch
         with torch.cuda.stream(bt._tstream):
             rho, u = self.arena.mom.gather(bt.cur, self.slot_dev)
-            rh = torch.empty(rho.shape, dtype=rho.dtype, pin_memory=True)
-            uh = torch.empty(u.shape, dtype=u.dtype, pin_memory=True)
+            (rh, uh), release = bt._result_buffer([tuple(rho.shape), tuple(u.shape)])
             rh.copy_(rho, non_blocking=True)
             uh.copy_(u, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(bt._tstream)
-        return PendingFields(ev, rh, uh, (rho, u))
+        return PendingFields(ev, rh, uh, (rho, u), release)
 
     def fields(self):
         """(pressure, velocity) per cell in the lattice's order."""

@@ -9,7 +9,8 @@ Requests (lattice solves with a step count) arrive over time. The server keeps o
 * all admitted cases advance together in graph-captured chunks of ``chunk`` steps; a request's step count is
   rounded up to whole chunks, so it finishes at a chunk boundary. Its results are then copied to pinned host
   memory behind that chunk and its ranges are returned to the arena allocators at once: stream order keeps the
-  copies ahead of any reuse, so a completion never drains the device. The host queues at most two chunks ahead;
+  copies ahead of any reuse, so a completion never drains the device (the copies land in a pinned pool allocated
+  up front: a pinned allocation per read can wait for the device). The host queues at most two chunks ahead;
   joining and leaving cases do not invalidate the captured graphs (fixed-capacity lists). Uploads of admitted
   cases go through a large pinned ring (``staging_bytes``), so admitting a burst does not wait for the device;
 * per request the server records arrival, admission and completion times: time-to-solution = completion -
@@ -23,6 +24,7 @@ from __future__ import annotations
 import collections
 import concurrent.futures as cf
 import os
+import sys
 import time
 
 import numpy as np
@@ -32,7 +34,7 @@ from . import bricks
 
 
 def _summary(p, u, ok):
-    sp = np.linalg.norm(u, axis=1)
+    sp = np.sqrt(np.einsum("ij,ij->i", u, u))
     return {"u_max": float(sp.max()), "u_mean": float(sp.mean()), "p_mean": float(p.mean()), "ok": bool(ok)}
 
 
@@ -45,7 +47,11 @@ def _build(make_case, spec):
     lay = bricks.build(lat)
     rec = B.SetupRecorder(lat, nu, lay)
     setup(rec)
-    return lat, nu, rec, lay
+    # the served case needs the layout and the setter calls only: the lattice's large arrays are released here, on
+    # the worker, not on the serving thread
+    slim = {k: lat[k] for k in ("origin", "dx", "shape", "n_outlets", "counts", "openings") if k in lat}
+    rec.lat = slim
+    return slim, nu, rec, lay
 
 
 class Server:
@@ -54,12 +60,14 @@ class Server:
     storage: population storage of the cases ("fp16" or "fp32"); budget_bytes: HBM the arenas may take (default
     288 GB, capped by the device's free memory less 2 GiB); chunk: steps per graph launch; arena_bricks /
     arena_records: arena capacity (an arena grows to fit a larger case); workers: host threads building
-    lattices (default: the process's CPUs less one, at most 15); keep: "summary" or "fields" (also keep the final (p, u) of each request); staging_bytes: pinned
-    host ring for the uploads of admitted cases."""
+    lattices (default: the process's CPUs less one, at most 15); keep: "summary" or "fields" (also keep the final
+    (p, u) of each request); staging_bytes: pinned host ring for the uploads of admitted cases; result_bytes: pinned
+    host pool the finished cases' results are copied into; switch_interval: the interpreter's thread switch interval
+    while serving (seconds)."""
 
     def __init__(self, storage="fp16", budget_bytes=None, chunk=200, arena_bricks=1 << 20, arena_records=1 << 18,
-                 workers=None, keep="summary", max_cases=256, variant=0, device=None, staging_bytes=512 << 20,
-                 **solver_kw):
+                 workers=None, keep="summary", max_cases=256, variant=0, device=None, staging_bytes=1 << 30,
+                 result_bytes=512 << 20, switch_interval=2e-4, **solver_kw):
         import torch
         free = None
         if torch.cuda.is_available():
@@ -70,10 +78,11 @@ class Server:
         self.chunk = int(chunk) + int(chunk) % 2
         self.batch = B.Batch(storage=storage, budget_bytes=budget, arena_bricks=arena_bricks,
                              arena_records=arena_records, max_cases=max_cases, variant=variant, device=device,
-                             staging_bytes=staging_bytes, **solver_kw)
+                             staging_bytes=staging_bytes, result_bytes=result_bytes, **solver_kw)
         # builder threads: one per CPU the process may use, less one for this loop, at most 15
         self.workers = int(workers) if workers else max(1, min(15, len(os.sched_getaffinity(0)) - 1))
         self.keep = keep
+        self.switch_interval = float(switch_interval)
 
     def serve(self, requests, make_case, time_scale=1.0, log=None):
         """Serve `requests` (dicts with "name", "arrival" in seconds, "steps") whose lattices come from
@@ -84,15 +93,14 @@ class Server:
         bt, chunk = self.batch, self.chunk
         pending = collections.deque(sorted(requests, key=lambda r: r["arrival"]))
         building = {}
-        built = []                                        # (arrival, name, spec, build result)
+        built = []                                        # (arrival, name, spec, build result, time built)
         active = {}                                       # cid -> record
         reading = []                                      # finished cases whose results are on their way
         records = []
         lups = 0
-        inflight = []                                     # (end event, records read behind it) of queued chunks
+        inflight = []                                     # (start, end) events of the chunks queued on the device
         chunks = []                                       # (start, end) events of every chunk
         host = {"admit": 0.0, "launch": 0.0, "wait": 0.0, "finish": 0.0, "captures": 0}
-        last_chunk_s = 0.0                                # device time of the latest chunk known complete
         t0 = time.perf_counter()
 
         def finish(rec):
@@ -106,87 +114,102 @@ class Server:
             if log:
                 log(f"done {rec['name']} at {rec['done']:.2f} s (latency {rec['done'] - rec['arrival']:.2f} s)")
 
-        with cf.ThreadPoolExecutor(self.workers) as pool:
-            while pending or building or built or active or reading:
-                now = time.perf_counter() - t0
-                while pending and pending[0]["arrival"] * time_scale <= now:
-                    r = pending.popleft()
-                    building[pool.submit(_build, make_case, r)] = r
-                for fut in [f for f in building if f.done()]:
-                    r = building.pop(fut)
-                    built.append((r["arrival"], r["name"], r, fut.result()))
-                built.sort(key=lambda x: (x[0], x[1]))
-                # admission shares the host with the launches: admit for as long as the queued chunks keep the device
-                # busy (half their expected time, at least 20 ms), then queue the next chunk
-                th = time.perf_counter()
-                queued = sum(not c[1].query() for c in chunks[-3:])
-                budget = 0.02 + 0.5 * queued * last_chunk_s
-                while built and bt.can_admit(built[0][3][3]) and (time.perf_counter() - th < budget or not active):
-                    _, _, r, (lat, nu, setup, lay) = built.pop(0)
-                    case = bt.add(lat, nu, layout=lay)
-                    setup.replay(case)
-                    target = -(-int(r["steps"]) // chunk) * chunk
-                    active[case.cid] = {"name": r["name"], "spec": r, "case": case, "target": target,
-                                        "arrival": r["arrival"] * time_scale,
-                                        "admitted": time.perf_counter() - t0, "cells": case.N}
-                    if log:
-                        log(f"admit {r['name']} ({case.N} cells, {target} steps) at {active[case.cid]['admitted']:.2f} s"
-                            f", {len(active)} active")
-                th2 = time.perf_counter()
-                host["admit"] += th2 - th
-                for rec in [rec for rec in reading if rec["read"].ready()]:
-                    finish(rec)
-                host["finish"] += time.perf_counter() - th2
-                if not active:
-                    if reading:                 # nothing to step: wait for the results on their way
-                        th = time.perf_counter()
-                        finish(reading[0])
-                        host["finish"] += time.perf_counter() - th
+        def launch():
+            """Queue one chunk for the admitted cases; the cases done at its end are read out behind it and leave the
+            batch at once (stream-ordered), so no launch waits for the device."""
+            nonlocal lups, t_one
+            ng = bt.captures
+            e0 = torch.cuda.Event(enable_timing=True)
+            e0.record(bt._tstream)
+            bt.run(chunk, chunk=chunk)
+            host["captures"] += bt.captures - ng
+            cells = sum(rec["cells"] for rec in active.values())
+            lups += chunk * cells
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(bt._tstream)
+            chunks.append((e0, ev, time.perf_counter() - t0, len(active), cells))
+            for cid in [c for c, rec in active.items() if rec["case"].steps >= rec["target"]]:
+                rec = active.pop(cid)
+                rec["read"] = rec["case"].read_async()
+                rec["steps"] = rec["case"].steps
+                bt.remove(rec["case"])
+                del rec["case"]
+                rec["left"] = time.perf_counter() - t0
+                reading.append(rec)
+            inflight.append((e0, ev))
+            if len(inflight) == 1:
+                t_one = time.perf_counter()
+
+        def admit():
+            _, _, r, (lat, nu, setup, lay), t_built = built.pop(0)
+            case = bt.add(lat, nu, layout=lay)
+            setup.replay(case)
+            target = -(-int(r["steps"]) // chunk) * chunk
+            active[case.cid] = {"name": r["name"], "spec": r, "case": case, "target": target,
+                                "arrival": r["arrival"] * time_scale, "built": t_built,
+                                "admitted": time.perf_counter() - t0, "cells": case.N}
+            if log:
+                log(f"admit {r['name']} ({case.N} cells, {target} steps) at {active[case.cid]['admitted']:.2f} s, "
+                    f"{len(active)} active")
+
+        # One event-driven loop. It never lets the device run dry while cases are admitted: with no chunk queued it
+        # launches at once; with one queued it admits built requests (in arrival order) for up to half that chunk's
+        # expected time, so that they join the next chunk, then queues the next; with two queued it keeps admitting.
+        # It collects results as their copies land and sleeps briefly only when none of this has work (the builder
+        # threads then have the interpreter).
+        last_chunk_s = 0.0                                # device time of the latest chunk known complete
+        t_one = None                                      # when the queue last came down to one chunk
+        # the loop gives up the interpreter often (device queries, short sleeps) and must get it back at once from
+        # the builder threads: a short switch interval while serving (at the default 5 ms every hand-back can wait)
+        switch = sys.getswitchinterval()
+        sys.setswitchinterval(self.switch_interval)
+        try:
+            with cf.ThreadPoolExecutor(self.workers) as pool:
+                while pending or building or built or active or reading:
+                    now = time.perf_counter() - t0
+                    while pending and pending[0]["arrival"] * time_scale <= now:
+                        r = pending.popleft()
+                        building[pool.submit(_build, make_case, r)] = r
+                    done_builds = [f for f in building if f.done()]
+                    for fut in done_builds:
+                        r = building.pop(fut)
+                        built.append((r["arrival"], r["name"], r, fut.result(), time.perf_counter() - t0))
+                    if done_builds:
+                        built.sort(key=lambda x: (x[0], x[1]))
+                    while inflight and inflight[0][1].query():
+                        e0, e1 = inflight.pop(0)
+                        last_chunk_s = e0.elapsed_time(e1) * 1e-3
+                        if len(inflight) == 1:
+                            t_one = time.perf_counter()
+                    th = time.perf_counter()
+                    ready = [rec for rec in reading if rec["read"].ready()]
+                    for rec in ready:
+                        finish(rec)
+                    host["finish"] += time.perf_counter() - th
+                    th = time.perf_counter()
+                    can = bool(built) and bt.can_admit(built[0][3][3])
+                    if active and (not inflight or (len(inflight) == 1 and
+                                                    (not can or th - t_one > 0.5 * last_chunk_s))):
+                        launch()
+                        host["launch"] += time.perf_counter() - th
                         continue
-                    if built:       # the batch is empty and the head of the queue still does not fit
+                    if can:
+                        admit()
+                        host["admit"] += time.perf_counter() - th
+                        continue
+                    if built and not active and not reading and not inflight:
+                        # the batch is empty and the head of the queue still does not fit
                         raise RuntimeError(f"request {built[0][1]} does not fit the HBM budget even alone")
-                    waits = list(building)
-                    if waits:
-                        cf.wait(waits, timeout=0.01, return_when=cf.FIRST_COMPLETED)
-                    elif pending:
-                        time.sleep(max(0.0, min(0.01, pending[0]["arrival"] * time_scale - now)))
-                    continue
-                th = time.perf_counter()
-                ng = len(bt.graphs)
-                e0 = torch.cuda.Event(enable_timing=True)
-                e0.record(bt._tstream)
-                bt.run(chunk, chunk=chunk)
-                host["captures"] += len(bt.graphs) > ng
-                lups += chunk * sum(rec["cells"] for rec in active.values())
-                ev = torch.cuda.Event(enable_timing=True)
-                ev.record(bt._tstream)
-                if len(chunks) >= 1 and chunks[-1][1].query():
-                    last_chunk_s = chunks[-1][0].elapsed_time(chunks[-1][1]) * 1e-3
-                chunks.append((e0, ev, th - t0, len(active), sum(rec["cells"] for rec in active.values())))
-                # cases done at the end of this chunk: their results are copied out behind it and they leave the
-                # batch at once (stream-ordered), so the next chunk is queued without waiting for the device
-                done_now = []
-                for cid in [c for c, rec in active.items() if rec["case"].steps >= rec["target"]]:
-                    rec = active.pop(cid)
-                    rec["read"] = rec["case"].read_async()
-                    rec["steps"] = rec["case"].steps
-                    bt.remove(rec["case"])
-                    del rec["case"]
-                    rec["left"] = time.perf_counter() - t0
-                    reading.append(rec)
-                    done_now.append(rec)
-                inflight.append((ev, done_now))
-                th2 = time.perf_counter()
-                host["launch"] += th2 - th
-                if len(inflight) > 2:            # run at most two chunks ahead of the device
-                    ev0, recs = inflight.pop(0)
-                    recs = [rec for rec in recs if "read" in rec]
-                    if recs:
-                        for rec in recs:         # their copies follow the chunk: waiting for them waits for it
-                            finish(rec)
+                    if ready:
+                        continue
+                    # nothing to do now: wait a little for a chunk, a copy, a build or an arrival
+                    if building and not inflight and not reading:
+                        cf.wait(list(building), timeout=0.002, return_when=cf.FIRST_COMPLETED)
                     else:
-                        ev0.synchronize()
-                host["wait"] += time.perf_counter() - th2
+                        time.sleep(0.0005)
+                    host["wait"] += time.perf_counter() - th
+        finally:
+            sys.setswitchinterval(switch)
         wall = time.perf_counter() - t0
         bt.synchronize()
         gpu = sum(c[0].elapsed_time(c[1]) for c in chunks) * 1e-3
