@@ -163,8 +163,8 @@ constexpr SlotTable make_oddpos() {
 }
 
 // The odd-step footprint of an interior brick in staged order: the own group (block offsets [0, kOwn) of the own
-// block), then the 26 neighbour records. kRecords.e[p] = (o << 11) | block offset, in brick tbl[o], of staged
-// position kOwn + p.
+// block), then the 26 neighbour records. Record table entry e[p] = (o << 11) | block offset, in the brick at
+// neighbour offset o, of staged position kOwn + p (exported for tests; the kernels use the runs below).
 constexpr int kOwn = 1000;
 constexpr int kRecs = Q * BC - kOwn;
 struct RecordTable {
@@ -184,11 +184,14 @@ constexpr RecordTable make_records() {
 }
 
 // The same entries as runs: the 26 records in staged order, record r covering staged positions [start[r],
-// start[r + 1]) with entry = base[r] + p (base[r] = (o << 11) + block offset of the record's first slot - start[r]).
-// The odd step computes its record addresses from these compile-time runs instead of loading a table.
+// start[r + 1]) with entry = base[r] + p (base[r] = (o << 11) + block offset of the record's first slot - start[r]),
+// i.e. block offset off[r] + p in the brick at neighbour offset o = nbo[r]. The odd step computes its record
+// addresses from these compile-time runs and the neighbour ids it holds in scalar registers.
 struct RecordRuns {
     int start[26];
     int base[26];
+    int nbo[26];
+    int off[26];
 };
 
 constexpr RecordRuns make_runs() {
@@ -200,6 +203,8 @@ constexpr RecordRuns make_runs() {
         const int g = 26 - h;
         t.start[r] = p;
         t.base[r] = (h << 11) + group_start(g) - p;
+        t.nbo[r] = h;
+        t.off[r] = group_start(g) - p;
         p += group_size(g);
         ++r;
     }
@@ -221,45 +226,38 @@ constexpr bool runs_match_records() {
 
 static_assert(group_start(13) == 0 && group_size(13) == kOwn, "own group first");
 static_assert(stage_start(stage_at(1)) == kOwn, "records after the own group");
-__constant__ RecordTable kRecords = make_records();
 static_assert(runs_match_records(), "record runs give the record table");
 
-// Entry (o << 11) | block offset of staged record position p in [LO, HI] (the positions one load instruction
-// covers): the last record starting at or before p (a select chain over the records the range touches).
+// Element address of staged record position p in [LO, HI] (the positions one load instruction covers): that of the
+// last record starting at or before p, nb[o] * 1728 + block offset, as a select chain over the records the range
+// touches. The neighbour ids nb are in scalar registers, so no table lookup stands between the brick's id and its
+// record loads.
 template <int LO, int HI>
-__device__ __forceinline__ uint32_t record_entry(int p) {
-    constexpr int lo = LO, hi = HI;
+__device__ __forceinline__ uint32_t record_addr(int p, const int (&nb)[27]) {
     uint32_t e = 0u;
     static_for<0, 26>([&](auto rc) {
         constexpr int r = decltype(rc)::value;
         constexpr int s = kRuns.start[r];
         constexpr int end = r + 1 < 26 ? kRuns.start[r + 1] : kRecs;
-        if constexpr (s <= hi && end > lo) {
-            if constexpr (s <= lo) e = (uint32_t)(kRuns.base[r] + p);
-            else if (p >= s) e = (uint32_t)(kRuns.base[r] + p);
+        if constexpr (s <= HI && end > LO) {
+            const uint32_t v = brick_base(nb[kRuns.nbo[r]]) + (uint32_t)(kRuns.off[r] + p);
+            if constexpr (s <= LO) e = v;
+            else if (p >= s) e = v;
         }
     });
     return e;
 }
+static_assert([] {
+    for (int r = 0; r < 26; ++r)
+        if (kRuns.base[r] != (kRuns.nbo[r] << 11) + kRuns.off[r]) return false;
+    return true;
+}(), "record runs: base = (o << 11) + off");
 __constant__ SlotTable kOddPos = make_oddpos();
 
-// Float16 odd step: the face and edge records as 4-byte pairs (staged positions kOwn + 2p, 2p + 1; entry (o << 11)
-// | block offset of the pair's first slot), then the 8 corner slots one by one (staged positions kOwn + 2 kNPairs
-// + c).
+// Float16 odd step: the face and edge records as 4-byte pairs (staged positions kOwn + 2p, 2p + 1, one record and
+// an even block offset), then the 8 corner slots one by one (staged positions kOwn + 2 kNPairs + c).
 constexpr int kNPairs = (6 * 100 + 12 * 10) / 2;
 constexpr int kNCorners = 8;
-struct PairTable {
-    uint16_t e[kNPairs];
-    uint16_t c[kNCorners];
-};
-
-constexpr PairTable make_pairs() {
-    PairTable t{};
-    constexpr RecordTable r = make_records();
-    for (int p = 0; p < kNPairs; ++p) t.e[p] = r.e[2 * p];
-    for (int c = 0; c < kNCorners; ++c) t.c[c] = r.e[2 * kNPairs + c];
-    return t;
-}
 
 constexpr bool pairs_valid() {
     constexpr RecordTable r = make_records();
@@ -270,21 +268,12 @@ constexpr bool pairs_valid() {
 }
 static_assert(pairs_valid(), "face and edge records start at even offsets and pair up inside one record");
 static_assert(kOwn % 8 == 0, "the staged records start 16-byte aligned in float16");
-__constant__ PairTable kPairs = make_pairs();
 
 // Cross-lane LDS hand-over inside one wavefront.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Copy a 27 x 64 uint16 table into LDS (216 x 16 bytes) with the workgroup's nthreads threads.
-__device__ __forceinline__ void load_table(uint32_t* lds, const SlotTable& src, int nthreads) {
-    for (int k = threadIdx.x; k < Q * BC * 2 / 16; k += nthreads) {
-        const uint4 v = reinterpret_cast<const uint4*>(&src)[k];
-        reinterpret_cast<uint4*>(lds)[k] = v;
-    }
 }
 
 // Direct global -> LDS load of `size` bytes per lane (16 or 4): lane L's bytes land at lds + L * size. The LDS base
@@ -297,13 +286,36 @@ __device__ __forceinline__ void load_to_lds(const void* src, void* lds) {
     else __builtin_amdgcn_global_load_lds(src, dst, 4, 0, 0);
 }
 
+// Copy a 27 x 64 uint16 table (216 x 16 bytes) into LDS with the workgroup's W waves, straight to LDS: nothing waits
+// for it here (vmcnt and a workgroup barrier before the first read).
+template <int W>
+__device__ __forceinline__ void load_table(uint32_t* lds, const SlotTable& src, int wave, int L) {
+    constexpr int NV = Q * BC * 2 / 16;
+    static_for<0, (NV + 64 * W - 1) / (64 * W)>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int row = j * W + wave;  // 64 vectors per row, wave-uniform
+        if (row * 64 < NV && row * 64 + L < NV)
+            load_to_lds<16>(reinterpret_cast<const uint4*>(&src) + row * 64 + L,
+                            reinterpret_cast<uint4*>(lds) + row * 64);
+    });
+}
+
+// Brick id of the position (x, y, z) + e_a (one unit step, SX + SY + SZ = +-1): the brick itself or its face
+// neighbour, from the neighbour ids in scalar registers.
+template <int SX, int SY, int SZ>
+__device__ __forceinline__ int face_brick(int x, int y, int z, const int (&nb)[27]) {
+    static_assert((SX != 0) + (SY != 0) + (SZ != 0) == 1, "one axis");
+    const bool out = SX < 0 ? x == 0 : SX > 0 ? x == 3 : SY < 0 ? y == 0 : SY > 0 ? y == 3 : SZ < 0 ? z == 0 : z == 3;
+    return out ? nb[13 + SX + 3 * SY + 9 * SZ] : nb[13];
+}
+
 // Move the footprint of brick b between global memory and the wave's LDS stage with full-row accesses. DMA: loads
 // go straight to LDS (interior bricks; boundary bricks keep the register copy, which times faster for them).
 // Even steps: the whole own block. Odd steps (interior bricks): the own group as 16-byte vectors, then the 26
-// neighbour records packed lane by lane (rec: the record table, in LDS or constant memory).
+// neighbour records packed lane by lane (nb: the brick's neighbour ids). Loads are only issued here: the caller
+// waits for them (vmcnt) before the stage is read.
 template <int PAR, bool STORE, bool DMA, typename T>
-__device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int* tbl, const uint16_t* rec,
-                                           uint32_t base, int L) {
+__device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int (&nb)[27], uint32_t base, int L) {
     if constexpr (PAR == 0) {
         // the whole block as 16-byte vectors (a block is Q * BC * sizeof(T) bytes, a multiple of 16)
         constexpr int NV = Q * BC * (int)sizeof(T) / 16;
@@ -321,7 +333,6 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
                 else l[k] = g[k];
             }
         });
-        if constexpr (!STORE && (DMA || NV % BC != 0)) __builtin_amdgcn_s_waitcnt(0);
     } else {
         static_assert(kOwn * sizeof(T) % 16 == 0, "own group in 16-byte vectors");
         constexpr int NV = kOwn * (int)sizeof(T) / 16;
@@ -343,16 +354,14 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
                 constexpr int j = decltype(jc)::value;
                 const int p = j * BC + L;
                 if ((j + 1) * BC <= kNPairs || p < kNPairs) {
-                    const uint32_t e = record_entry<2 * j * BC, 2 * (j * BC + BC - 1)>(2 * p);
-                    uint32_t* g = reinterpret_cast<uint32_t*>(F + brick_base(tbl[e >> 11]) + (e & 2047u));
+                    uint32_t* g = reinterpret_cast<uint32_t*>(F + record_addr<2 * j * BC, 2 * (j * BC + BC - 1)>(2 * p, nb));
                     if constexpr (STORE) *g = lp[p];
                     else if constexpr (DMA) load_to_lds<4>(g, lp + j * BC);
                     else lp[p] = *g;
                 }
             });
             if (L < kNCorners) {
-                const uint32_t e = record_entry<2 * kNPairs, 2 * kNPairs + kNCorners - 1>(2 * kNPairs + L);
-                const uint32_t at = brick_base(tbl[e >> 11]) + (e & 2047u);
+                const uint32_t at = record_addr<2 * kNPairs, 2 * kNPairs + kNCorners - 1>(2 * kNPairs + L, nb);
                 if constexpr (STORE) F[at] = stage[kOwn + 2 * kNPairs + L];
                 else stage[kOwn + 2 * kNPairs + L] = F[at];
             }
@@ -361,15 +370,13 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
                 constexpr int j = decltype(jc)::value;
                 const int p = j * BC + L;
                 if ((j + 1) * BC <= kRecs || p < kRecs) {
-                    const uint32_t e = record_entry<j * BC, j * BC + BC - 1>(p);
-                    const uint32_t at = brick_base(tbl[e >> 11]) + (e & 2047u);
+                    const uint32_t at = record_addr<j * BC, j * BC + BC - 1>(p, nb);
                     if constexpr (STORE) F[at] = stage[kOwn + p];
                     else if constexpr (DMA) load_to_lds<4>(F + at, stage + kOwn + j * BC);
                     else stage[kOwn + p] = F[at];
                 }
             });
         }
-        if constexpr (!STORE && DMA) __builtin_amdgcn_s_waitcnt(0);
     }
 }
 
@@ -427,22 +434,36 @@ __device__ __forceinline__ float h16_decode_g(_Float16 h, const float4& m) {
 
 // Stage the previous-step moments {drho - 1.5 |u|^2, ux, uy, uz} of the brick and its one-cell shell in LDS
 // (mom: float4 {drho, ux, uy, uz} per slot). Slots of absent neighbour bricks (boundary bricks only: every neighbour
-// of an interior brick exists) read as zero.
+// of an interior brick exists) read as zero. Two parts: halo_fetch issues the loads into registers (absent: bit k
+// for chunk k), halo_put converts and stores them once they have arrived, so that other loads issue in between.
+constexpr int kHaloChunks = (kHalo + BC - 1) / BC;
+
 template <bool BND>
-__device__ __forceinline__ void load_halo(float4* halo, const int* tbl, const float4* __restrict__ mom, int L) {
-    static_for<0, (kHalo + BC - 1) / BC>([&](auto kc) {
+__device__ __forceinline__ void halo_fetch(float4 (&m)[kHaloChunks], uint32_t& absent, const int* tbl,
+                                           const float4* __restrict__ mom, int L) {
+    static_for<0, kHaloChunks>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int e = min(k * BC + L, kHalo - 1);  // lanes past the shell re-read its last entry (not stored)
+        const int hx = e % 6, hy = (e / 6) % 6, hz = e / 36;
+        const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
+                      9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
+        const int l = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
+        const int nb = tbl[o];
+        m[k] = mom[(uint32_t)(BND && nb < 0 ? 0 : nb) * BC + l];
+        if (BND && nb < 0) absent |= 1u << k;
+    });
+}
+
+template <bool BND>
+__device__ __forceinline__ void halo_put(float4* halo, const float4 (&m)[kHaloChunks], uint32_t absent, int L) {
+    static_for<0, kHaloChunks>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         const int e = k * BC + L;
         if ((k + 1) * BC <= kHalo || e < kHalo) {
-            const int hx = e % 6, hy = (e / 6) % 6, hz = e / 36;
-            const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
-                          9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
-            const int l = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
-            const int nb = tbl[o];
-            float4 m = mom[(uint32_t)(BND && nb < 0 ? 0 : nb) * BC + l];
-            m.x = m.x - 1.5f * (m.y * m.y + m.z * m.z + m.w * m.w);
-            if (BND && nb < 0) m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            halo[e] = m;
+            float4 v = m[k];
+            v.x = v.x - 1.5f * (v.y * v.y + v.z * v.z + v.w * v.w);
+            if (BND && ((absent >> k) & 1u)) v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            halo[e] = v;
         }
     });
 }
@@ -499,10 +520,15 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     // form (every formula is linear in the populations; the Bouzidi side copies stay actual populations), and the
     // collision writes the stored form directly (collide<1, false, true, true>).
     constexpr bool FUSED16 = H16 && MODEL == 1;
-    __shared__ int tbl[WAVES][32];
+    // per-lane lookups of the neighbour ids (odd boundary gathers, the float16 moment halo) read them from LDS
+    constexpr bool LDS_TBL = H16 || (BND && PAR == 1);
+    // FP32 + GAL: the neighbour velocities of the gradient stencil are loaded with the populations, except in odd
+    // boundary steps (their direct gathers hold more registers: holding these too would lower the occupancy)
+    constexpr bool HOIST_GRAD = !H16 && GAL && (PAR == 0 || !BND);
+    __shared__ int tbl[LDS_TBL ? WAVES : 1][32];
     __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
-    // the odd-step record table (FP32) is read from constant memory, so that five workgroups of four waves fit the
-    // LDS of a compute unit; FP16 moves the records as pairs (kPairs, also constant memory)
+    // the odd-step record addresses are computed from compile-time runs, so that five workgroups of four waves fit
+    // the LDS of a compute unit
     __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
     __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
     // odd boundary steps: each wave's copy of its brick's link and side runs (even steps use the free stage)
@@ -511,27 +537,33 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
     const int per = gridDim.x >> 3;
     const int wg = xcd ? (int)((blockIdx.x & 7) * per + (blockIdx.x >> 3)) : (int)blockIdx.x;
-    // a workgroup whose entries are all skips (the free capacity of a batch's lists) returns before any set-up; the
-    // test is uniform over the workgroup, so no barrier is left waiting
-    bool any = false;
-    for (int w = 0; w < WAVES; ++w) {
-        const int64_t lw = (int64_t)wg * WAVES + w;
-        any |= lw < a.n_list && a.list[lw] != CS_LIST_SKIP;
-    }
-    if (!any) return;
-    load_table(slots32, ODD_STAGED ? kOddPos : kSlots, 64 * W);
-    const uint16_t* rec = kRecords.e;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int L = threadIdx.x & 63;
     const int64_t li = (int64_t)wg * WAVES + wave;
-    const uint32_t ent = li < a.n_list ? a.list[li] : CS_LIST_SKIP;
+    // list entry = brick | case << CS_CASE_SHIFT (case 0 for a single lattice). A wave past the list or on a skip
+    // entry only takes its part of the slot table copy (the workgroup barrier below waits for every wave).
+    const uint32_t e0 = a.list[li < a.n_list ? li : a.n_list - 1];  // (launches have n_list >= 1)
+    const uint32_t ent = li < a.n_list ? e0 : CS_LIST_SKIP;
     const bool valid = ent != CS_LIST_SKIP;
-    // list entry = brick | case << CS_CASE_SHIFT (case 0 for a single lattice)
     const int b = valid ? (int)(ent & ((1u << CS_CASE_SHIFT) - 1u)) : 0;
     const int cs = (int)(ent >> CS_CASE_SHIFT);
-    if (L < 27) tbl[wave][L] = valid ? a.nbr[(int64_t)b * 27 + L] : -1;
-    __syncthreads();
-    if (!valid) return;  // wave-uniform
+    // The brick's 27 neighbour ids (13: the brick itself; -1: absent, around boundary bricks only), loaded as
+    // scalars: the odd-step record addresses and the gradient stencil select among them, so that no table lookup
+    // stands between the brick's id and its loads, and every load of the wave issues in one burst below.
+    int nb[27];
+    static_for<0, 27>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        nb[k] = a.nbr[(int64_t)b * 27 + k];
+    });
+    if constexpr (LDS_TBL) {
+        int v = -1;
+        static_for<0, 27>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            v = L == k ? nb[k] : v;
+        });
+        if (L < 32) tbl[wave][L] = v;
+        wave_sync();
+    }
 
     const int x = L & 3, y = (L >> 2) & 3, z = L >> 4;
     const uint32_t S = (uint32_t)a.n_slots;
@@ -540,66 +572,79 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     T* __restrict__ F = reinterpret_cast<T*>(a.f);
     T* stage = stage_all + (STAGED ? wave * Q * BC : 0);
     float4* halo = halo_all + (H16 ? wave * kHalo : 0);
-    if constexpr (H16) {
-        load_halo<BND>(halo, tbl[wave], reinterpret_cast<const float4*>(a.vel_old), L);
-        wave_sync();
-    }
 
     uint32_t mask = kAllLinks;
     uint32_t sm = 0u, lo = 0u;  // boundary bricks: Bouzidi side bits and the lane's link-run offsets
     uint32_t wq = 0u, ws = 0u;  // boundary bricks: the brick's link unions (ldir)
     int bb = 0, qs = 0, ss = 0;  // boundary bricks: record, starts of its link and side runs
-    bool own_om = false;         // the brick has per-cell omegas
-    if constexpr (BND) {
-        bb = a.brec ? a.brec[li] : (int)li;  // the entry's boundary record: bidx[brick] == bb
-        // the per-cell boundary words and the brick's run starts are loaded up front, so the link loads later wait
-        // for one round trip only
-        mask = a.lmask[(uint32_t)bb * BC + L];
-        sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
-        lo = a.loff[(uint32_t)bb * BC + L];
-        wq = a.ldir[2 * bb];
-        ws = a.bouzidi ? a.ldir[2 * bb + 1] : 0u;
-        qs = a.qstart[bb];
-        ss = a.sstart[bb];
-        own_om = a.om_brick[b] != 0;
+    uint32_t own_om = 0u;        // the brick has per-cell omegas (om_brick byte)
+    // Inputs that do not depend on the populations are loaded before the populations arrive (the moment halo, the
+    // boundary words, the cell's omega and previous velocity, the gradient stencil's neighbour velocities), so that
+    // all round trips of the wave overlap. Every lane loads them (solid lanes too: all the addresses exist), so
+    // that no divergent branch makes the wave wait for them; the omega of the cell is selected where it is used.
+    float om_c = 0.0f, om_k = 0.0f, d = 0.0f, up[3] = {0.0f, 0.0f, 0.0f};
+    // velocity ax at x - e_ax, x + e_ax (HOIST_GRAD)
+    float gnb[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float4 hm[(kHalo + BC - 1) / BC];
+    uint32_t habs = 0u;
+    if (valid) {
+        // the wave-uniform values first: scalar loads, issued before any direct-to-LDS copy (a store, after which
+        // the compiler no longer takes a uniform load for a scalar one)
+        if (a.case_omega) om_k = a.case_omega[cs];
+        else om_k = a.omega;
+        d = a.drive[cs];
+        if constexpr (BND) {
+            bb = a.brec ? a.brec[li] : (int)li;  // the entry's boundary record: bidx[brick] == bb
+            wq = a.ldir[2 * bb];
+            ws = a.bouzidi ? a.ldir[2 * bb + 1] : 0u;
+            qs = a.qstart[bb];
+            ss = a.sstart[bb];
+        }
+        if constexpr (H16) halo_fetch<BND>(hm, habs, tbl[wave], reinterpret_cast<const float4*>(a.vel_old), L);
+        own_om = a.om_brick[b];
+        if constexpr (BND) {
+            mask = a.lmask[(uint32_t)bb * BC + L];
+            sm = a.bouzidi ? a.smask[(uint32_t)bb * BC + L] : 0u;
+            lo = a.loff[(uint32_t)bb * BC + L];
+            om_c = a.om_cell[slot];
+            if constexpr (H16) {
+                const float4 mo = reinterpret_cast<const float4*>(a.vel_old)[slot];
+                up[0] = mo.y;
+                up[1] = mo.z;
+                up[2] = mo.w;
+            } else {
+                up[0] = a.vel_old[slot];
+                up[1] = a.vel_old[S + slot];
+                up[2] = a.vel_old[2 * S + slot];
+            }
+        }
+        if constexpr (HOIST_GRAD)
+            static_for<0, 3>([&](auto ac) {
+                constexpr int ax = decltype(ac)::value;
+                constexpr int sx = ax == 0, sy = ax == 1, sz = ax == 2;
+                int d1, l1, d2, l2;
+                shift_pos<-sx, -sy, -sz>(x, y, z, d1, l1);
+                shift_pos<sx, sy, sz>(x, y, z, d2, l2);
+                const float* V = a.vel_old + ax * S;
+                // around boundary bricks an absent neighbour brick (-1) reads brick 0 instead: a valid address
+                // whose value the link rules discard (the link is then not fluid)
+                gnb[2 * ax] = V[(uint32_t)max(face_brick<-sx, -sy, -sz>(x, y, z, nb), 0) * BC + l1];
+                gnb[2 * ax + 1] = V[(uint32_t)max(face_brick<sx, sy, sz>(x, y, z, nb), 0) * BC + l2];
+            });
+        if constexpr (STAGED) stage_copy<PAR, false, !BND>(F, stage, nb, base, L);
+    }
+    load_table<W>(slots32, ODD_STAGED ? kOddPos : kSlots, wave, L);
+    // the slot table (every wave's part) and the stage have landed in LDS
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (!valid) return;  // wave-uniform
+    // the byte is first used here: keeps its test (and the wait for it) after the barrier, not amid the loads
+    asm volatile("" : "+v"(own_om));
+    if constexpr (H16) {
+        halo_put<BND>(halo, hm, habs, L);
+        wave_sync();
     }
     const bool live = mask != 0u;  // solid slots of boundary bricks only join the staging copies
-    // Boundary bricks wait on memory for most of their life: the inputs that do not depend on the populations (the
-    // cell's omega, its previous velocity and, in FP32, the neighbour velocities of the gradient stencil, read at
-    // addresses that always exist) are loaded before the streaming copy, so that their round trips overlap it.
-    // Every lane loads them (solid lanes too: all the addresses exist), so that no divergent branch makes the wave
-    // wait for them before the copy; the omega of the cell is selected where it is used.
-    float om_c = 0.0f, om_k = 0.0f, up[3] = {0.0f, 0.0f, 0.0f};
-    // FP32 + GAL: velocity ax at x - e_ax, x + e_ax. Loaded here in even steps; odd steps load them at their use
-    // (their direct gathers hold more registers: holding these too would lower the occupancy)
-    constexpr bool HOIST_GRAD = PAR == 0;
-    float gnb[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if constexpr (BND) {
-        om_c = a.om_cell[slot];
-        om_k = a.case_omega ? a.case_omega[cs] : a.omega;
-        if constexpr (H16) {
-            const float4 mo = reinterpret_cast<const float4*>(a.vel_old)[slot];
-            up[0] = mo.y;
-            up[1] = mo.z;
-            up[2] = mo.w;
-        } else {
-            up[0] = a.vel_old[slot];
-            up[1] = a.vel_old[S + slot];
-            up[2] = a.vel_old[2 * S + slot];
-            if constexpr (GAL && HOIST_GRAD)
-                static_for<0, 3>([&](auto ac) {
-                    constexpr int ax = decltype(ac)::value;
-                    int d1, l1, d2, l2;
-                    shift_pos<ax == 0 ? -1 : 0, ax == 1 ? -1 : 0, ax == 2 ? -1 : 0>(x, y, z, d1, l1);
-                    shift_pos<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z, d2, l2);
-                    const float* V = a.vel_old + ax * S;
-                    // an absent neighbour brick (-1) reads brick 0 instead: a valid address whose value the link
-                    // rules discard (the link is then not fluid)
-                    gnb[2 * ax] = V[(uint32_t)max(tbl[wave][d1], 0) * BC + l1];
-                    gnb[2 * ax + 1] = V[(uint32_t)max(tbl[wave][d2], 0) * BC + l2];
-                });
-        }
-    }
 
     // ---- streaming: raw values ---------------------------------------------------------------------------
     float f[Q];
@@ -620,8 +665,6 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         }
     };
     if constexpr (STAGED) {
-        stage_copy<PAR, false, !BND>(F, stage, tbl[wave], rec, base, L);
-        wave_sync();
         if (live)
             static_for<0, Q>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
@@ -672,9 +715,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         wave_sync();
     }
     if (live) {
-        const float om = BND ? (own_om ? om_c : om_k)
-                             : (a.om_brick[b] ? a.om_cell[slot] : (a.case_omega ? a.case_omega[cs] : a.omega));
-        const float d = a.drive[cs];
+        // interior bricks read a per-cell omega only where the brick has them (sponge bricks: rare)
+        const float om = own_om ? (BND ? om_c : a.om_cell[slot]) : om_k;
         const float* p_out = a.p_out + (int64_t)cs * a.p_out_stride;
         // ---- boundary links and the previous velocity --------------------------------------------------------
         float q16[6];  // BND + GAL: qc of links 1..6 (read by the gradient stencil), loaded with the link batch
@@ -753,15 +795,19 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                 shift_pos<ax == 0 ? -1 : 0, ax == 1 ? -1 : 0, ax == 2 ? -1 : 0>(x, y, z, d1, l1);
                 shift_pos<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z, d2, l2);
                 const float* V = a.vel_old + ax * S;
-                // neighbour velocity component ax of the previous step at x -+ e_ax
+                // neighbour velocity component ax of the previous step at x -+ e_ax: from the halo (FP16), loaded
+                // with the populations (HOIST_GRAD) or here (odd boundary steps)
                 auto nbv = [&](bool minus) -> float {
                     if constexpr (H16) {
                         const int e = minus ? halo_index<ax == 0 ? -1 : 0, ax == 1 ? -1 : 0, ax == 2 ? -1 : 0>(x, y, z)
                                             : halo_index<ax == 0 ? 1 : 0, ax == 1 ? 1 : 0, ax == 2 ? 1 : 0>(x, y, z);
                         const float4 m = halo[e];
                         return ax == 0 ? m.y : (ax == 1 ? m.z : m.w);
+                    } else if constexpr (HOIST_GRAD) {
+                        return minus ? gnb[2 * ax] : gnb[2 * ax + 1];
                     } else {
-                        return minus ? V[(uint32_t)tbl[wave][d1] * BC + l1] : V[(uint32_t)tbl[wave][d2] * BC + l2];
+                        return minus ? V[(uint32_t)max(tbl[wave][d1], 0) * BC + l1]
+                                     : V[(uint32_t)max(tbl[wave][d2], 0) * BC + l2];
                     }
                 };
                 if constexpr (!BND) {
@@ -773,17 +819,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                     // address that always exists), so the loads issue together; then the wall and open-link rules
                     // select: a wall link at distance max(q, 0.1) with zero velocity, an open link the cell's own
                     const bool fl1 = (mask >> i1) & 1u, fl2 = (mask >> i2) & 1u;
-                    float n1, n2;
-                    if constexpr (H16) {
-                        n1 = nbv(true);
-                        n2 = nbv(false);
-                    } else if constexpr (HOIST_GRAD) {
-                        n1 = gnb[2 * ax];
-                        n2 = gnb[2 * ax + 1];
-                    } else {
-                        n1 = V[(uint32_t)max(tbl[wave][d1], 0) * BC + l1];
-                        n2 = V[(uint32_t)max(tbl[wave][d2], 0) * BC + l2];
-                    }
+                    const float n1 = nbv(true), n2 = nbv(false);
                     const float c1 = q16[i1 - 1], c2 = q16[i2 - 1];
                     const bool w1 = !fl1 && c1 >= 0.0f, w2 = !fl2 && c2 >= 0.0f;
                     const float h1 = w1 ? fmaxf(c1, 0.1f) : 1.0f, h2 = w2 ? fmaxf(c2, 0.1f) : 1.0f;
@@ -891,7 +927,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                 stage[slt[io * BC + L]] = (T)f[i];
             });
         wave_sync();
-        stage_copy<PAR, true, !BND>(F, stage, tbl[wave], rec, base, L);
+        stage_copy<PAR, true, !BND>(F, stage, nb, base, L);
     }
 }
 
