@@ -314,8 +314,15 @@ __device__ __forceinline__ int face_brick(int x, int y, int z, const int (&nb)[2
 // Even steps: the whole own block. Odd steps (interior bricks): the own group as 16-byte vectors, then the 26
 // neighbour records packed lane by lane (nb: the brick's neighbour ids). Loads are only issued here: the caller
 // waits for them (vmcnt) before the stage is read.
+// Record addresses per lane of an odd step's staged footprint, one per access instruction: FP32 12 rows of 4-byte
+// slots; FP16 6 rows of 4-byte slot pairs and the row of corner slots. The load pass computes them (ra), the store
+// pass reuses them.
+template <typename T>
+constexpr int kRecAddrs = sizeof(T) == 2 ? (kNPairs + BC - 1) / BC + 1 : (kRecs + BC - 1) / BC;
+
 template <int PAR, bool STORE, bool DMA, typename T>
-__device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int (&nb)[27], uint32_t base, int L) {
+__device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int (&nb)[27], uint32_t base, int L,
+                                           uint32_t (&ra)[kRecAddrs<T>]) {
     if constexpr (PAR == 0) {
         // the whole block as 16-byte vectors (a block is Q * BC * sizeof(T) bytes, a multiple of 16)
         constexpr int NV = Q * BC * (int)sizeof(T) / 16;
@@ -354,14 +361,17 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
                 constexpr int j = decltype(jc)::value;
                 const int p = j * BC + L;
                 if ((j + 1) * BC <= kNPairs || p < kNPairs) {
-                    uint32_t* g = reinterpret_cast<uint32_t*>(F + record_addr<2 * j * BC, 2 * (j * BC + BC - 1)>(2 * p, nb));
+                    if constexpr (!STORE) ra[j] = record_addr<2 * j * BC, 2 * (j * BC + BC - 1)>(2 * p, nb);
+                    uint32_t* g = reinterpret_cast<uint32_t*>(F + ra[j]);
                     if constexpr (STORE) *g = lp[p];
                     else if constexpr (DMA) load_to_lds<4>(g, lp + j * BC);
                     else lp[p] = *g;
                 }
             });
+            constexpr int jc = kRecAddrs<T> - 1;
             if (L < kNCorners) {
-                const uint32_t at = record_addr<2 * kNPairs, 2 * kNPairs + kNCorners - 1>(2 * kNPairs + L, nb);
+                if constexpr (!STORE) ra[jc] = record_addr<2 * kNPairs, 2 * kNPairs + kNCorners - 1>(2 * kNPairs + L, nb);
+                const uint32_t at = ra[jc];
                 if constexpr (STORE) F[at] = stage[kOwn + 2 * kNPairs + L];
                 else stage[kOwn + 2 * kNPairs + L] = F[at];
             }
@@ -370,7 +380,8 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
                 constexpr int j = decltype(jc)::value;
                 const int p = j * BC + L;
                 if ((j + 1) * BC <= kRecs || p < kRecs) {
-                    const uint32_t at = record_addr<j * BC, j * BC + BC - 1>(p, nb);
+                    if constexpr (!STORE) ra[j] = record_addr<j * BC, j * BC + BC - 1>(p, nb);
+                    const uint32_t at = ra[j];
                     if constexpr (STORE) F[at] = stage[kOwn + p];
                     else if constexpr (DMA) load_to_lds<4>(F + at, stage + kOwn + j * BC);
                     else stage[kOwn + p] = F[at];
@@ -587,6 +598,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     float gnb[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     float4 hm[(kHalo + BC - 1) / BC];
     uint32_t habs = 0u;
+    uint32_t ra[kRecAddrs<T>];  // odd interior steps: record addresses, computed for the loads, reused for the stores
     if (valid) {
         // the wave-uniform values first: scalar loads, issued before any direct-to-LDS copy (a store, after which
         // the compiler no longer takes a uniform load for a scalar one)
@@ -631,7 +643,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                 gnb[2 * ax] = V[(uint32_t)max(face_brick<-sx, -sy, -sz>(x, y, z, nb), 0) * BC + l1];
                 gnb[2 * ax + 1] = V[(uint32_t)max(face_brick<sx, sy, sz>(x, y, z, nb), 0) * BC + l2];
             });
-        if constexpr (STAGED) stage_copy<PAR, false, !BND>(F, stage, nb, base, L);
+        if constexpr (STAGED) stage_copy<PAR, false, !BND>(F, stage, nb, base, L, ra);
     }
     load_table<W>(slots32, ODD_STAGED ? kOddPos : kSlots, wave, L);
     // the slot table (every wave's part) and the stage have landed in LDS
@@ -927,7 +939,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                 stage[slt[io * BC + L]] = (T)f[i];
             });
         wave_sync();
-        stage_copy<PAR, true, !BND>(F, stage, nb, base, L);
+        stage_copy<PAR, true, !BND>(F, stage, nb, base, L, ra);
     }
 }
 
