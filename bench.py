@@ -179,14 +179,18 @@ def oracle_checks():
 
 def timed_run(s, steps, warmup):
     """`warmup` untimed steps, then `steps` steps timed on the wall clock as the product runs them: one captured
-    graph of the even part of `steps` (captured before the timed region; capture launches nothing) and an odd last
-    step launched eagerly. Returns seconds."""
+    graph of the even part of `steps` and an odd last step launched eagerly. The graph is captured before the
+    warm-up (capture launches nothing), so that the timed steps follow the warm-up without an idle gap. Returns
+    seconds."""
     import torch
-    if warmup:
-        s.run(warmup, chunk=max(2, warmup - warmup % 2))
-    chunk = steps - steps % 2
-    if chunk:
-        s.prepare(chunk)
+    if warmup % 2:
+        s.run(1)                                   # the odd warm-up step first: the graphs below keep the parity
+    chunk, wchunk = steps - steps % 2, warmup - warmup % 2
+    for c in (chunk, wchunk):
+        if c:
+            s.prepare(c)
+    if wchunk:
+        s.run(wchunk, chunk=wchunk)
     s.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -155,6 +155,9 @@ class ArterySolver(Solver):
         A.work, A.work_chunks = self.f_work.data_ptr(), self.f_chunks
         _native.call("cs_flux_bc", ctypes.byref(A), self.stream_handle)
 
+    def _stepped_state(self):
+        return [self.Qlast] if hasattr(self, "Qlast") else []
+
     def gather(self, idx, slot, n_probe):
         """Probe reduction on the newest fields: (sum of rho / 3 per probe, max |u| over the probe cells)."""
         import torch

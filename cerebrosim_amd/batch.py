@@ -723,26 +723,26 @@ class Batch:
     def variants(self):
         return launch_variants()
 
-    def autotune(self, steps=20, repeats=2):
-        """Time every launch variant on this batch (HIP events on the batch stream, `steps` eager steps each, best of
-        `repeats`), keep the fastest, and restore the state the timing runs changed. Returns {variant: ms/step}."""
+    def autotune(self, steps=20, repeats=3):
+        """Time every launch variant on this batch as run() steps it (HIP events on the batch stream around replays
+        of a captured graph of `steps` steps, made even: one untimed replay, then the best of `repeats`), keep the
+        fastest, and restore the state the timing runs changed. Returns {variant: ms/step}."""
         torch = _torch()
         self._flush()
         snap = self._snapshot()
+        counters = (self.cur, self.steps, {cid: c.steps for cid, c in self.case_by_id.items()})
+        steps = max(2, int(steps) + int(steps) % 2)
         times = {}
         try:
             for var in self.variants():
                 self.variant = var["variant"]
+                self.run(steps, chunk=steps)
                 best = None
                 for _ in range(repeats):
                     e0 = torch.cuda.Event(enable_timing=True)
                     e1 = torch.cuda.Event(enable_timing=True)
                     e0.record(self._tstream)
-                    a = self.cur
-                    for _ in range(steps):
-                        self._clock()
-                        self._launch(a)
-                        a = 1 - a
+                    self.run(steps, chunk=steps)
                     e1.record(self._tstream)
                     self.synchronize()
                     ms = e0.elapsed_time(e1) / steps
@@ -750,6 +750,9 @@ class Batch:
                 times[var["variant"]] = best
         finally:
             self._restore(snap)
+            self.cur, self.steps = counters[0], counters[1]
+            for cid, n in counters[2].items():
+                self.case_by_id[cid].steps = n
         self.variant = min(times, key=times.get)
         self.graphs = {}
         return times

@@ -593,39 +593,41 @@ class Solver:
     def synchronize(self):
         self._tstream.synchronize()
 
-    def autotune(self, steps=20, repeats=2):
+    def autotune(self, steps=20, repeats=3):
         """Time each launch variant of the brick kernels (workgroup size, XCD order) on this lattice and keep the
-        fastest; the timing steps are undone (state restored). Returns {variant: ms per step}. All variants give the
-        same results."""
+        fastest; the timing steps are undone (state restored). Each variant is timed as run() steps it: replays of a
+        captured graph of `steps` steps (made even; one untimed replay first, then the best of `repeats`). Returns
+        {variant: ms per step}. All variants give the same results."""
         if self.layout != "bricks":
             return {}
         from .batch import launch_variants
         torch = _torch()
         b = self.backend
         # the snapshot and the restore run on the solver stream, after the work already queued there
-        state = [b.f, b.side] + b.mom.tensors() + [self.tick, self.p_out]
+        state = [b.f, b.side] + b.mom.tensors() + [self.tick, self.p_out] + self._stepped_state()
         self.synchronize()
         with torch.cuda.stream(self._tstream):
             saved = [t.clone() for t in state]
         cur, nsteps = self.cur, self.steps
+        steps = max(2, int(steps) + int(steps) % 2)
         times = {}
         for v in range(len(launch_variants())):
+            self._invalidate()
             self.variant = v
+            g = self.prepare(steps)
+            g.launch(self.stream_handle)
             best = None
             for _ in range(repeats):
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record(self._tstream)
-                a = self.cur
-                for _ in range(steps):
-                    self._clock()
-                    self._launch(a)
-                    a = 1 - a
+                g.launch(self.stream_handle)
                 e1.record(self._tstream)
                 self.synchronize()
                 ms = e0.elapsed_time(e1) / steps
                 best = ms if best is None else min(best, ms)
             times[v] = best
-        self.synchronize()
+            del g
+        self._invalidate()
         with torch.cuda.stream(self._tstream):
             for dst, src in zip(state, saved):
                 dst.copy_(src)
@@ -635,6 +637,11 @@ class Solver:
         self.variant = min(times, key=times.get)
         self.graphs = {}
         return times
+
+    def _stepped_state(self):
+        """Device state beyond the populations, moments, clock and opening pressures that steps change (subclasses
+        add theirs); autotune restores it."""
+        return []
 
     # --- outputs (lbm.py:450-455)
     def fields(self):

@@ -21,10 +21,10 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("layout", LAYOUTS)
-def test_artery_solver_matches_oracle(layout):
+def tilted_outlet_pipe():
+    """A tilted pipe with two pressure openings, its flow slabs and the resistance coupling (and the grid offset A
+    and the axis, for probes)."""
     from cerebrosim_amd.geometry import OFFSET
-    # two pressure openings on a tilted pipe
     axis = np.array([1.0, 2.0, 2.0]) / 3.0
     Lp, R = 36.0, 5.0
     base, _, _, _ = open_pipe(axis, Lp, lambda s: np.full_like(np.asarray(s, float), R))
@@ -43,6 +43,12 @@ def test_artery_solver_matches_oracle(layout):
     assert lat["n_outlets"] == 2 and lat["counts"]["inlet"] == 0
     flux = artery.flux_slabs(lat, openings, 1.0, slab=(3.0, 5.0))
     R_lu, pref, alpha = [0.0, 0.01], 0.002, 0.05        # stable explicit coupling (R Q of the order of pref)
+    return lat, flux, R_lu, pref, alpha, A, axis
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_artery_solver_matches_oracle(layout):
+    lat, flux, R_lu, pref, alpha, A, axis = tilted_outlet_pipe()
     coef = [0.004, 0.001, 0.0005]
     nu = 0.02
     s = artery.ArterySolver(lat, nu, flux, R_lu, pref, alpha, model="regularised", layout=layout)
@@ -81,3 +87,35 @@ def test_artery_solver_matches_oracle(layout):
     expect = np.array([(o.rho[gidx[gslot == k]] / 3.0).astype(np.float64).sum() for k in range(3)])
     assert np.allclose(out, expect, rtol=1e-4, atol=1e-9)
     assert abs(umax - np.linalg.norm(uo[gidx], axis=1).max()) <= 1e-5 * umax
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_autotune_restores_the_state(storage):
+    """ArterySolver.autotune times every launch variant and then puts back what its timing steps changed: the
+    populations, the moments, the clock, the opening pressures and the last flows are bitwise those before it, and
+    the steps after it give the bits of a run that never autotuned."""
+    import torch
+    lat, flux, R_lu, pref, alpha, _, _ = tilted_outlet_pipe()
+    runs = []
+    for tune in (False, True):
+        s = artery.ArterySolver(lat, 0.02, flux, R_lu, pref, alpha, model="regularised", layout="bricks",
+                                storage=storage)
+        s.set_outlet_pressure([0.0, -pref])
+        s.set_drive([0.004, 0.001, 0.0005], period=40, ramp=10)
+        s.run(21, chunk=10)
+        s.synchronize()
+        before = (s.get_f(), s.opening_pressures(), s.flows(), s.tick.cpu().numpy().copy(), s.cur, s.steps)
+        if tune:
+            times = s.autotune(steps=4, repeats=1)
+            assert set(times) and s.variant in times
+            after = (s.get_f(), s.opening_pressures(), s.flows(), s.tick.cpu().numpy().copy(), s.cur, s.steps)
+            for x, y in zip(before[:4], after[:4]):
+                assert np.array_equal(x, y)
+            assert before[4:] == after[4:]
+        s.run(30, chunk=10)
+        s.synchronize()
+        runs.append((s.get_f(), s.opening_pressures(), s.flows()))
+        del s
+        torch.cuda.empty_cache()
+    for x, y in zip(*runs):
+        assert np.array_equal(x, y)
