@@ -254,6 +254,23 @@ static_assert([] {
 }(), "record runs: base = (o << 11) + off");
 __constant__ SlotTable kOddPos = make_oddpos();
 
+// Odd steps of boundary bricks access global memory directly. pullsrc[i][l] = (d << 11) | block offset: where lane l
+// pulls direction i from, population opp(i) of the cell l - c_i, in the brick at neighbour offset d (13: the brick
+// itself). The same entry for direction opp(i) is where lane l pushes P_i (slot i of the cell l + c_i).
+constexpr SlotTable make_pullsrc() {
+    SlotTable t{};
+    constexpr SlotTable sl = make_slots();
+    for (int i = 0; i < Q; ++i)
+        for (int l = 0; l < BC; ++l) {
+            const int x = (l & 3) - Lat::cx[i], y = ((l >> 2) & 3) - Lat::cy[i], z = (l >> 4) - Lat::cz[i];
+            const int ml = (x & 3) + 4 * (y & 3) + 16 * (z & 3);
+            t.p[i][l] = (uint16_t)((offset_index(x, y, z) << 11) | sl.p[Lat::opp[i]][ml]);
+        }
+    return t;
+}
+static_assert(Q * BC <= 2048, "block offsets fit 11 bits");
+__constant__ SlotTable kPullSrc = make_pullsrc();
+
 // Float16 odd step: the face and edge records as 4-byte pairs (staged positions kOwn + 2p, 2p + 1, one record and
 // an even block offset), then the 8 corner slots one by one (staged positions kOwn + 2 kNPairs + c).
 constexpr int kNPairs = (6 * 100 + 12 * 10) / 2;
@@ -268,6 +285,15 @@ constexpr bool pairs_valid() {
 }
 static_assert(pairs_valid(), "face and edge records start at even offsets and pair up inside one record");
 static_assert(kOwn % 8 == 0, "the staged records start 16-byte aligned in float16");
+
+// c ? a : b as one v_cndmask on the wave mask of c. The compiler turns a select whose operand needs an LDS read into
+// a divergent branch around that read; boundary bricks want both addresses computed and one access issued.
+__device__ __forceinline__ uint32_t select_u32(bool c, uint32_t a, uint32_t b) {
+    uint32_t r;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(c);
+    asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
+    return r;
+}
 
 // Cross-lane LDS hand-over inside one wavefront.
 __device__ __forceinline__ void wave_sync() {
@@ -538,6 +564,8 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     constexpr bool HOIST_GRAD = !H16 && GAL && (PAR == 0 || !BND);
     __shared__ int tbl[LDS_TBL ? WAVES : 1][32];
     __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
+    // odd boundary steps: the pull-source table beside the slot table
+    __shared__ __align__(16) uint32_t pull32[BND && PAR == 1 ? Q * BC / 2 : 4];
     // the odd-step record addresses are computed from compile-time runs, so that five workgroups of four waves fit
     // the LDS of a compute unit
     __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
@@ -646,6 +674,7 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         if constexpr (STAGED) stage_copy<PAR, false, !BND>(F, stage, nb, base, L, ra);
     }
     load_table<W>(slots32, ODD_STAGED ? kOddPos : kSlots, wave, L);
+    if constexpr (BND && PAR == 1) load_table<W>(pull32, kPullSrc, wave, L);
     // the slot table (every wave's part) and the stage have landed in LDS
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -662,18 +691,33 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     float f[Q];
     // value of direction i as stored -> float. FP16: a fluid link holds P_i(n - c_i), a non-fluid one the bounced
     // P_opp(i)(n); each is decoded with its writer's moments and its own direction.
+    // Boundary bricks select without branching: the writer's moments (the upstream cell's, or the own cell's for a
+    // missing link) and the sign of c.u (decoding with opp(i) flips c_i; w_opp(i) = w_i).
     auto value = [&](auto ic, T raw) -> float {
         constexpr int i = decltype(ic)::value;
         if constexpr (!H16) {
             return raw;
-        } else if constexpr (FUSED16) {
-            if (!BND || ((mask >> i) & 1u))
-                return h16_decode_g<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
-            return h16_decode_g<Lat::opp[i]>(raw, halo[halo_index<0, 0, 0>(x, y, z)]);
+        } else if constexpr (!BND || PAR == 0) {
+            // even steps keep the branch: selecting would issue all 27 halo reads up front (124 VGPRs in FP16)
+            if constexpr (FUSED16) {
+                if (!BND || ((mask >> i) & 1u))
+                    return h16_decode_g<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
+                return h16_decode_g<Lat::opp[i]>(raw, halo[halo_index<0, 0, 0>(x, y, z)]);
+            } else {
+                if (!BND || ((mask >> i) & 1u))
+                    return h16_decode<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
+                return h16_decode<Lat::opp[i]>(raw, halo[halo_index<0, 0, 0>(x, y, z)]);
+            }
         } else {
-            if (!BND || ((mask >> i) & 1u))
-                return h16_decode<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
-            return h16_decode<Lat::opp[i]>(raw, halo[halo_index<0, 0, 0>(x, y, z)]);
+            const bool fl = (mask >> i) & 1u;
+            const float4 m = halo[select_u32(fl, halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z),
+                                             halo_index<0, 0, 0>(x, y, z))];
+            const float c = cdot<i>(m.y, m.z, m.w);
+            const float cu = fl ? c : -c;
+            const float g = __builtin_fmaf((float)raw, 1.0f / kH16Scale,
+                                           __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), m.x));
+            if constexpr (FUSED16) return g;
+            else return Lat::w[i] * g;
         }
     };
     if constexpr (STAGED) {
@@ -684,19 +728,17 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
             });
     } else {
         if (live) {
+            // a fluid link pulls from its source slot (pullsrc), a missing one reads the cell's own slot i: the
+            // address is selected, so each direction is one load without a divergent branch
+            const uint16_t* pull = reinterpret_cast<const uint16_t*>(pull32);
             static_for<0, Q>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
-                constexpr int io = Lat::opp[i];
                 if constexpr (i == 0) {
                     f[i] = value(ic, F[base + slt[L]]);
                 } else {
-                    int dd, l;
-                    shift_pos<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z, dd, l);
-                    if (!BND || ((mask >> i) & 1u)) {
-                        f[i] = value(ic, F[brick_base(tbl[wave][dd]) + slt[io * BC + l]]);
-                    } else {
-                        f[i] = value(ic, F[base + slt[i * BC + L]]);
-                    }
+                    const uint32_t e = pull[i * BC + L];
+                    const uint32_t from = brick_base(tbl[wave][e >> 11]) + (e & 2047u);
+                    f[i] = value(ic, F[select_u32((mask >> i) & 1u, from, base + slt[i * BC + L])]);
                 }
             });
         }
@@ -913,17 +955,17 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
         if constexpr (!STAGED) {
             // odd: P_i(n) -> slot i of n + c_i (where the next even step of n + c_i reads it), or back into the own
             // slot opp(i) when n + c_i is not fluid
+            const uint16_t* pull = reinterpret_cast<const uint16_t*>(pull32);
             static_for<0, Q>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 constexpr int io = Lat::opp[i];
                 if constexpr (i == 0) {
                     F[base + slt[L]] = (T)f[i];
-                } else if (!BND || ((mask >> io) & 1u)) {
-                    int dd, l;
-                    shift_pos<Lat::cx[i], Lat::cy[i], Lat::cz[i]>(x, y, z, dd, l);
-                    F[brick_base(tbl[wave][dd]) + slt[i * BC + l]] = (T)f[i];
                 } else {
-                    F[base + slt[io * BC + L]] = (T)f[i];
+                    // slot i of n + c_i is where n + c_i pulls direction i from: pullsrc[opp i]
+                    const uint32_t e = pull[io * BC + L];
+                    const uint32_t to = brick_base(tbl[wave][e >> 11]) + (e & 2047u);
+                    F[select_u32((mask >> io) & 1u, to, base + slt[io * BC + L])] = (T)f[i];
                 }
             });
         }
