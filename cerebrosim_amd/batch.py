@@ -606,10 +606,11 @@ class Batch:
         if A.n_list:
             _native.call("cs_bricks_step", ctypes.byref(A), self.stream_handle)
 
-    def _step_launches(self, a):
-        """[(arena, boundary, args)] of the step launches of parity a, with the drive pointer of table row self._k.
-        The argument blocks are kept while the arenas, their list extents and the variant stay the same (graph
-        captures then cost little more than the launches themselves)."""
+    def _step_pairs(self, a):
+        """[(arena, boundary args, interior args)] of the step of parity a, with the drive pointer of table row
+        self._k: one paired launch per arena with bricks (cs_bricks_step_pair; an empty list is skipped there). The
+        argument blocks are kept while the arenas, their list extents and the variant stay the same (graph captures
+        then cost little more than the launches themselves)."""
         key = (self.topo, self.variant, tuple((id(ar), ar.n_interior, ar.n_boundary,
                                                ar.interior.data_ptr() if ar.interior is not None else 0,
                                                ar.boundary.data_ptr() if ar.boundary is not None else 0,
@@ -619,56 +620,48 @@ class Batch:
             self._args_key, self._args = key, {}
         got = self._args.get(a)
         if got is None:
-            got = [(ar, boundary, self.step_args(ar, a, boundary)) for ar in self.arenas for boundary in (False, True)]
-            got = self._args[a] = [x for x in got if x[2].n_list]
+            got = [(ar, self.step_args(ar, a, True), self.step_args(ar, a, False)) for ar in self.arenas]
+            got = self._args[a] = [x for x in got if x[1].n_list or x[2].n_list]
         drive = self._drive_ptr()
-        for _, _, A in got:
-            A.drive = drive
+        for _, Ab, Ai in got:
+            Ab.drive = Ai.drive = drive
         return got
 
     def _native_blocks(self):
-        """(argument blocks of both parities, blocks per parity, side-stream flags) for cs_bricks_run, with the drive
-        pointers at table row 0; kept while the step argument blocks and the drive table stay the same."""
+        """(argument blocks of both parities, pairs per parity) for cs_bricks_run_pairs, with the drive pointers at
+        table row 0; kept while the step argument blocks and the drive table stay the same."""
         self._k = 0
-        b0, b1 = self._step_launches(0), self._step_launches(1)
+        b0, b1 = self._step_pairs(0), self._step_pairs(1)
         key = (self._args_key, self.drive.data_ptr())
         if self._blocks is None or self._blocks[0] != key:
-            assert [(id(ar), bnd) for ar, bnd, _ in b0] == [(id(ar), bnd) for ar, bnd, _ in b1]
+            assert [id(ar) for ar, _, _ in b0] == [id(ar) for ar, _, _ in b1]
             n = len(b0)
-            arr = (_native.BricksStepArgs * max(1, 2 * n))(*[A for _, _, A in b0 + b1])
-            side = (ctypes.c_int32 * max(1, n))(*[int(bnd) for _, bnd, _ in b0])
-            self._blocks = (key, arr, n, side)
+            blocks = [A for _, Ab, Ai in b0 + b1 for A in (Ab, Ai)]
+            arr = (_native.BricksStepArgs * max(1, 4 * n))(*blocks)
+            self._blocks = (key, arr, n)
         return self._blocks[1:]
 
     def _steps(self, n, a):
-        """The launches of n steps from parity a, drives at table rows 0..n-1: one native call (cs_bricks_run) that
-        forks the boundary launches onto the side stream each step; the per-step Python launches when cases measure
-        flows (their launch follows each step)."""
+        """The launches of n steps from parity a, drives at table rows 0..n-1: one native call (cs_bricks_run_pairs:
+        one paired launch per arena and step); the per-step Python launches when cases measure flows (their launch
+        follows each step)."""
         if any(ar.flux is not None for ar in self.arenas):
             for k in range(n):
                 self._k = k
                 self._launch(a)
                 a = 1 - a
             return
-        arr, nargs, side = self._native_blocks()
-        if nargs:
-            _native.call("cs_bricks_run", arr, nargs, side, int(n), int(a), self.max_cases, self.stream_handle,
-                         self._fork.handle)
+        arr, npairs = self._native_blocks()
+        if npairs:
+            _native.call("cs_bricks_run_pairs", arr, npairs, int(n), int(a), self.max_cases, self.stream_handle)
         self._k = n - 1
 
     def _launch(self, a):
-        """The launches of one step of parity a (drives at table row self._k). The brick launches of a step are
-        independent (as in lbm._BricksBackend.step; arenas share nothing), so the boundary bricks run on a side
-        stream concurrently with the interior bricks; the flow launches read the step's velocities after the join."""
-        launches = self._step_launches(a)
-        forked = any(b for _, b, _ in launches) and not all(b for _, b, _ in launches)
-        if forked:
-            self._fork.fork()
-        for ar, boundary, A in launches:
-            st = self._fork.handle if forked and boundary else self.stream_handle
-            _native.call("cs_bricks_step", ctypes.byref(A), st)
-        if forked:
-            self._fork.join()
+        """The launches of one step of parity a (drives at table row self._k): one paired launch per arena (the
+        brick lists of a step are independent, as in lbm._BricksBackend.step; arenas share nothing), then the flow
+        launches, which read the step's velocities."""
+        for ar, Ab, Ai in self._step_pairs(a):
+            _native.call("cs_bricks_step_pair", ctypes.byref(Ab), ctypes.byref(Ai), self.stream_handle)
         for ar in self.arenas:
             if ar.flux is not None:
                 self._flux(ar, a)

@@ -77,7 +77,8 @@ __global__ __launch_bounds__(256) void k_convert(cs_bricks_convert_args a) {
 }  // namespace
 }  // namespace cs
 
-extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
+// Argument checks shared by the step entry points (the list may be empty).
+static int check_step(const cs_bricks_step_args* a) {
     CS_REQUIRE(a && a->f && a->nbr && a->list && a->bidx && a->om_cell && a->om_brick && a->p_out && a->drive &&
                    a->vel_old && a->vel,
                "cs_bricks_step: null argument");
@@ -94,11 +95,17 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
     CS_REQUIRE(a->n_slots > 0 && a->n_slots % cs::BC == 0, "cs_bricks_step: n_slots %lld", (long long)a->n_slots);
     CS_REQUIRE(a->n_slots * cs::Q < ((int64_t)1 << 32), "cs_bricks_step: %lld slots exceed 32-bit population offsets",
                (long long)a->n_slots);
-    if (a->n_list == 0) return CS_OK;
-    CS_REQUIRE(a->n_list > 0 && a->n_list < ((int64_t)1 << 31), "cs_bricks_step: n_list %lld",
+    CS_REQUIRE(a->n_list >= 0 && a->n_list < ((int64_t)1 << 31), "cs_bricks_step: n_list %lld",
                (long long)a->n_list);
     CS_REQUIRE(a->variant >= 0 && a->variant < cs::kVariants, "cs_bricks_step: variant %d not in [0, %d)", a->variant,
                cs::kVariants);
+    return CS_OK;
+}
+
+extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
+    const int rc = check_step(a);
+    if (rc != CS_OK) return rc;
+    if (a->n_list == 0) return CS_OK;
     const cs::Variant v = cs::kVariant[a->variant];
     hipStream_t st = cs::as_stream(stream);
     hipError_t e = v.waves == 4 ? cs::bricks_launch_w4(*a, st, v.xcd)
@@ -106,6 +113,54 @@ extern "C" int cs_bricks_step(const cs_bricks_step_args* a, void* stream) {
     if (e != hipSuccess) {
         cs::set_error("cs_bricks_step launch failed: %s", hipGetErrorString(e));
         return CS_E_HIP;
+    }
+    return CS_OK;
+}
+
+extern "C" int cs_bricks_step_pair(const cs_bricks_step_args* boundary, const cs_bricks_step_args* interior,
+                                   void* stream) {
+    CS_REQUIRE(boundary && interior, "cs_bricks_step_pair: null argument");
+    CS_REQUIRE(boundary->boundary == 1 && interior->boundary == 0,
+               "cs_bricks_step_pair: the first block must list boundary bricks, the second interior bricks");
+    if (boundary->n_list == 0) return cs_bricks_step(interior, stream);
+    if (interior->n_list == 0) return cs_bricks_step(boundary, stream);
+    int rc = check_step(boundary);
+    if (rc == CS_OK) rc = check_step(interior);
+    if (rc != CS_OK) return rc;
+    CS_REQUIRE(boundary->parity == interior->parity && boundary->storage == interior->storage &&
+                   boundary->model == interior->model && boundary->galilean == interior->galilean &&
+                   boundary->variant == interior->variant,
+               "cs_bricks_step_pair: the two blocks differ in parity, storage, model, galilean flag or variant");
+    const cs::Variant v = cs::kVariant[boundary->variant];
+    hipStream_t st = cs::as_stream(stream);
+    const hipError_t e = v.waves == 4 ? cs::bricks_launch_pair_w4(*boundary, *interior, st, v.xcd)
+                       : v.waves == 2 ? cs::bricks_launch_pair_w2(*boundary, *interior, st, v.xcd)
+                                      : cs::bricks_launch_pair_w1(*boundary, *interior, st, v.xcd);
+    if (e != hipSuccess) {
+        cs::set_error("cs_bricks_step_pair launch failed: %s", hipGetErrorString(e));
+        return CS_E_HIP;
+    }
+    return CS_OK;
+}
+
+extern "C" int cs_bricks_run_pairs(const cs_bricks_step_args* args, int32_t n_pairs, int64_t n_steps, int32_t parity0,
+                                   int64_t drive_stride, void* stream) {
+    CS_REQUIRE(args && stream, "cs_bricks_run_pairs: null argument");
+    CS_REQUIRE(n_pairs >= 0 && n_steps >= 0 && (parity0 == 0 || parity0 == 1) && drive_stride >= 0,
+               "cs_bricks_run_pairs: n_pairs %d, n_steps %lld, parity0 %d, drive_stride %lld", n_pairs,
+               (long long)n_steps, parity0, (long long)drive_stride);
+    for (int k = 0; k < 4 * n_pairs; ++k)
+        CS_REQUIRE(args[k].parity == (k < 2 * n_pairs ? 0 : 1), "cs_bricks_run_pairs: block %d has parity %d", k,
+                   args[k].parity);
+    for (int64_t s = 0; s < n_steps; ++s) {
+        const cs_bricks_step_args* blk = args + (size_t)((parity0 + s) & 1) * 2 * n_pairs;
+        for (int k = 0; k < n_pairs; ++k) {
+            cs_bricks_step_args ab = blk[2 * k], ai = blk[2 * k + 1];
+            ab.drive = blk[2 * k].drive + s * drive_stride;
+            ai.drive = blk[2 * k + 1].drive + s * drive_stride;
+            const int rc = cs_bricks_step_pair(&ab, &ai, stream);
+            if (rc != CS_OK) return rc;
+        }
     }
     return CS_OK;
 }

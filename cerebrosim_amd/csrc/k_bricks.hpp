@@ -538,12 +538,31 @@ __device__ __forceinline__ float link_value(float code, float fo, bool opp_fluid
     return -fo + 2.0f * wI * (3.0f * p_out[k] + 4.5f * cu * cu - 1.5f * upsq);
 }
 
-// One wavefront per brick, W bricks per workgroup. xcd: remap workgroup ids so that each XCD (which takes
-// workgroups round-robin) processes a contiguous run of the list.
+// LDS of one workgroup of the step (byte offsets, 16-byte aligned): the slot table; in odd boundary steps the
+// pull-source table; each wave's neighbour ids where lanes look them up; each wave's stage (staged steps), moment
+// halo (FP16) and link-run buffer (odd boundary steps).
+template <int W, bool BND, int PAR, bool H16>
+struct BrickLds {
+    static constexpr int kT = H16 ? 2 : 4;
+    static constexpr bool STAGED = PAR == 0 || !BND;
+    static constexpr bool TBL = H16 || (BND && PAR == 1);
+    static constexpr int up16(int n) { return (n + 15) & ~15; }
+    static constexpr int slots = 0;
+    static constexpr int pull = slots + Q * BC * 2;
+    static constexpr int tbl = pull + (BND && PAR == 1 ? Q * BC * 2 : 0);
+    static constexpr int stage = up16(tbl + (TBL ? W * 32 * 4 : 0));
+    static constexpr int halo = up16(stage + (STAGED ? W * Q * BC * kT : 0));
+    static constexpr int lbuf = up16(halo + (H16 ? W * kHalo * 16 : 0));
+    static constexpr int bytes = up16(lbuf + (BND && !STAGED ? W * kLinkLds * 4 : 0));
+};
+
+// One wavefront per brick, W bricks per workgroup; wg: the workgroup's index into the list (wg * W + wave is the
+// wave's entry); smem: the workgroup's LDS (BrickLds layout).
 template <int W, int MODEL, bool GAL, bool BND, int PAR, bool H16>
-__global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xcd) {
+__device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg, unsigned char* smem) {
     constexpr int WAVES = W;
     using T = std::conditional_t<H16, _Float16, float>;
+    using Lds = BrickLds<W, BND, PAR, H16>;
     // Even steps stage the brick's own block in LDS: vector loads and stores instead of 27 gathers that each touch
     // many lines of the block (in an even step no other wavefront touches the block). Odd steps of interior bricks
     // stage their footprint (own group + one record in each neighbour block: slots that no other wavefront
@@ -562,20 +581,17 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     // FP32 + GAL: the neighbour velocities of the gradient stencil are loaded with the populations, except in odd
     // boundary steps (their direct gathers hold more registers: holding these too would lower the occupancy)
     constexpr bool HOIST_GRAD = !H16 && GAL && (PAR == 0 || !BND);
-    __shared__ int tbl[LDS_TBL ? WAVES : 1][32];
-    __shared__ __align__(16) uint32_t slots32[Q * BC / 2];
-    // odd boundary steps: the pull-source table beside the slot table
-    __shared__ __align__(16) uint32_t pull32[BND && PAR == 1 ? Q * BC / 2 : 4];
+    static_assert(Lds::TBL == LDS_TBL && Lds::STAGED == STAGED, "LDS layout");
     // the odd-step record addresses are computed from compile-time runs, so that five workgroups of four waves fit
-    // the LDS of a compute unit
-    __shared__ __align__(16) T stage_all[STAGED ? WAVES * Q * BC : 1];
-    __shared__ float4 halo_all[H16 ? WAVES * kHalo : 1];
-    // odd boundary steps: each wave's copy of its brick's link and side runs (even steps use the free stage)
-    __shared__ float lbuf_all[BND && !STAGED ? WAVES * kLinkLds : 1];
+    // the LDS of a compute unit; odd boundary steps keep each wave's copy of its brick's link and side runs in lbuf
+    // (even steps use the free stage)
+    int (*tbl)[32] = reinterpret_cast<int (*)[32]>(smem + Lds::tbl);
+    uint32_t* slots32 = reinterpret_cast<uint32_t*>(smem + Lds::slots);
+    uint32_t* pull32 = reinterpret_cast<uint32_t*>(smem + Lds::pull);
+    T* stage_all = reinterpret_cast<T*>(smem + Lds::stage);
+    float4* halo_all = reinterpret_cast<float4*>(smem + Lds::halo);
+    float* lbuf_all = reinterpret_cast<float*>(smem + Lds::lbuf);
     const uint16_t* slt = reinterpret_cast<const uint16_t*>(slots32);
-    // XCD-aware order: the 8 XCDs take workgroups round-robin; give each XCD a contiguous run of bricks
-    const int per = gridDim.x >> 3;
-    const int wg = xcd ? (int)((blockIdx.x & 7) * per + (blockIdx.x >> 3)) : (int)blockIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int L = threadIdx.x & 63;
     const int64_t li = (int64_t)wg * WAVES + wave;
@@ -782,6 +798,10 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                 // entry (at most the end of its run: the arrays carry one element of padding) and ignores it.
                 const uint32_t miss = ~mask & (kAllLinks - 1u);
                 const uint32_t qo = lo & 0xFFFFu, so = lo >> 16;
+                // held in a scalar register: otherwise the flag is reloaded from the kernel arguments (and waited
+                // for, together with the outstanding LDS reads) at every link direction
+                int bz = a.bouzidi;
+                asm volatile("" : "+s"(bz));
                 const float upsq = up[0] * up[0] + up[1] * up[1] + up[2] * up[2];
                 float uind[3] = {0.0f, 0.0f, 0.0f};
                 if (mask & kInletBit) {
@@ -815,11 +835,11 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
                         constexpr float sw = FUSED16 ? 1.0f / Lat::w[i] : 1.0f;
                         if (!fi) {
                             const float ps = ((sm >> i) & 1u) ? pv[k] * sw : rj;
-                            vi = link_value<i, FUSED16>(qv[k], ri, fj, rj, ps, a.bouzidi, uind, up, upsq, p_out);
+                            vi = link_value<i, FUSED16>(qv[k], ri, fj, rj, ps, bz, uind, up, upsq, p_out);
                         }
                         if (!fj) {
                             const float ps = ((sm >> j) & 1u) ? pv[k + 1] * sw : ri;
-                            vj = link_value<j, FUSED16>(qv[k + 1], rj, fi, ri, ps, a.bouzidi, uind, up, upsq, p_out);
+                            vj = link_value<j, FUSED16>(qv[k + 1], rj, fi, ri, ps, bz, uind, up, upsq, p_out);
                         }
                         f[i] = vi;
                         f[j] = vj;
@@ -985,6 +1005,36 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
     }
 }
 
+// One list per launch. xcd: remap workgroup ids so that each XCD (which takes workgroups round-robin) processes a
+// contiguous run of the list.
+template <int W, int MODEL, bool GAL, bool BND, int PAR, bool H16>
+__global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xcd) {
+    __shared__ __align__(16) unsigned char smem[BrickLds<W, BND, PAR, H16>::bytes];
+    const int per = gridDim.x >> 3;
+    const int wg = xcd ? (int)((blockIdx.x & 7) * per + (blockIdx.x >> 3)) : (int)blockIdx.x;
+    brick_step<W, MODEL, GAL, BND, PAR, H16>(a, wg, smem);
+}
+
+// Both lists of a step in one launch (the two are independent within a step: every slot is read and then written by
+// one thread of the step). Workgroups [0, gb) take the boundary list, dispatched first since their waves live
+// longer, the rest the interior list. Both parts hold a multiple of 8 workgroups, so each keeps its XCD-aware order.
+// One launch per step instead of two on forked streams saves the fork and join between steps.
+template <int W, int MODEL, bool GAL, int PAR, bool H16>
+__global__ __launch_bounds__(64 * W) void k_bricks_pair(cs_bricks_step_args ab, cs_bricks_step_args ai, int gb,
+                                                        int xcd) {
+    constexpr int kB = BrickLds<W, true, PAR, H16>::bytes, kI = BrickLds<W, false, PAR, H16>::bytes;
+    __shared__ __align__(16) unsigned char smem[kB > kI ? kB : kI];
+    const int bid = (int)blockIdx.x;
+    if (bid < gb) {
+        const int wg = xcd ? (bid & 7) * (gb >> 3) + (bid >> 3) : bid;
+        brick_step<W, MODEL, GAL, true, PAR, H16>(ab, wg, smem);
+    } else {
+        const int g = bid - gb, gi = (int)gridDim.x - gb;
+        const int wg = xcd ? (g & 7) * (gi >> 3) + (g >> 3) : g;
+        brick_step<W, MODEL, GAL, false, PAR, H16>(ai, wg, smem);
+    }
+}
+
 template <int W, int MODEL, bool GAL, bool BND>
 hipError_t launch_par(const cs_bricks_step_args& a, hipStream_t s, int xcd) {
     const int64_t groups = (a.n_list + W - 1) / W;
@@ -1012,10 +1062,36 @@ hipError_t launch_bricks(const cs_bricks_step_args& a, hipStream_t s, int xcd) {
     return a.model == CS_MODEL_REGULARISED ? launch_model<W, 1>(a, s, xcd) : launch_model<W, 0>(a, s, xcd);
 }
 
+// The paired launch: both lists non-empty, same model, Galilean flag, parity and storage (checked by the caller).
+template <int W, int MODEL, bool GAL>
+hipError_t launch_pair_par(const cs_bricks_step_args& ab, const cs_bricks_step_args& ai, hipStream_t s, int xcd) {
+    const int64_t gb = ((ab.n_list + W - 1) / W + 7) / 8 * 8, gi = ((ai.n_list + W - 1) / W + 7) / 8 * 8;
+    const unsigned grid = (unsigned)(gb + gi);
+    const bool h = ab.storage == CS_STORAGE_FP16;
+    if (ab.parity == 0) {
+        if (h) k_bricks_pair<W, MODEL, GAL, 0, true><<<grid, 64 * W, 0, s>>>(ab, ai, (int)gb, xcd);
+        else k_bricks_pair<W, MODEL, GAL, 0, false><<<grid, 64 * W, 0, s>>>(ab, ai, (int)gb, xcd);
+    } else {
+        if (h) k_bricks_pair<W, MODEL, GAL, 1, true><<<grid, 64 * W, 0, s>>>(ab, ai, (int)gb, xcd);
+        else k_bricks_pair<W, MODEL, GAL, 1, false><<<grid, 64 * W, 0, s>>>(ab, ai, (int)gb, xcd);
+    }
+    return hipGetLastError();
+}
+
+template <int W>
+hipError_t launch_bricks_pair(const cs_bricks_step_args& ab, const cs_bricks_step_args& ai, hipStream_t s, int xcd) {
+    if (ab.model == CS_MODEL_REGULARISED)
+        return ab.galilean ? launch_pair_par<W, 1, true>(ab, ai, s, xcd) : launch_pair_par<W, 1, false>(ab, ai, s, xcd);
+    return ab.galilean ? launch_pair_par<W, 0, true>(ab, ai, s, xcd) : launch_pair_par<W, 0, false>(ab, ai, s, xcd);
+}
+
 }  // namespace
 
 // Launchers of the step for 1, 2 and 4 bricks (wavefronts) per workgroup (k_bricks_w1/2/4.hip).
 hipError_t bricks_launch_w1(const cs_bricks_step_args& a, hipStream_t s, int xcd);
 hipError_t bricks_launch_w2(const cs_bricks_step_args& a, hipStream_t s, int xcd);
 hipError_t bricks_launch_w4(const cs_bricks_step_args& a, hipStream_t s, int xcd);
+hipError_t bricks_launch_pair_w1(const cs_bricks_step_args& ab, const cs_bricks_step_args& ai, hipStream_t s, int xcd);
+hipError_t bricks_launch_pair_w2(const cs_bricks_step_args& ab, const cs_bricks_step_args& ai, hipStream_t s, int xcd);
+hipError_t bricks_launch_pair_w4(const cs_bricks_step_args& ab, const cs_bricks_step_args& ai, hipStream_t s, int xcd);
 }  // namespace cs

@@ -282,18 +282,17 @@ class _BricksBackend:
         return A
 
     def step(self, s, a):
-        """The step's two launches. They are independent: in the AA pattern every slot is read and then written by
-        one thread of the step (interior or boundary brick alike), the moments read are the previous step's, and
-        the Bouzidi side copies are a boundary cell's own. So the boundary bricks run on a side stream, concurrently
-        with the interior bricks (the boundary kernels are bound by instruction issue, the interior ones by HBM)."""
+        """The step's bricks in one launch (cs_bricks_step_pair): the boundary and interior lists are independent
+        within a step (in the AA pattern every slot is read and then written by one thread of the step, interior or
+        boundary brick alike; the moments read are the previous step's; the Bouzidi side copies are a boundary
+        cell's own), so one grid holds both, the boundary workgroups first (the boundary kernels are bound by
+        instruction issue and latency, the interior ones by HBM)."""
         if not (self.L["n_interior"] and self.L["n_boundary"]):
             for boundary in (False, True):
                 self.step_part(s, a, boundary)
             return
-        s._fork.fork()
-        self.step_part(s, a, False)
-        self.step_part(s, a, True, s._fork.handle)
-        s._fork.join()
+        Ab, Ai = self._args(s, a, True), self._args(s, a, False)
+        _native.call("cs_bricks_step_pair", ctypes.byref(Ab), ctypes.byref(Ai), s.stream_handle)
 
     def step_part(self, s, a, boundary, stream=None):
         """One of the step's two launches: interior bricks or boundary bricks (on the solver stream by default)."""

@@ -289,6 +289,18 @@ typedef struct cs_bricks_step_args {
                                    can hold its live entries only while records keep their places */
 } cs_bricks_step_args;
 int cs_bricks_step(const cs_bricks_step_args* a, void* stream);
+/* One step's boundary and interior lists in a single launch (replaces the two k_step launches of a step's bricks,
+ * lbm.py:427, for the brick layout): the boundary block (boundary = 1) and the interior block (boundary = 0) must
+ * agree in parity, storage, model, galilean flag and variant. Either list may be empty (then the other one runs
+ * alone through cs_bricks_step, and the empty block is not otherwise checked). Gives the bits of cs_bricks_step on
+ * each block; one launch instead of two forked ones. */
+int cs_bricks_step_pair(const cs_bricks_step_args* boundary, const cs_bricks_step_args* interior, void* stream);
+/* n_steps consecutive steps of n_pairs (boundary, interior) block pairs, one paired launch per pair and step, all on
+ * `stream` (the chunk runner of cs_bricks_step_pair). args holds 4 * n_pairs blocks: the pairs for parity 0
+ * (boundary, interior, boundary, ...), then those for parity 1; step s launches the pairs of parity
+ * (parity0 + s) & 1 with drive advanced by s * drive_stride floats. Works under stream capture. */
+int cs_bricks_run_pairs(const cs_bricks_step_args* args, int32_t n_pairs, int64_t n_steps, int32_t parity0,
+                        int64_t drive_stride, void* stream);
 /* The launches of n_steps consecutive steps of a set of brick lists, in one call (replaces the per-step Python loop
  * of a chunk, the launches lbm.py:422-428 makes per step). args holds 2 * n_args blocks: those for parity 0, then
  * those for parity 1; step s launches the blocks of parity (parity0 + s) & 1, in order, with drive advanced by
