@@ -166,7 +166,6 @@ SIGNATURES = {
     "cs_bricks_copy": [c_void_p, ctypes.c_char_p, c_void_p],
     "cs_bricks_free": [c_void_p],
     "cs_bricks_step": [ctypes.POINTER(BricksStepArgs), c_void_p],
-    "cs_bricks_run": [ctypes.POINTER(BricksStepArgs), c_i32, c_void_p, c_i64, c_i32, c_i64, c_void_p, c_void_p],
     "cs_bricks_step_pair": [ctypes.POINTER(BricksStepArgs), ctypes.POINTER(BricksStepArgs), c_void_p],
     "cs_bricks_run_pairs": [ctypes.POINTER(BricksStepArgs), c_i32, c_i64, c_i32, c_i64, c_void_p],
     "cs_bricks_slot_table": [c_void_p],

@@ -31,7 +31,7 @@ import time
 import numpy as np
 
 from . import _native
-from .lbm import BrickMoments, StreamFork, _require_gpu, sponge_tau
+from .lbm import BrickMoments, _require_gpu, sponge_tau
 
 BC = 64
 BLOCK = 27 * BC
@@ -297,7 +297,6 @@ class Batch:
         self.variant = int(variant)
         self._tstream = torch.cuda.Stream(device=self.dev)          # default priority
         self.stream_handle = ctypes.c_void_p(self._tstream.cuda_stream)
-        self._fork = StreamFork(self._tstream, self.dev)
         M, K = self.max_cases, self.max_open
         with torch.cuda.stream(self._tstream):
             z = lambda shape, d=torch.float32: torch.zeros(shape, dtype=d, device=self.dev)

@@ -165,59 +165,6 @@ extern "C" int cs_bricks_run_pairs(const cs_bricks_step_args* args, int32_t n_pa
     return CS_OK;
 }
 
-extern "C" int cs_bricks_run(const cs_bricks_step_args* args, int32_t n_args, const int32_t* on_side,
-                             int64_t n_steps, int32_t parity0, int64_t drive_stride, void* stream, void* side_stream) {
-    CS_REQUIRE(args && on_side && stream, "cs_bricks_run: null argument");
-    CS_REQUIRE(n_args >= 0 && n_steps >= 0 && (parity0 == 0 || parity0 == 1) && drive_stride >= 0,
-               "cs_bricks_run: n_args %d, n_steps %lld, parity0 %d, drive_stride %lld", n_args, (long long)n_steps,
-               parity0, (long long)drive_stride);
-    bool fork = false, main = false;
-    for (int k = 0; k < 2 * n_args; ++k) {
-        CS_REQUIRE(args[k].parity == (k < n_args ? 0 : 1), "cs_bricks_run: block %d has parity %d", k, args[k].parity);
-        fork |= on_side[k % n_args] != 0;
-        main |= on_side[k % n_args] == 0;
-    }
-    CS_REQUIRE(!fork || side_stream, "cs_bricks_run: blocks for the side stream but no side stream");
-    fork = fork && main;  // all blocks on one stream: no fork needed
-    hipStream_t st = cs::as_stream(stream), sd = fork ? cs::as_stream(side_stream) : st;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    if (fork) {
-        for (auto& e : ev) {
-            const hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
-            if (r != hipSuccess) {
-                for (auto& f : ev)
-                    if (f) (void)hipEventDestroy(f);
-                cs::set_error("cs_bricks_run: hipEventCreateWithFlags: %s", hipGetErrorString(r));
-                return CS_E_HIP;
-            }
-        }
-    }
-    int rc = CS_OK;
-    auto check = [&](hipError_t r, const char* what) {
-        if (r != hipSuccess && rc == CS_OK) {
-            cs::set_error("cs_bricks_run: %s: %s", what, hipGetErrorString(r));
-            rc = CS_E_HIP;
-        }
-        return rc == CS_OK;
-    };
-    for (int64_t s = 0; s < n_steps && rc == CS_OK; ++s) {
-        const cs_bricks_step_args* blk = args + (size_t)((parity0 + s) & 1) * n_args;
-        if (fork && !(check(hipEventRecord(ev[0], st), "fork record") &&
-                      check(hipStreamWaitEvent(sd, ev[0], 0), "fork wait")))
-            break;
-        for (int k = 0; k < n_args && rc == CS_OK; ++k) {
-            cs_bricks_step_args a = blk[k];
-            a.drive = blk[k].drive + s * drive_stride;
-            rc = cs_bricks_step(&a, on_side[k] ? (void*)sd : (void*)st);
-        }
-        if (rc == CS_OK && fork)
-            (void)(check(hipEventRecord(ev[1], sd), "join record") && check(hipStreamWaitEvent(st, ev[1], 0), "join wait"));
-    }
-    for (auto& e : ev)
-        if (e) (void)hipEventDestroy(e);
-    return rc;
-}
-
 static int convert(const cs_bricks_convert_args* a, void* stream, bool set) {
     CS_REQUIRE(a && a->f && a->nbr && a->bidx && a->slot_of_cell && a->f_ref && a->lmask && a->smask && a->side &&
                    a->sstart && a->loff,

@@ -188,29 +188,6 @@ class BrickMoments:
                 t[lo:hi].zero_()
 
 
-class StreamFork:
-    """A side stream for launches that may run concurrently with those of a main stream. fork() makes the side stream
-    wait for everything queued on the main stream so far, join() makes the main stream wait for everything queued on
-    the side stream. Both are event dependencies, so a graph captured from the main stream records them as edges and
-    its replays run the two branches concurrently. Default priority (both streams)."""
-
-    def __init__(self, main, dev):
-        torch = _torch()
-        self.main = main
-        self.side = torch.cuda.Stream(device=dev)
-        self.handle = ctypes.c_void_p(self.side.cuda_stream)
-
-    def fork(self):
-        e = _torch().cuda.Event()
-        e.record(self.main)
-        self.side.wait_event(e)
-
-    def join(self):
-        e = _torch().cuda.Event()
-        e.record(self.side)
-        self.main.wait_event(e)
-
-
 class _BricksBackend:
     """4^3 brick storage with AA in-place streaming (see csrc/k_bricks.hip): populations [bricks][27][64] once,
     moments per slot (BrickMoments). Two launches per step: the interior bricks (no boundary data) and the boundary
@@ -412,7 +389,6 @@ class Solver:
         dev = self.dev
         self._tstream = torch.cuda.Stream(device=dev)          # default priority
         self.stream_handle = ctypes.c_void_p(self._tstream.cuda_stream)
-        self._fork = StreamFork(self._tstream, dev)
         with torch.cuda.stream(self._tstream):
             self.p_out = torch.zeros(max(1, lat["n_outlets"]), dtype=torch.float32, device=dev)
             self.tick = torch.zeros(1, dtype=torch.int32, device=dev)

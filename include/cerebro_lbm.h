@@ -301,14 +301,6 @@ int cs_bricks_step_pair(const cs_bricks_step_args* boundary, const cs_bricks_ste
  * (parity0 + s) & 1 with drive advanced by s * drive_stride floats. Works under stream capture. */
 int cs_bricks_run_pairs(const cs_bricks_step_args* args, int32_t n_pairs, int64_t n_steps, int32_t parity0,
                         int64_t drive_stride, void* stream);
-/* The launches of n_steps consecutive steps of a set of brick lists, in one call (replaces the per-step Python loop
- * of a chunk, the launches lbm.py:422-428 makes per step). args holds 2 * n_args blocks: those for parity 0, then
- * those for parity 1; step s launches the blocks of parity (parity0 + s) & 1, in order, with drive advanced by
- * s * drive_stride floats. Blocks with on_side[k] != 0 run on side_stream: each step forks it from stream (an event
- * the side stream waits for) and joins it back, which is valid because the brick launches of one step are
- * independent (every slot is read and written by one thread of the step). Works under stream capture. */
-int cs_bricks_run(const cs_bricks_step_args* args, int32_t n_args, const int32_t* on_side, int64_t n_steps,
-                  int32_t parity0, int64_t drive_stride, void* stream, void* side_stream);
 /* Launch variant `variant`: wavefronts (bricks) per workgroup and XCD-aware workgroup order. Returns CS_OK, or the
  * number of variants when `variant` is out of range (no outputs written). */
 int cs_bricks_variant_info(int32_t variant, int32_t* waves_per_group, int32_t* xcd_order);
