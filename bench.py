@@ -66,25 +66,49 @@ def make_solver(lat, layout, storage="fp32"):
 
 
 def kernel_times_ms(s, steps=40):
-    """Mean duration of each launch of a step (HIP events recorded on the solver's stream around each launch of
-    eager steps; the step's launches run back to back as in the graph)."""
+    """Mean duration of the step's launch as the product makes it (the brick layout: one paired launch of the
+    boundary and interior lists, ``k_bricks_pair``), and of each list launched alone in sequence
+    (``k_bricks_interior``, ``k_bricks_boundary``): HIP events on the solver's stream around each launch of eager
+    steps, parities alternating. Returns {name: (ms, cells)}."""
     import torch
     st = s._tstream
     parts = s.backend.launch_parts()
     ev = {name: [] for name, _ in parts}
+    out = {}
+
+    def timed(fn):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        fn()
+        e1.record(st)
+        return e0, e1
+
+    if s.layout == "bricks" and len(parts) == 2 and all(c for _, c in parts):
+        pair = []
+        for _ in range(steps):
+            s._clock()
+            pair.append(timed(lambda: s.backend.step(s, s.cur)))
+            s.cur = 1 - s.cur
+            s.steps += 1
+        st.synchronize()
+        out["k_bricks_pair"] = (float(np.mean([a.elapsed_time(b) for a, b in pair])), s.N)
     for _ in range(steps):
         s._clock()
         for k, (name, _) in enumerate(parts):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(st)
-            s.backend.step_part(s, s.cur, k == 1)
-            e1.record(st)
-            ev[name].append((e0, e1))
+            ev[name].append(timed(lambda: s.backend.step_part(s, s.cur, k == 1)))
         s.cur = 1 - s.cur
         s.steps += 1
     st.synchronize()
-    return {name: (float(np.mean([a.elapsed_time(b) for a, b in ev[name]])), cells) for name, cells in parts}
+    out.update({name: (float(np.mean([a.elapsed_time(b) for a, b in ev[name]])), cells) for name, cells in parts})
+    return out
+
+
+def dominant_launch(kt):
+    """The product's launch of a step: the paired brick launch when there is one, else the longest launch."""
+    if "k_bricks_pair" in kt:
+        return "k_bricks_pair", kt["k_bricks_pair"]
+    return max(kt.items(), key=lambda kv: kv[1][0])
 
 
 def copy_bandwidth_gbs(nbytes=1 << 30, reps=10):
@@ -233,7 +257,7 @@ def form_precision(lat, steps, warmup):
     secs = timed_run(s, steps, warmup)
     ok = s.ok()
     kt = kernel_times_ms(s)
-    name, (k_ms, k_cells) = max(kt.items(), key=lambda kv: kv[1][0])
+    name, (k_ms, k_cells) = dominant_launch(kt)
     ach = BYTES_PER_LUP["fp16"] * k_cells / (k_ms * 1e-3) / 1e9
     variant = s.variant
     del s
@@ -411,23 +435,26 @@ def load_traffic(layout, storage="fp32", variant=0):
 
 
 def pmc_probe(layout, storage, variant=0):
-    """Short run for hardware-counter passes: the bench workload, 4 eager steps (2 even, 2 odd) with the interior and
-    boundary launches in sequence, in launch variant `variant`, and a 1 GiB cs_stream_copy whose counter readings
-    calibrate FETCH_SIZE / WRITE_SIZE."""
+    """Short run for hardware-counter passes: the bench workload in launch variant `variant`, 4 eager steps (2 even,
+    2 odd) as the product launches them (one paired launch per step), then 4 with the interior and boundary lists
+    launched alone one after the other, and a 1 GiB cs_stream_copy whose counter readings calibrate FETCH_SIZE /
+    WRITE_SIZE."""
     import ctypes
 
     import torch
     from cerebrosim_amd import _native
     s = make_solver(build_workload(), layout, storage)
     s.variant = int(variant)
-    # the step's launches one after the other on the solver stream (in a run they overlap on two streams, and a
-    # counter read around one dispatch would then include the other's traffic)
-    for _ in range(4):
-        s._clock()
-        for part in (False, True):
-            s.backend.step_part(s, s.cur, part)
-        s.cur = 1 - s.cur
-        s.steps += 1
+    for separate in (False, True):
+        for _ in range(4):
+            s._clock()
+            if separate:
+                for part in (False, True):
+                    s.backend.step_part(s, s.cur, part)
+            else:
+                s.backend.step(s, s.cur)
+            s.cur = 1 - s.cur
+            s.steps += 1
     s.synchronize()
     x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
@@ -437,6 +464,47 @@ def pmc_probe(layout, storage, variant=0):
                  ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
     torch.cuda.synchronize()
     print("pmc probe done", flush=True)
+
+
+def traffic_summary(fetch_csv, write_csv, storage, variant, out_path):
+    """HBM bytes per dispatch of each brick kernel from two rocprofv3 --pmc runs of --pmc-probe (FETCH_SIZE, then
+    WRITE_SIZE: KiB per dispatch), each scaled by what the same counter reads for the probe's 1 GiB copy kernel, merged
+    into the JSON summary out_path under `variant`. The dominant launch is the paired step kernel (mean of its even
+    and odd dispatches)."""
+    import csv
+    import re
+
+    def per_kernel(path, counter):
+        got = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"]
+            if "k_stream_copy" in name:
+                key = "copy"
+            elif "k_bricks" in name:
+                key = re.sub(r"^.*?(k_bricks(?:_pair)?<[^>]*>).*$", r"\1", name)
+            else:
+                continue
+            got.setdefault(key, []).append(float(r["Counter_Value"]) * 1024.0)
+        return {k: float(np.mean(v)) for k, v in got.items()}
+
+    rd, wr = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    sr, sw = float(1 << 30) / rd["copy"], float(1 << 30) / wr["copy"]
+    kernels = {k: {"read": int(round(rd[k] * sr)), "write": int(round(wr[k] * sw))} for k in rd if k != "copy"}
+    pair = [v["read"] + v["write"] for k, v in kernels.items() if k.startswith("k_bricks_pair")]
+    rec = json.load(open(out_path)) if os.path.exists(out_path) else {}
+    N = build_workload()["nbr"].shape[1]
+    rec.update({"kernel": "k_bricks_pair (both brick lists of a step in one launch), mean of even and odd dispatches",
+                "workload": "womersley_pipe_R40_L480", "storage": storage,
+                "algorithmic_bytes_per_launch": BYTES_PER_LUP[storage] * N})
+    rec.setdefault("bytes_per_launch_by_variant", {})[str(variant)] = int(round(np.mean(pair))) if pair else None
+    rec.setdefault("per_launch_bytes_by_variant", {})[str(variant)] = kernels
+    rec.setdefault("scale_by_variant", {})[str(variant)] = {"read": round(sr, 4), "write": round(sw, 4)}
+    with open(out_path, "w") as fh:
+        json.dump(rec, fh, indent=1)
+    print(json.dumps({"variant": variant, "bytes_per_launch": rec["bytes_per_launch_by_variant"][str(variant)],
+                      "kernels": kernels}))
 
 
 # the reference's published speed on the story artery (RTX 3090): solver.json:179-194, step3_report.md:58-64
@@ -475,6 +543,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the launch variant given by --variant")
     ap.add_argument("--variant", type=int, default=0, help="launch variant without autotuning (and of --pmc-probe)")
     ap.add_argument("--no-artery", action="store_true", help="skip the story-artery speed comparison")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the HBM traffic summary of --storage / --variant into profiles/ from two --pmc runs "
+                         "of --pmc-probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (the step of one lattice "
@@ -482,6 +553,11 @@ def main():
         sys.exit(2)
     if args.pmc_probe:
         pmc_probe(args.layout, args.storage, args.variant)
+        return
+    if args.traffic_from:
+        tag = "bricks" if args.storage == "fp32" else f"bricks_{args.storage}"
+        traffic_summary(*args.traffic_from, args.storage, args.variant,
+                        os.path.join(ROOT, "profiles", f"traffic_step_{tag}_womersley.json"))
         return
     import torch
     t0 = time.perf_counter()
@@ -496,7 +572,7 @@ def main():
         raise RuntimeError("solver diverged during the benchmark")
     mlups = N * args.steps / secs / 1e6
     kt = kernel_times_ms(s)
-    name, (k_ms, k_cells) = max(kt.items(), key=lambda kv: kv[1][0])   # dominant launch
+    name, (k_ms, k_cells) = dominant_launch(kt)   # the step's launch (both brick lists in one kernel)
     bpl = BYTES_PER_LUP[args.storage]
     achieved = bpl * k_cells / (k_ms * 1e-3) / 1e9
     traffic = load_traffic(s.layout, args.storage, s.variant)
