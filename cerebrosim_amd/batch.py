@@ -454,6 +454,20 @@ class Batch:
         self.topo += 1
         return a
 
+    def reserve(self):
+        """Open one arena of the default capacity now (a serving loop's start-up), so that the first admission does
+        not pay for allocating and clearing it. Nothing happens when an arena exists or the budget has no room."""
+        if self.arenas:
+            return
+        nb, nr = min(self.arena_bricks, MAX_BRICKS), self.arena_records
+        nl, ns = LINKS_PER_RECORD * nr, SIDES_PER_RECORD * nr
+        if arena_bytes(nb, nr, self.storage, nl, ns) > self.budget:
+            return
+        with _torch().cuda.stream(self._tstream):
+            self.arenas.append(Arena(self, nb, nr, nl, ns))
+        self.topo += 1
+        self.synchronize()
+
     def can_admit(self, lay):
         """Whether a case of this layout fits now (free case slot and room in an arena or the budget)."""
         if len(self.case_by_id) >= self.max_cases:
