@@ -264,6 +264,57 @@ def _pow2(n, lo=64):
     return c
 
 
+class _PinnedRing:
+    """A ring of pinned host memory for host-to-device copies queued on one stream: a region is reused once the copies
+    that last read it have completed (their events); arrays larger than the ring get a fresh pinned copy."""
+
+    def __init__(self, nbytes, stream):
+        torch = _torch()
+        self.stream = stream
+        self.buf = torch.empty(int(nbytes), dtype=torch.uint8, pin_memory=True)
+        self.np = self.buf.numpy()
+        self.pos, self.busy, self.pending = 0, collections.deque(), []
+
+    def stage(self, x):
+        torch = _torch()
+        x = np.ascontiguousarray(x)
+        n = (x.nbytes + 255) // 256 * 256
+        cap = self.buf.numel()
+        if n > cap:
+            return torch.from_numpy(x).pin_memory()
+        if self.pos + n > cap:
+            self.staged()                  # the regions of this pass get their event before the ring wraps
+            self.pos = 0
+        lo, hi = self.pos, self.pos + n
+        # the busy regions form a queue in the order their copies were queued on the stream, which is the order they
+        # complete in: copies already done leave the front; if a region still in flight overlaps the new one, waiting
+        # for the last such region's event completes it and every region before it
+        busy = self.busy
+        while busy and busy[0][2].query():
+            busy.popleft()
+        last = -1
+        for k, (a, b, _) in enumerate(busy):
+            if a < hi and lo < b:
+                last = k
+        if last >= 0:
+            busy[last][2].synchronize()
+            for _ in range(last + 1):
+                busy.popleft()
+        self.np[lo:lo + x.nbytes] = x.reshape(-1).view(np.uint8)
+        self.pos = hi
+        self.pending.append((lo, hi))
+        return self.buf[lo:lo + x.nbytes].view(torch.from_numpy(x[:0]).dtype).reshape(x.shape)
+
+    def staged(self):
+        """One event on the ring's stream for the regions staged since the last call (after their copies were
+        queued)."""
+        if self.pending:
+            ev = _torch().cuda.Event()
+            ev.record(self.stream)
+            self.busy.extend((a, b, ev) for a, b in self.pending)
+            self.pending = []
+
+
 class Batch:
     """A set of cases stepped together (see the module docstring).
 
@@ -272,11 +323,13 @@ class Batch:
     ``budget_bytes`` (default: the device's free memory less 2 GiB). ``staging_bytes``: the pinned host ring that
     uploads go through; the host waits only when it wraps onto copies still queued behind the steps in flight.
     ``result_bytes``: the pinned host pool that ``Case.read_async`` copies results into (a read that does not fit
-    takes a pinned allocation of its own)."""
+    takes a pinned allocation of its own). ``keep_arenas``: an arena left without cases stays allocated (a resident
+    server), instead of being freed after a synchronisation."""
 
     def __init__(self, storage="fp32", model="regularised", magic=3.0 / 16.0, bouzidi=True, galilean=None,
                  device=None, arena_bricks=65536, arena_records=16384, budget_bytes=None, max_cases=256,
-                 max_open=8, max_harmonics=16, variant=0, staging_bytes=64 << 20, result_bytes=256 << 20):
+                 max_open=8, max_harmonics=16, variant=0, staging_bytes=64 << 20, result_bytes=256 << 20,
+                 keep_arenas=False):
         torch = _torch()
         if storage not in STORAGES:
             raise ValueError(f"unknown storage {storage!r}")
@@ -295,6 +348,7 @@ class Batch:
         self.budget = int(budget_bytes)
         self.max_cases, self.max_open, self.max_harm = int(max_cases), int(max_open), int(max_harmonics)
         self.variant = int(variant)
+        self.keep_arenas = bool(keep_arenas)
         self._tstream = torch.cuda.Stream(device=self.dev)          # default priority
         self.stream_handle = ctypes.c_void_p(self._tstream.cuda_stream)
         M, K = self.max_cases, self.max_open
@@ -317,9 +371,8 @@ class Batch:
             self._work = z(_native.CS_REDUCE_WORK, torch.float64)
             self._red = z(2, torch.float64)
         self._tstream.synchronize()
-        self._ring = torch.empty(int(staging_bytes), dtype=torch.uint8, pin_memory=True)   # upload staging
-        self._ring_np = self._ring.numpy()
-        self._ring_pos, self._ring_busy, self._ring_pending = 0, collections.deque(), []
+        self._ring = _PinnedRing(int(staging_bytes), self._tstream)   # upload staging (batch stream)
+        self._cstream, self._bring, self._blob_bytes = None, None, int(staging_bytes)   # packed admissions
         # pinned pool of result reads, in 256-byte units, allocated at the first read (allocating pinned memory per
         # read can wait for the device)
         self._rpool, self._rpool_bytes = None, int(result_bytes)
@@ -336,44 +389,28 @@ class Batch:
         self._blocks = None
 
     def _stage(self, x):
-        """A pinned host tensor holding x: a region of the batch's staging ring, reused once the copies that last
-        read it have completed (their events), or a fresh pinned copy for arrays larger than the ring."""
-        torch = _torch()
-        x = np.ascontiguousarray(x)
-        n = (x.nbytes + 255) // 256 * 256
-        cap = self._ring.numel()
-        if n > cap:
-            return torch.from_numpy(x).pin_memory()
-        if self._ring_pos + n > cap:
-            self._staged()                 # the regions of this pass get their event before the ring wraps
-            self._ring_pos = 0
-        lo, hi = self._ring_pos, self._ring_pos + n
-        # the busy regions form a queue in the order their copies were queued on the stream, which is the order they
-        # complete in: copies already done leave the front; if a region still in flight overlaps the new one, waiting
-        # for the last such region's event completes it and every region before it
-        busy = self._ring_busy
-        while busy and busy[0][2].query():
-            busy.popleft()
-        last = -1
-        for k, (a, b, _) in enumerate(busy):
-            if a < hi and lo < b:
-                last = k
-        if last >= 0:
-            busy[last][2].synchronize()
-            for _ in range(last + 1):
-                busy.popleft()
-        self._ring_np[lo:lo + x.nbytes] = x.reshape(-1).view(np.uint8)
-        self._ring_pos = hi
-        self._ring_pending.append((lo, hi))
-        return self._ring[lo:lo + x.nbytes].view(torch.from_numpy(x[:0]).dtype).reshape(x.shape)
+        """A pinned host tensor holding x, in the batch's staging ring (copies on the batch stream)."""
+        return self._ring.stage(x)
 
     def _staged(self):
         """Record one event for the ring regions staged since the last call (after their copies were queued)."""
-        if self._ring_pending:
-            ev = _torch().cuda.Event()
-            ev.record(self._tstream)
-            self._ring_busy.extend((a, b, ev) for a, b in self._ring_pending)
-            self._ring_pending = []
+        self._ring.staged()
+
+    def _blob_to_device(self, buf):
+        """A new device tensor holding the uint8 buffer buf (a case's packed arrays), copied on the batch's copy
+        stream: the copy does not queue behind the chunks in flight (it writes fresh memory), so its staging region
+        frees as soon as it is done. The batch stream waits for it before any launch that follows."""
+        torch = _torch()
+        if self._cstream is None:
+            self._cstream = torch.cuda.Stream(device=self.dev)          # default priority
+            self._bring = _PinnedRing(self._blob_bytes, self._cstream)
+        src = self._bring.stage(buf)
+        with torch.cuda.stream(self._cstream):
+            out = src.to(self.dev, non_blocking=True)
+        self._bring.staged()
+        out.record_stream(self._tstream)
+        self._tstream.wait_stream(self._cstream)
+        return out
 
     def _result_buffer(self, shapes):
         """Pinned host float32 tensors of the given shapes for a read's copies, in one region of the result pool, and
@@ -478,11 +515,26 @@ class Batch:
                 a.release(lay, spot)
                 return True
         nb, nr, nl, ns = self._arena_shape(lay)
-        return self.nbytes() + arena_bytes(nb, nr, self.storage, nl, ns) <= self.budget
+        # arenas kept without cases (keep_arenas) give their room back when a new one is needed
+        held = sum(a.nbytes() for a in self.arenas if a.cases)
+        return held + arena_bytes(nb, nr, self.storage, nl, ns) <= self.budget
+
+    def _drop_empty_arenas(self):
+        """Free the arenas without cases (after a synchronisation); True when there was one."""
+        empty = [a for a in self.arenas if not a.cases]
+        if not empty:
+            return False
+        self.synchronize()
+        for a in empty:
+            self.arenas.remove(a)
+        self.topo += 1
+        return True
 
     # --- cases ----------------------------------------------------------------------------------------------
-    def add(self, lat, nu_lu, layout=None):
-        """Admit a lattice as a new case at rest (f~ = 0, u = 0); returns its ``Case`` handle. Raises BatchFull."""
+    def add(self, lat, nu_lu, layout=None, packed=None):
+        """Admit a lattice as a new case at rest (f~ = 0, u = 0); returns its ``Case`` handle. Raises BatchFull.
+        packed: the case's upload arrays from ``pack_admission`` (prepared off this thread), so that admitting costs
+        one staging copy and one host-to-device copy here."""
         from . import bricks
         lay = layout if layout is not None else bricks.build(lat)
         if len(self.case_by_id) >= self.max_cases:
@@ -496,13 +548,15 @@ class Batch:
                 break
         if spot is None:
             a = self._new_arena(lay)
+            if a is None and self._drop_empty_arenas():
+                a = self._new_arena(lay)
             if a is None:
                 raise BatchFull("HBM budget exhausted")
             p = a.place(lay)
             if p is None:
                 raise BatchFull("case does not fit a fresh arena")
             spot = (a, p)
-        c = Case(self, cid, lat, nu_lu, lay, spot[0], spot[1])
+        c = Case(self, cid, lat, nu_lu, lay, spot[0], spot[1], packed=packed)
         self.case_by_id[cid] = c
         spot[0].cases.append(c)
         self._topology_changed(spot[0])
@@ -522,7 +576,7 @@ class Batch:
         del self.case_by_id[case.cid]
         case.arena = None
         self._topology_changed(a)
-        if not a.cases:
+        if not a.cases and not self.keep_arenas:
             self.synchronize()
             self.arenas.remove(a)
             self.topo += 1
@@ -795,6 +849,63 @@ def sponge_slots(lat, lay, tau0, length, tau_max, kinds=("inlet", "outlet")):
     return slots, own
 
 
+class PackedArrays:
+    """Host arrays packed into one buffer (each at a 256-byte aligned offset) and the device views of its copy."""
+
+    def __init__(self, arrays):
+        self.offs, total = {}, 0
+        arrays = {k: np.ascontiguousarray(a) for k, a in arrays.items()}
+        for k, a in arrays.items():
+            self.offs[k] = (total, a.dtype, a.shape)
+            total += (a.nbytes + 255) // 256 * 256
+        self.buf = np.empty(max(total, 256), np.uint8)
+        for k, a in arrays.items():
+            o = self.offs[k][0]
+            self.buf[o:o + a.nbytes] = a.reshape(-1).view(np.uint8)
+
+    def __contains__(self, key):
+        return key in self.offs
+
+    def view(self, dev, key):
+        """The array `key` in the device copy `dev` of the buffer (a view: dtype and shape of the host array)."""
+        torch = _torch()
+        o, dt, shape = self.offs[key]
+        n = int(np.prod(shape)) * dt.itemsize
+        tdt = torch.from_numpy(np.empty(0, dt)).dtype
+        return dev[o:o + n].view(tdt).view(shape)
+
+
+def pack_admission(lay, rec=None):
+    """The arrays a case's admission uploads, in their device types and packed into one buffer: the layout's arrays
+    (Case) and the per-slot omegas of a recorded sponge (``SetupRecorder``). Runs on a serving worker thread, so that
+    the serving thread's admission is one staging copy, one host-to-device copy and device-side copies into place."""
+    nb, nr = int(lay["n_bricks"]), int(lay["n_boundary"])
+    a = {"slot": np.asarray(lay["slot_of_cell"], np.int32), "nbr": np.asarray(lay["nbr"], np.int32).reshape(-1),
+         "bidx": np.asarray(lay["bidx"], np.int32)}
+    if nr:
+        a["lmask"] = lay["lmask"].view(np.int32).reshape(-1)
+        a["smask"] = lay["smask"].view(np.int32).reshape(-1)
+        inl = np.flatnonzero((lay["lmask"] & np.uint32(1 << 27)).any(axis=1))
+        if len(inl):
+            a["inl_idx"] = inl.astype(np.int64)
+            a["inl_rows"] = np.asarray(lay["uin"], np.float32).reshape(nr, 3 * BC)[inl]
+        a["qstart"] = np.asarray(lay["qstart"][:-1], np.int32)
+        a["sstart"] = np.asarray(lay["sstart"][:-1], np.int32)
+        a["loff"] = lay["loff"].view(np.int32).reshape(-1)
+        a["ldir"] = lay["ldir"].view(np.int32).reshape(-1)
+        if int(lay["n_links"]):
+            a["qlink"] = np.asarray(lay["qlink"][:int(lay["n_links"])], np.float32)
+    for name, args, _ in (rec.calls if rec is not None else []):
+        if name == "set_omega_slots":
+            slots, own = args
+            rows = np.flatnonzero(own)
+            a["om_own"] = np.asarray(own, np.uint8)
+            if len(rows):
+                a["om_rows"] = rows.astype(np.int64)
+                a["om_data"] = np.asarray(slots, np.float32).reshape(nb, BC)[rows]
+    return PackedArrays(a)
+
+
 class SetupRecorder:
     """Stands in for a Case while a request is prepared off the serving thread: records the setter calls, and
     computes the sponge's per-slot omega there. ``replay(case)`` applies them to the admitted case."""
@@ -802,6 +913,7 @@ class SetupRecorder:
     def __init__(self, lat, nu_lu, lay):
         self.lat, self.lay, self.tau = lat, lay, 3.0 * nu_lu + 0.5
         self.calls = []
+        self.packed = None                 # pack_admission(lay, self), when prepared
 
     def set_drive(self, *a, **k):
         self.calls.append(("set_drive", a, k))
@@ -820,6 +932,8 @@ class SetupRecorder:
 
     def replay(self, case):
         for name, a, k in self.calls:
+            if name == "set_omega_slots" and case.packed_sponge:
+                continue                   # applied at admission from the packed arrays
             getattr(case, name)(*a, **k)
 
 
@@ -854,7 +968,7 @@ class Case:
     """One lattice inside a Batch, with the lbm.Solver surface (lbm.py:346-455) and the resistance-outlet flow
     measurement of simulate.py (ArterySolver)."""
 
-    def __init__(self, batch, cid, lat, nu_lu, lay, arena, spot):
+    def __init__(self, batch, cid, lat, nu_lu, lay, arena, spot, packed=None):
         torch = _torch()
         b0, r0, q0, s0 = spot
         self.batch, self.cid, self.lat, self.lay, self.arena = batch, cid, lat, lay, arena
@@ -868,37 +982,55 @@ class Case:
         # Every upload below is asynchronous on the batch stream: queued after the chunks in flight (which may still
         # step a case that left this range) and before the next launches. The layout's arrays go up as they are and
         # the arena offsets are added on the device (less host work per admission); one ring event covers them all.
-        up = lambda dst, x: batch._upload(dst, x, staged=False)
+        # With packed arrays (pack_admission, prepared on a worker thread) they go up as one buffer and are copied
+        # into place on the device; the device buffer stays with the case (its slot table is a view of it).
+        self.packed_sponge = packed is not None and "om_own" in packed
+        dev = batch._blob_to_device(packed.buf) if packed is not None else None
+        self._packed_dev = dev
+
+        def up(dst, x, key=None):
+            if dev is not None and key is not None:
+                with torch.cuda.stream(batch._tstream):
+                    dst.copy_(packed.view(dev, key))
+            else:
+                batch._upload(dst, x, staged=False)
+
         with torch.cuda.stream(batch._tstream):
             # slot of each cell (int32: the gathers index with it too)
-            self.slot32 = batch._to_device(lay["slot_of_cell"], staged=False)
+            self.slot32 = (packed.view(dev, "slot") if dev is not None
+                           else batch._to_device(lay["slot_of_cell"], staged=False))
             self.slot32.add_(b0 * BC)
             self.slot_dev = self.slot32
             nbr = arena.nbr[b0 * 27:(b0 + nb) * 27]
-            up(nbr, lay["nbr"].reshape(-1))
+            up(nbr, lay["nbr"].reshape(-1), "nbr")
             nbr.add_(nbr.ge(0).int().mul_(b0))                      # neighbour bricks; -1 (absent) stays
             bidx = arena.bidx[b0:b0 + nb]
-            up(bidx, lay["bidx"])
+            up(bidx, lay["bidx"], "bidx")
             bidx.add_(bidx.ge(0).int().mul_(r0))                    # boundary records; -1 (interior) stays
             arena.om_brick[b0:b0 + nb].zero_()
             arena.om[b0 * BC:(b0 + nb) * BC].zero_()
             arena.f[b0 * BLOCK:(b0 + nb) * BLOCK].zero_()
             arena.mom.zero(b0 * BC, (b0 + nb) * BC)
             if nr:
-                up(arena.lmask[r0 * BC:(r0 + nr) * BC], lay["lmask"].view(np.int32).reshape(-1))
-                up(arena.smask[r0 * BC:(r0 + nr) * BC], lay["smask"].view(np.int32).reshape(-1))
+                up(arena.lmask[r0 * BC:(r0 + nr) * BC], lay["lmask"].view(np.int32).reshape(-1), "lmask")
+                up(arena.smask[r0 * BC:(r0 + nr) * BC], lay["smask"].view(np.int32).reshape(-1), "smask")
                 # inlet profiles: only the records with inlet cells read theirs
-                inl = np.flatnonzero((lay["lmask"] & np.uint32(1 << 27)).any(axis=1))
-                batch._scatter_rows(arena.uin.view(-1, 3 * BC), inl + r0, lay["uin"].reshape(nr, 3 * BC)[inl])
+                if dev is not None:
+                    if "inl_idx" in packed:
+                        idx = packed.view(dev, "inl_idx").add_(r0)
+                        arena.uin.view(-1, 3 * BC).index_copy_(0, idx, packed.view(dev, "inl_rows"))
+                else:
+                    inl = np.flatnonzero((lay["lmask"] & np.uint32(1 << 27)).any(axis=1))
+                    batch._scatter_rows(arena.uin.view(-1, 3 * BC), inl + r0, lay["uin"].reshape(nr, 3 * BC)[inl])
                 qs, ss = arena.qstart[r0:r0 + nr], arena.sstart[r0:r0 + nr]
-                up(qs, lay["qstart"][:-1])
+                up(qs, lay["qstart"][:-1], "qstart")
                 qs.add_(q0)
-                up(ss, lay["sstart"][:-1])
+                up(ss, lay["sstart"][:-1], "sstart")
                 ss.add_(s0)
-                up(arena.loff[r0 * BC:(r0 + nr) * BC], lay["loff"].view(np.int32).reshape(-1))
-                up(arena.ldir[r0 * 2:(r0 + nr) * 2], lay["ldir"].view(np.int32).reshape(-1))
+                up(arena.loff[r0 * BC:(r0 + nr) * BC], lay["loff"].view(np.int32).reshape(-1), "loff")
+                up(arena.ldir[r0 * 2:(r0 + nr) * 2], lay["ldir"].view(np.int32).reshape(-1), "ldir")
                 if lay["n_links"]:
-                    up(arena.qlink[q0:q0 + int(lay["n_links"])], lay["qlink"][:int(lay["n_links"])])
+                    up(arena.qlink[q0:q0 + int(lay["n_links"])], lay["qlink"][:int(lay["n_links"])], "qlink")
                 if lay["n_side"]:
                     arena.side[s0:s0 + int(lay["n_side"])].zero_()
             # per-case table entries by fill kernels (an indexed assignment of a Python number copies it from host
@@ -910,6 +1042,12 @@ class Case:
             batch.force[c].zero_()
             batch.p_out[c].zero_()
             batch.q_last[c].zero_()
+            if self.packed_sponge:
+                # the recorded sponge: per-slot omegas of the flagged bricks and the flags
+                if "om_rows" in packed:
+                    rows = packed.view(dev, "om_rows").add_(b0)
+                    arena.om.view(-1, BC).index_copy_(0, rows, packed.view(dev, "om_data"))
+                arena.om_brick[b0:b0 + nb].copy_(packed.view(dev, "om_own"))
         batch._staged()
         self.set_drive([1.0])
 

@@ -38,11 +38,12 @@ def _summary(p, u, ok):
     return {"u_max": float(sp.max()), "u_mean": float(sp.mean()), "p_mean": float(p.mean()), "ok": bool(ok)}
 
 
-def _build(make_case, spec):
-    """Worker-thread part of a request: lattice, brick layout, and the setup with its host work done here (one
-    native builder thread per request: the pool spreads the requests over the CPUs)."""
+def _build(make_case, spec, threads=1):
+    """Worker-thread part of a request: lattice, brick layout, and the setup with its host work done here. threads:
+    native builder threads for its lattice (the pool spreads a busy queue over the CPUs, one thread per request; a
+    request that arrives to an idle pool takes the idle CPUs too)."""
     from .lattice import builder_threads
-    with builder_threads(1):
+    with builder_threads(threads):
         lat, nu, setup = make_case(spec)
     lay = bricks.build(lat)
     rec = B.SetupRecorder(lat, nu, lay)
@@ -51,6 +52,9 @@ def _build(make_case, spec):
     # the worker, not on the serving thread
     slim = {k: lat[k] for k in ("origin", "dx", "shape", "n_outlets", "counts", "openings") if k in lat}
     rec.lat = slim
+    # the admission's uploads, converted and packed here, so that admitting costs the serving thread one staging
+    # copy and one host-to-device copy
+    rec.packed = B.pack_admission(lay, rec)
     return slim, nu, rec, lay
 
 
@@ -60,7 +64,7 @@ class Server:
     storage: population storage of the cases ("fp16" or "fp32"); budget_bytes: HBM the arenas may take (default
     288 GB, capped by the device's free memory less 2 GiB); chunk: steps per graph launch; arena_bricks /
     arena_records: arena capacity (an arena grows to fit a larger case); workers: host threads building
-    lattices (default: the process's CPUs less one, at most 15); keep: "summary" or "fields" (also keep the final
+    lattices (default: the process's CPUs less one, at most 10); keep: "summary" or "fields" (also keep the final
     (p, u) of each request); staging_bytes: pinned host ring for the uploads of admitted cases; result_bytes: pinned
     host pool the finished cases' results are copied into; switch_interval: the interpreter's thread switch interval
     while serving (seconds)."""
@@ -78,9 +82,12 @@ class Server:
         self.chunk = int(chunk) + int(chunk) % 2
         self.batch = B.Batch(storage=storage, budget_bytes=budget, arena_bricks=arena_bricks,
                              arena_records=arena_records, max_cases=max_cases, variant=variant, device=device,
-                             staging_bytes=staging_bytes, result_bytes=result_bytes, **solver_kw)
-        # builder threads: one per CPU the process may use, less one for this loop, at most 15
-        self.workers = int(workers) if workers else max(1, min(15, len(os.sched_getaffinity(0)) - 1))
+                             staging_bytes=staging_bytes, result_bytes=result_bytes, keep_arenas=True,
+                             **solver_kw)
+        # builder threads: one per CPU the process may use, less one for this loop, at most 10. More threads hold the
+        # interpreter more often against this loop (96 requests, 16 CPUs: wall 6.17-6.26 s with 10 workers, 6.25-6.27 s
+        # with 6, 6.77-6.83 s with 15)
+        self.workers = int(workers) if workers else max(1, min(10, len(os.sched_getaffinity(0)) - 1))
         self.keep = keep
         self.switch_interval = float(switch_interval)
 
@@ -91,7 +98,7 @@ class Server:
         bt = self.batch
         bt.reserve()
         lat, nu, rec, lay = _build(make_case, spec)
-        case = bt.add(lat, nu, layout=lay)
+        case = bt.add(lat, nu, layout=lay, packed=rec.packed)
         rec.replay(case)
         bt.run(self.chunk, chunk=self.chunk)
         read = case.read_async()
@@ -125,7 +132,7 @@ class Server:
             (p, u), ok = rec.pop("read").result()
             reading.remove(rec)
             rec["done"] = time.perf_counter() - t0
-            rec["result"] = _summary(p, u, ok)
+            rec["result"] = pool.submit(_summary, p, u, ok)   # summarised on a worker, off this loop
             if self.keep == "fields":
                 rec["fields"] = (p, u)
             records.append(rec)
@@ -160,7 +167,7 @@ class Server:
 
         def admit():
             _, _, r, (lat, nu, setup, lay), t_built = built.pop(0)
-            case = bt.add(lat, nu, layout=lay)
+            case = bt.add(lat, nu, layout=lay, packed=setup.packed)
             setup.replay(case)
             target = -(-int(r["steps"]) // chunk) * chunk
             active[case.cid] = {"name": r["name"], "spec": r, "case": case, "target": target,
@@ -187,7 +194,8 @@ class Server:
                     now = time.perf_counter() - t0
                     while pending and pending[0]["arrival"] * time_scale <= now:
                         r = pending.popleft()
-                        building[pool.submit(_build, make_case, r)] = r
+                        threads = max(1, self.workers // (len(building) + 1))
+                        building[pool.submit(_build, make_case, r, threads)] = r
                     done_builds = [f for f in building if f.done()]
                     for fut in done_builds:
                         r = building.pop(fut)
@@ -228,6 +236,8 @@ class Server:
                     host["wait"] += time.perf_counter() - th
         finally:
             sys.setswitchinterval(switch)
+        for rec in records:
+            rec["result"] = rec["result"].result()
         wall = time.perf_counter() - t0
         bt.synchronize()
         gpu = sum(c[0].elapsed_time(c[1]) for c in chunks) * 1e-3

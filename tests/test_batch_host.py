@@ -64,3 +64,34 @@ def test_arena_bytes():
     fp16 = B.arena_bytes(10, 4, "fp16")
     assert fp32 - fp16 == 10 * (27 * 64 * 2 - 64 * 4)      # half the populations, one more density plane
     assert B.MAX_BRICKS * 27 * 64 < 2 ** 32
+
+
+def test_packed_admission_arrays_round_trip():
+    """pack_admission packs a layout's upload arrays (and a recorded sponge) into one buffer; each view of the buffer
+    gives back the array in its device type and shape (here on a CPU tensor standing in for the device copy)."""
+    import torch
+
+    from cerebrosim_amd import bricks
+    from cerebrosim_amd.geometry import open_pipe
+    lat, _, _, _ = open_pipe((0, 0, 1), 10, lambda s: np.full_like(np.asarray(s, float), 4.0))
+    lay = bricks.build(lat)
+    rec = B.SetupRecorder(lat, 0.05, lay)
+    rec.set_drive([0.01, 0.002, 0.0], period=40)
+    rec.set_sponge(4, 0.7, kinds=("outlet",))
+    packed = B.pack_admission(lay, rec)
+    dev = torch.from_numpy(packed.buf.copy())
+    got = lambda k: packed.view(dev, k).numpy()
+    assert np.array_equal(got("slot"), lay["slot_of_cell"].astype(np.int32))
+    assert np.array_equal(got("nbr"), lay["nbr"].reshape(-1).astype(np.int32))
+    assert np.array_equal(got("lmask").view(np.uint32), lay["lmask"].reshape(-1))
+    assert np.array_equal(got("qstart"), lay["qstart"][:-1].astype(np.int32))
+    assert np.array_equal(got("qlink"), lay["qlink"][:int(lay["n_links"])].astype(np.float32))
+    inl = np.flatnonzero((lay["lmask"] & np.uint32(1 << 27)).any(axis=1))
+    assert np.array_equal(got("inl_idx"), inl)
+    nr = int(lay["n_boundary"])
+    assert np.array_equal(got("inl_rows"), lay["uin"].reshape(nr, -1)[inl].astype(np.float32))
+    slots, own = rec.calls[-1][1]
+    rows = np.flatnonzero(own)
+    assert len(rows) and np.array_equal(got("om_rows"), rows)
+    assert np.array_equal(got("om_data"), slots.reshape(-1, 64)[rows]) and np.array_equal(got("om_own"), own)
+    assert all(o % 256 == 0 for o, _, _ in packed.offs.values())
