@@ -54,18 +54,42 @@ def open_pipe(axis, L, radius, pad=3):
     hi = ends.max(0) + ext
     n = np.ceil(hi - lo).astype(int) + 1
     A = -lo + OFFSET                                     # pipe start in grid coordinates
-    G = np.stack(np.meshgrid(*[np.arange(k) for k in n], indexing="ij"), -1).reshape(-1, 3) - A
-    s = G @ axis
-    rad = np.linalg.norm(G - s[:, None] * axis, axis=1)
-    phi = np.maximum(np.maximum(rad - radius(s), -s), s - L).reshape(n)
+    a = _grid_axis(axis)
+    if a is None:
+        G = np.stack(np.meshgrid(*[np.arange(k) for k in n], indexing="ij"), -1).reshape(-1, 3) - A
+        s = G @ axis
+        rad = np.linalg.norm(G - s[:, None] * axis, axis=1)
+        phi = np.maximum(np.maximum(rad - radius(s), -s), s - L).reshape(n)
+    else:
+        # along a grid axis the axial coordinate is that axis' node coordinate and the radial distance depends only
+        # on the other two, so both are computed on the grid's lines and planes and broadcast: the same floating
+        # point operations on the same values as the general form above (s = G . e_a = G_a exactly; the radial
+        # vector's axial component is exactly 0), so phi has the same bits, without the (cells, 3) temporaries
+        b, c = [k for k in range(3) if k != a]
+        s = np.arange(n[a]) - A[a]
+        gb, gc = np.arange(n[b]) - A[b], np.arange(n[c]) - A[c]
+        rad = np.sqrt(gb[:, None] * gb[:, None] + gc[None, :] * gc[None, :])          # (n_b, n_c)
+        phi = np.maximum(np.maximum(rad[:, :, None] - radius(s)[None, None, :], -s[None, None, :]),
+                         (s - L)[None, None, :])                                         # axes (b, c, a)
+        phi = np.ascontiguousarray(np.moveaxis(phi, 2, a) if a != 2 else phi)
     R0, R1 = float(radius(np.array([0.0]))[0]), float(radius(np.array([L]))[0])
     openings = [{"kind": "inlet", "centre": A, "normal": -axis, "radius": R0},
                 {"kind": "outlet", "centre": A + axis * L, "normal": axis, "radius": R1}]
     lat = build_lattice(phi, [0, 0, 0], 1.0, openings=openings)
     P = lat["coords"] - A
-    sc = P @ axis
-    rc = np.linalg.norm(P - sc[:, None] * axis, axis=1)
+    if a is None:
+        sc = P @ axis
+        rc = np.linalg.norm(P - sc[:, None] * axis, axis=1)
+    else:
+        sc = np.ascontiguousarray(P[:, a])
+        rc = np.sqrt(P[:, b] * P[:, b] + P[:, c] * P[:, c])
     return lat, sc, rc, axis
+
+
+def _grid_axis(axis):
+    """Index of the grid axis that the unit vector `axis` points along (+ direction only), else None."""
+    nz = np.flatnonzero(axis != 0.0)
+    return int(nz[0]) if len(nz) == 1 and axis[nz[0]] == 1.0 else None
 
 
 def plug_inlet(lat):

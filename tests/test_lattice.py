@@ -38,6 +38,25 @@ def test_open_pipe_counts(name, axis, cells, wall, inlet, outlet):
     assert lat["counts"] == {"wall": wall, "inlet": inlet, "outlet": [outlet]}
 
 
+@pytest.mark.parametrize("axis", [(0, 0, 1), (0, 1, 0), (1, 0, 0)])
+def test_open_pipe_grid_axis_path_equals_general_form(monkeypatch, axis):
+    """Pipes along a grid axis compute the signed distance on the grid's lines and planes: the lattice and the
+    axial / radial coordinates have the same bits as the general (cells, 3) form."""
+    import cerebrosim_amd.geometry as G
+
+    def radius(s):
+        s = np.asarray(s, float)
+        return 9.3 * (1 - 0.35 * (1 - np.cos(2 * np.pi * np.clip((s - 20) / 30, 0, 1))))
+
+    fast = G.open_pipe(axis, 90, radius)
+    monkeypatch.setattr(G, "_grid_axis", lambda a: None)
+    ref = G.open_pipe(axis, 90, radius)
+    for k in ("nbr", "q", "uin", "coords"):
+        assert np.array_equal(fast[0][k], ref[0][k]), k
+    assert fast[0]["counts"] == ref[0]["counts"]
+    assert np.array_equal(fast[1], ref[1]) and np.array_equal(fast[2], ref[2])
+
+
 def test_stenosis_cells():
     lat, _, _, D = stenosis_pipe(1.0e-4)                  # geometry/solver.json:120 (20 cells across)
     assert lat["nbr"].shape[1] == 144678
