@@ -386,14 +386,18 @@ def form_batching(full, steps=400):
     return out
 
 
-def form_serving(full, rate=None):
-    """Stenosis requests served with continuous batching (FP16) vs solved one after the other (FP32)."""
+def form_serving(full, rate=None, size=None):
+    """Stenosis requests served with continuous batching (FP16) vs solved one after the other (FP32). size:
+    "default" (96 requests, 0.2 / 0.1 mm grids, 500-step cycles), "mid" (32 requests at 8/s on all three grids, so
+    that 0.05 mm requests of about 5 M cells share the batch, 500-step cycles) or "full" (192 requests, all three
+    grids, 2000-step cycles)."""
     import torch
     from cerebrosim_amd import serving, workloads
-    n = 192 if full else 96
-    grids = (0.2e-3, 0.1e-3, 0.05e-3) if full else (0.2e-3, 0.1e-3)
-    beat = 2000 if full else 500
-    rate = rate or 32.0                              # arrivals faster than service: the queue builds up
+    size = size or ("full" if full else "default")
+    n = {"default": 96, "mid": 32, "full": 192}[size]
+    grids = (0.2e-3, 0.1e-3) if size == "default" else (0.2e-3, 0.1e-3, 0.05e-3)
+    beat = 2000 if size == "full" else 500
+    rate = rate or (8.0 if size == "mid" else 32.0)  # arrivals faster than service: the queue builds up
     reqs = workloads.request_specs(n, rate_per_s=rate, seed=2, beats=4, beat_steps=beat, grids=grids)
     # graph chunks of 400 steps: fewer launches and captures; measured faster than 100 and 200 on this stream
     srv = serving.Server(storage="fp16", chunk=400)
@@ -406,7 +410,9 @@ def form_serving(full, rate=None):
     done = np.sort([r["done"] for r in rep["records"]])
     a, b = done[len(done) // 4], done[(3 * len(done)) // 4]
     steady = ((3 * len(done)) // 4 - len(done) // 4) / max(b - a, 1e-9) * 3600.0
-    return {"workload": f"stenosis_requests_seed2_n{n}_{'full' if full else 'grids_0.2_0.1mm'}_beat{beat}",
+    tag = {"default": "grids_0.2_0.1mm", "mid": "grids_0.2_0.1_0.05mm", "full": "full"}[size]
+    cells = [r["cells"] for r in rep["records"]]
+    return {"workload": f"stenosis_requests_seed2_n{n}_{tag}_beat{beat}", "max_cells": int(max(cells)),
             "requests": n, "rate_per_s": rate, "storage": "fp16", "chunk": rep["chunk"],
             "cases_per_hour": round(rep["cases_per_hour"], 1), "p50_s": round(rep["p50_s"], 3),
             "p99_s": round(rep["p99_s"], 3), "wall_s": round(rep["wall_s"], 2), "mlups": round(rep["mlups"], 1),
@@ -538,7 +544,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc passes")
     ap.add_argument("--forms", default="precision,batching,serving",
-                    help="comma-separated chosen forms to measure after the headline ('none' to skip)")
+                    help="comma-separated chosen forms to measure after the headline ('none' to skip; "
+                         "'serving_mid' adds the mid-size request sample with 0.05 mm grids)")
     ap.add_argument("--full", action="store_true", help="full-size batching / serving workloads")
     ap.add_argument("--no-autotune", action="store_true", help="keep the launch variant given by --variant")
     ap.add_argument("--variant", type=int, default=0, help="launch variant without autotuning (and of --pmc-probe)")
@@ -612,6 +619,9 @@ def main():
         if "serving" in forms:
             out["forms"]["serving"] = form_serving(args.full)
             print(f"bench: serving form done", file=sys.stderr, flush=True)
+        if "serving_mid" in forms:
+            out["forms"]["serving_mid"] = form_serving(False, size="mid")
+            print(f"bench: mid-size serving sample done", file=sys.stderr, flush=True)
     checks = out.get("cpu_baseline", {}).get("checks")
     if checks and "forms" in out:
         if "precision" in out["forms"]:

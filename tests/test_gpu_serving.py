@@ -60,3 +60,38 @@ def test_budget_queues_requests():
     got = {r["name"]: r for r in rep["records"]}
     for s in seq["records"]:
         assert np.array_equal(got[s["name"]]["fields"][1], s["fields"][1])
+
+
+def test_fine_grid_request_shares_the_budget():
+    """A 0.05 mm request (about 5 M cells) arrives with 0.2 / 0.1 mm ones under a budget that holds it and one
+    0.1 mm request: the arena bytes of the cases in the batch never exceed the budget, every request finishes, and
+    each result is the same bits as the request solved alone (FP16 storage)."""
+    from cerebrosim_amd import batch as B
+    from cerebrosim_amd import bricks
+    reqs = workloads.request_specs(4, rate_per_s=50.0, seed=2, beats=1, beat_steps=40, grids=(0.2e-3, 0.1e-3))
+    fine = workloads.request_specs(1, rate_per_s=50.0, seed=5, beats=1, beat_steps=40, grids=(0.05e-3,))[0]
+    reqs.insert(1, dict(fine, name="fine"))
+    for r in reqs:
+        r["arrival"] = 0.0
+    print("requests:", [(r["name"], r["dx"], r["steps"]) for r in reqs], flush=True)
+    probe = B.Batch(storage="fp16", arena_bricks=1, arena_records=1)
+    need = {r["name"]: probe.arena_bytes_for(bricks.build(workloads.request_case(r)[0])) for r in reqs}
+    del probe
+    coarse = max(v for k, v in need.items() if k != "fine")
+    budget = int(need["fine"] + 1.1 * coarse)
+    srv = serving.Server(storage="fp16", chunk=100, arena_bricks=1, arena_records=1, budget_bytes=budget,
+                         keep="fields")
+    rep = srv.serve(reqs, workloads.request_case)
+    assert rep["cases"] == len(reqs)
+    recs = rep["records"]
+    assert max(r["cells"] for r in recs) > 4_000_000
+    for r in recs:
+        held = sum(need[x["name"]] for x in recs if x["admitted"] <= r["admitted"] < x["left"])
+        assert held <= budget, (r["name"], held, budget)
+    seq = serving.run_sequential(reqs, workloads.request_case, storage="fp16", chunk=100, keep="fields")
+    got = {r["name"]: r for r in recs}
+    for s in seq["records"]:
+        g = got[s["name"]]
+        assert g["steps"] == s["steps"] and g["result"]["ok"]
+        assert np.array_equal(g["fields"][1], s["fields"][1]) and np.array_equal(g["fields"][0], s["fields"][0]), \
+            s["name"]
