@@ -474,17 +474,37 @@ __device__ __forceinline__ float h16_decode_g(_Float16 h, const float4& m) {
 // of an interior brick exists) read as zero. Two parts: halo_fetch issues the loads into registers (absent: bit k
 // for chunk k), halo_put converts and stores them once they have arrived, so that other loads issue in between.
 constexpr int kHaloChunks = (kHalo + BC - 1) / BC;
+static_assert(kHaloChunks == 4, "four halo chunks: one 16-bit source word each, two per 32-bit word");
 
+// Where halo entry k * 64 + L comes from, per lane L and chunk k: neighbour offset o (0..26) | cell l << 5. Lanes
+// past the shell repeat its last entry (loaded, not stored). A compile-time table read with one 8-byte load per
+// lane, instead of the base-6 digits of the entry computed in every wave.
+struct HaloSrc {
+    uint16_t v[BC][kHaloChunks];
+};
+constexpr HaloSrc make_halo_src() {
+    HaloSrc t{};
+    for (int L = 0; L < BC; ++L)
+        for (int k = 0; k < kHaloChunks; ++k) {
+            const int e = k * BC + L < kHalo ? k * BC + L : kHalo - 1;
+            const int hx = e % 6, hy = (e / 6) % 6, hz = e / 36;
+            const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
+                          9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
+            const int l = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
+            t.v[L][k] = (uint16_t)(o | l << 5);
+        }
+    return t;
+}
+__constant__ HaloSrc kHaloSrc = make_halo_src();
+
+// src: the lane's row of kHaloSrc (chunks 0, 1 in .x, chunks 2, 3 in .y)
 template <bool BND>
 __device__ __forceinline__ void halo_fetch(float4 (&m)[kHaloChunks], uint32_t& absent, const int* tbl,
-                                           const float4* __restrict__ mom, int L) {
+                                           const float4* __restrict__ mom, uint2 src) {
     static_for<0, kHaloChunks>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const int e = min(k * BC + L, kHalo - 1);  // lanes past the shell re-read its last entry (not stored)
-        const int hx = e % 6, hy = (e / 6) % 6, hz = e / 36;
-        const int o = (hx == 0 ? 0 : (hx == 5 ? 2 : 1)) + 3 * (hy == 0 ? 0 : (hy == 5 ? 2 : 1)) +
-                      9 * (hz == 0 ? 0 : (hz == 5 ? 2 : 1));
-        const int l = ((hx + 3) & 3) + 4 * ((hy + 3) & 3) + 16 * ((hz + 3) & 3);
+        const uint32_t w = ((k < 2 ? src.x : src.y) >> (16 * (k & 1))) & 0xFFFFu;
+        const int o = (int)(w & 31u), l = (int)(w >> 5);
         const int nb = tbl[o];
         m[k] = mom[(uint32_t)(BND && nb < 0 ? 0 : nb) * BC + l];
         if (BND && nb < 0) absent |= 1u << k;
@@ -610,12 +630,13 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
         constexpr int k = decltype(kc)::value;
         nb[k] = a.nbr[(int64_t)b * 27 + k];
     });
+    // the lanes' halo sources (FP16): a load that depends on nothing, issued first
+    uint2 hsrc = make_uint2(0u, 0u);
+    if constexpr (H16) hsrc = reinterpret_cast<const uint2*>(&kHaloSrc.v[0][0])[L];
     if constexpr (LDS_TBL) {
-        int v = -1;
-        static_for<0, 27>([&](auto kc) {
-            constexpr int k = decltype(kc)::value;
-            v = L == k ? nb[k] : v;
-        });
+        // the same ids as a vector load of lanes 0..26 (the same cache lines): one load and one LDS write instead of
+        // a select of each lane's id among the 27 scalars
+        const int v = L < 27 ? a.nbr[(int64_t)b * 27 + L] : -1;
         if (L < 32) tbl[wave][L] = v;
         wave_sync();
     }
@@ -656,7 +677,7 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
             qs = a.qstart[bb];
             ss = a.sstart[bb];
         }
-        if constexpr (H16) halo_fetch<BND>(hm, habs, tbl[wave], reinterpret_cast<const float4*>(a.vel_old), L);
+        if constexpr (H16) halo_fetch<BND>(hm, habs, tbl[wave], reinterpret_cast<const float4*>(a.vel_old), hsrc);
         own_om = a.om_brick[b];
         if constexpr (BND) {
             mask = a.lmask[(uint32_t)bb * BC + L];
