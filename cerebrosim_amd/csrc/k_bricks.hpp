@@ -734,8 +734,7 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
         constexpr int i = decltype(ic)::value;
         if constexpr (!H16) {
             return raw;
-        } else if constexpr (!BND || PAR == 0) {
-            // even steps keep the branch: selecting would issue all 27 halo reads up front (124 VGPRs in FP16)
+        } else if constexpr (!BND) {
             if constexpr (FUSED16) {
                 if (!BND || ((mask >> i) & 1u))
                     return h16_decode_g<i>(raw, halo[halo_index<-Lat::cx[i], -Lat::cy[i], -Lat::cz[i]>(x, y, z)]);
