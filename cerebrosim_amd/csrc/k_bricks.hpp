@@ -1036,15 +1036,33 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
 }
 
 // Both lists of a step in one launch (the two are independent within a step: every slot is read and then written by
-// one thread of the step). Workgroups [0, gb) take the boundary list, dispatched first since their waves live
-// longer, the rest the interior list. Both parts hold a multiple of 8 workgroups, so each keeps its XCD-aware order.
-// One launch per step instead of two on forked streams saves the fork and join between steps.
+// one thread of the step). gb workgroups take the boundary list, the rest the interior list; both parts hold a
+// multiple of 8 workgroups, so each keeps its XCD-aware order. One launch per step instead of two on forked streams
+// saves the fork and join between steps.
+// Order: workgroups [0, gb) are the boundary ones, dispatched first since their waves live longer, except in FP32 odd
+// steps, where the boundary rounds of 8 workgroups (one per XCD) are spread evenly among the interior ones: their
+// direct gathers and scatters then overlap the interior bricks' streaming instead of running as a phase of their own
+// (paired odd launch 140.9 -> 136.5 us on the Womersley pipe; even steps and FP16 were slower that way).
 template <int W, int MODEL, bool GAL, int PAR, bool H16>
 __global__ __launch_bounds__(64 * W) void k_bricks_pair(cs_bricks_step_args ab, cs_bricks_step_args ai, int gb,
                                                         int xcd) {
     constexpr int kB = BrickLds<W, true, PAR, H16>::bytes, kI = BrickLds<W, false, PAR, H16>::bytes;
     __shared__ __align__(16) unsigned char smem[kB > kI ? kB : kI];
     const int bid = (int)blockIdx.x;
+    if constexpr (PAR == 1 && !H16) {
+        // round sb (8 workgroups, one per XCD) is a boundary round when floor(sb B8 / G8) steps up at sb: B8 of the
+        // G8 rounds, evenly spread; kb rounds before sb are boundary ones, sb - kb interior ones
+        const int sb = bid >> 3, x = bid & 7;
+        const int64_t G8 = (int64_t)(gridDim.x >> 3), B8 = gb >> 3;
+        const int kb = (int)(sb * B8 / G8), kb1 = (int)((sb + 1) * B8 / G8);
+        if (kb1 > kb) {
+            brick_step<W, MODEL, GAL, true, PAR, H16>(ab, xcd ? x * (int)B8 + kb : kb * 8 + x, smem);
+        } else {
+            const int ki = sb - kb, I8 = (int)(G8 - B8);
+            brick_step<W, MODEL, GAL, false, PAR, H16>(ai, xcd ? x * I8 + ki : ki * 8 + x, smem);
+        }
+        return;
+    }
     if (bid < gb) {
         const int wg = xcd ? (bid & 7) * (gb >> 3) + (bid >> 3) : bid;
         brick_step<W, MODEL, GAL, true, PAR, H16>(ab, wg, smem);
