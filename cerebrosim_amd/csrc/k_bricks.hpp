@@ -1039,26 +1039,31 @@ __global__ __launch_bounds__(64 * W) void k_bricks(cs_bricks_step_args a, int xc
 // one thread of the step). gb workgroups take the boundary list, the rest the interior list; both parts hold a
 // multiple of 8 workgroups, so each keeps its XCD-aware order. One launch per step instead of two on forked streams
 // saves the fork and join between steps.
-// Order: workgroups [0, gb) are the boundary ones, dispatched first since their waves live longer, except in FP32 odd
-// steps, where the boundary rounds of 8 workgroups (one per XCD) are spread evenly among the interior ones: their
-// direct gathers and scatters then overlap the interior bricks' streaming instead of running as a phase of their own
-// (paired odd launch 140.9 -> 136.5 us on the Womersley pipe; even steps and FP16 were slower that way).
+// Order, in rounds of 8 workgroups (one per XCD): the boundary rounds are spread evenly over the first kSpread %
+// of the grid's rounds, the interior rounds fill the rest (0: every boundary round first, since their waves live
+// longer). Spread, the boundary bricks' work overlaps the interior bricks' streaming instead of running as a phase
+// of its own. Measured on the Womersley pipe and the story artery (paired launch, mean of both parities, HIP events):
+// FP32 odd steps spread over 75 % (pipe 136.4 -> 134.9 us against spreading over the whole grid, 140.9 with the
+// boundary first; artery 15.1 -> 15.7 k MLUPS), FP16 even steps over 50 % (102.2 -> 101.3 us); FP32 even and FP16
+// odd steps are fastest boundary first.
 template <int W, int MODEL, bool GAL, int PAR, bool H16>
 __global__ __launch_bounds__(64 * W) void k_bricks_pair(cs_bricks_step_args ab, cs_bricks_step_args ai, int gb,
                                                         int xcd) {
     constexpr int kB = BrickLds<W, true, PAR, H16>::bytes, kI = BrickLds<W, false, PAR, H16>::bytes;
     __shared__ __align__(16) unsigned char smem[kB > kI ? kB : kI];
     const int bid = (int)blockIdx.x;
-    if constexpr (PAR == 1 && !H16) {
-        // round sb (8 workgroups, one per XCD) is a boundary round when floor(sb B8 / G8) steps up at sb: B8 of the
-        // G8 rounds, evenly spread; kb rounds before sb are boundary ones, sb - kb interior ones
+    constexpr int kSpread = H16 ? (PAR == 0 ? 50 : 0) : (PAR == 1 ? 75 : 0);
+    if constexpr (kSpread > 0) {
+        // round sb is a boundary round when floor(sb B8 / G8) steps up at sb: B8 of the first G8 rounds, evenly
+        // spread; kb rounds before sb are boundary ones, sb - kb interior ones
         const int sb = bid >> 3, x = bid & 7;
-        const int64_t G8 = (int64_t)(gridDim.x >> 3), B8 = gb >> 3;
-        const int kb = (int)(sb * B8 / G8), kb1 = (int)((sb + 1) * B8 / G8);
+        const int64_t B8 = gb >> 3;
+        const int64_t G8 = max(B8, (int64_t)(gridDim.x >> 3) * kSpread / 100);
+        const int kb = (int)min(B8, sb * B8 / G8), kb1 = (int)min(B8, (sb + 1) * B8 / G8);
         if (kb1 > kb) {
             brick_step<W, MODEL, GAL, true, PAR, H16>(ab, xcd ? x * (int)B8 + kb : kb * 8 + x, smem);
         } else {
-            const int ki = sb - kb, I8 = (int)(G8 - B8);
+            const int ki = sb - kb, I8 = (int)((int64_t)(gridDim.x >> 3) - B8);
             brick_step<W, MODEL, GAL, false, PAR, H16>(ai, xcd ? x * I8 + ki : ki * 8 + x, smem);
         }
         return;
