@@ -346,10 +346,13 @@ __device__ __forceinline__ int face_brick(int x, int y, int z, const int (&nb)[2
 template <typename T>
 constexpr int kRecAddrs = sizeof(T) == 2 ? (kNPairs + BC - 1) / BC + 1 : (kRecs + BC - 1) / BC;
 
-template <int PAR, bool STORE, bool DMA, typename T>
+// PART: 0 everything, 1 the brick's own block (even) or own group (odd) only, 2 the odd step's records only (the
+// parts that need the neighbour ids)
+template <int PAR, bool STORE, bool DMA, int PART = 0, typename T>
 __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const int (&nb)[27], uint32_t base, int L,
                                            uint32_t (&ra)[kRecAddrs<T>]) {
     if constexpr (PAR == 0) {
+        if constexpr (PART == 2) return;
         // the whole block as 16-byte vectors (a block is Q * BC * sizeof(T) bytes, a multiple of 16)
         constexpr int NV = Q * BC * (int)sizeof(T) / 16;
         uint4* g = reinterpret_cast<uint4*>(F + base);
@@ -371,16 +374,18 @@ __device__ __forceinline__ void stage_copy(T* __restrict__ F, T* stage, const in
         constexpr int NV = kOwn * (int)sizeof(T) / 16;
         uint4* g = reinterpret_cast<uint4*>(F + base);
         uint4* l = reinterpret_cast<uint4*>(stage);
-        static_for<0, (NV + BC - 1) / BC>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            const int k = j * BC + L;
-            if (NV % BC == 0 || (j + 1) * BC <= NV || k < NV) {
-                if constexpr (STORE) g[k] = l[k];
-                else if constexpr (DMA) load_to_lds<16>(g + k, l + j * BC);
-                else l[k] = g[k];
-            }
-        });
-        if constexpr (sizeof(T) == 2) {
+        if constexpr (PART != 2)
+            static_for<0, (NV + BC - 1) / BC>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const int k = j * BC + L;
+                if (NV % BC == 0 || (j + 1) * BC <= NV || k < NV) {
+                    if constexpr (STORE) g[k] = l[k];
+                    else if constexpr (DMA) load_to_lds<16>(g + k, l + j * BC);
+                    else l[k] = g[k];
+                }
+            });
+        if constexpr (PART == 1) {
+        } else if constexpr (sizeof(T) == 2) {
             // float16: face and edge records as 4-byte pairs (straight to LDS when DMA), corners one by one
             uint32_t* lp = reinterpret_cast<uint32_t*>(stage + kOwn);
             static_for<0, (kNPairs + BC - 1) / BC>([&](auto jc) {
@@ -601,6 +606,7 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
     // FP32 + GAL: the neighbour velocities of the gradient stencil are loaded with the populations, except in odd
     // boundary steps (their direct gathers hold more registers: holding these too would lower the occupancy)
     constexpr bool HOIST_GRAD = !H16 && GAL && (PAR == 0 || !BND);
+    constexpr bool EARLY_OWN = !BND && !H16;
     static_assert(Lds::TBL == LDS_TBL && Lds::STAGED == STAGED, "LDS layout");
     // the odd-step record addresses are computed from compile-time runs, so that five workgroups of four waves fit
     // the LDS of a compute unit; odd boundary steps keep each wave's copy of its brick's link and side runs in lbuf
@@ -677,6 +683,9 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
             qs = a.qstart[bb];
             ss = a.sstart[bb];
         }
+        // FP32 interior bricks: the brick's own part of the stage (the whole block in even steps, the own group in
+        // odd ones) needs no neighbour id, so it is in flight before the wait for the ids
+        if constexpr (EARLY_OWN) stage_copy<PAR, false, true, 1>(F, stage, nb, base, L, ra);
         if constexpr (H16) halo_fetch<BND>(hm, habs, tbl[wave], reinterpret_cast<const float4*>(a.vel_old), hsrc);
         own_om = a.om_brick[b];
         if constexpr (BND) {
@@ -708,7 +717,8 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
                 gnb[2 * ax] = V[(uint32_t)max(face_brick<-sx, -sy, -sz>(x, y, z, nb), 0) * BC + l1];
                 gnb[2 * ax + 1] = V[(uint32_t)max(face_brick<sx, sy, sz>(x, y, z, nb), 0) * BC + l2];
             });
-        if constexpr (STAGED) stage_copy<PAR, false, !BND>(F, stage, nb, base, L, ra);
+        if constexpr (EARLY_OWN) stage_copy<PAR, false, true, 2>(F, stage, nb, base, L, ra);
+        else if constexpr (STAGED) stage_copy<PAR, false, !BND>(F, stage, nb, base, L, ra);
     }
     load_table<W>(slots32, ODD_STAGED ? kOddPos : kSlots, wave, L);
     if constexpr (BND && PAR == 1) load_table<W>(pull32, kPullSrc, wave, L);
