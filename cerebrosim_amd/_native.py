@@ -119,7 +119,7 @@ class BricksStepArgs(ctypes.Structure):
                 ("rho_old", c_void_p), ("n_slots", c_i64), ("force", c_float * 3), ("magic", c_float),
                 ("bouzidi", c_i32), ("model", c_i32), ("galilean", c_i32), ("parity", c_i32), ("boundary", c_i32),
                 ("storage", c_i32), ("case_omega", c_void_p), ("case_force", c_void_p), ("p_out_stride", c_i32),
-                ("variant", c_i32), ("brec", c_void_p)]
+                ("variant", c_i32), ("brec", c_void_p), ("hybrid", c_i32), ("sigma", c_float)]
 
 
 class BricksConvertArgs(ctypes.Structure):

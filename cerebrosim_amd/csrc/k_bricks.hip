@@ -91,6 +91,9 @@ static int check_step(const cs_bricks_step_args* a) {
     CS_REQUIRE(a->storage == CS_STORAGE_FP32 || a->storage == CS_STORAGE_FP16, "cs_bricks_step: storage %d",
                a->storage);
     CS_REQUIRE(a->vel_old != a->vel, "cs_bricks_step: vel_old and vel must be distinct buffers");
+    CS_REQUIRE(!a->hybrid || (a->model == CS_MODEL_REGULARISED && a->storage == CS_STORAGE_FP32 && a->sigma >= 0.0f &&
+                              a->sigma < 1.0f),
+               "cs_bricks_step: the hybrid blend needs the regularised model, FP32 storage and sigma in [0, 1)");
     CS_REQUIRE(a->storage == CS_STORAGE_FP16 || a->rho, "cs_bricks_step: FP32 storage needs rho");
     CS_REQUIRE(a->n_slots > 0 && a->n_slots % cs::BC == 0, "cs_bricks_step: n_slots %lld", (long long)a->n_slots);
     CS_REQUIRE(a->n_slots * cs::Q < ((int64_t)1 << 32), "cs_bricks_step: %lld slots exceed 32-bit population offsets",
@@ -129,8 +132,9 @@ extern "C" int cs_bricks_step_pair(const cs_bricks_step_args* boundary, const cs
     if (rc != CS_OK) return rc;
     CS_REQUIRE(boundary->parity == interior->parity && boundary->storage == interior->storage &&
                    boundary->model == interior->model && boundary->galilean == interior->galilean &&
-                   boundary->variant == interior->variant,
-               "cs_bricks_step_pair: the two blocks differ in parity, storage, model, galilean flag or variant");
+                   boundary->variant == interior->variant && boundary->hybrid == interior->hybrid &&
+                   (!boundary->hybrid || boundary->sigma == interior->sigma),
+               "cs_bricks_step_pair: the two blocks differ in parity, storage, model, galilean flag, variant or blend");
     const cs::Variant v = cs::kVariant[boundary->variant];
     hipStream_t st = cs::as_stream(stream);
     const hipError_t e = v.waves == 4 ? cs::bricks_launch_pair_w4(*boundary, *interior, st, v.xcd)

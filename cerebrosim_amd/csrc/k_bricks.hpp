@@ -601,11 +601,16 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
     // form (every formula is linear in the populations; the Bouzidi side copies stay actual populations), and the
     // collision writes the stored form directly (collide<1, false, true, true>).
     constexpr bool FUSED16 = H16 && MODEL == 1;
+    // MODEL 2: the regularised collision with the hybrid blend (sigma < 1, lbm.py:224-229), FP32 storage only; it
+    // needs the full velocity gradient of the previous step
+    static_assert(MODEL != 2 || !H16, "the hybrid blend runs in FP32 storage");
+    constexpr bool HYB = MODEL == 2;
+    constexpr int CM = HYB ? 1 : MODEL;  // the collision model
     // per-lane lookups of the neighbour ids (odd boundary gathers, the float16 moment halo) read them from LDS
     constexpr bool LDS_TBL = H16 || (BND && PAR == 1);
     // FP32 + GAL: the neighbour velocities of the gradient stencil are loaded with the populations, except in odd
     // boundary steps (their direct gathers hold more registers: holding these too would lower the occupancy)
-    constexpr bool HOIST_GRAD = !H16 && GAL && (PAR == 0 || !BND);
+    constexpr bool HOIST_GRAD = !H16 && GAL && !HYB && (PAR == 0 || !BND);
     constexpr bool EARLY_OWN = !BND && !H16;
     static_assert(Lds::TBL == LDS_TBL && Lds::STAGED == STAGED, "LDS layout");
     // the odd-step record addresses are computed from compile-time runs, so that five workgroups of four waves fit
@@ -819,7 +824,7 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
         const float om = own_om ? (BND ? om_c : a.om_cell[slot]) : om_k;
         const float* p_out = a.p_out + (int64_t)cs * a.p_out_stride;
         // ---- boundary links and the previous velocity --------------------------------------------------------
-        float q16[6];  // BND + GAL: qc of links 1..6 (read by the gradient stencil), loaded with the link batch
+        float q16[6];  // BND + GAL or HYB: qc of links 1..6 (read by the gradient stencil), loaded with the link batch
         if constexpr (BND) {
             if ((mask & kAllLinks) != kAllLinks) {
                 // compact link data: each lane's links in direction order from its offset in the brick's run, so
@@ -852,7 +857,7 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
                         qv[k] = ((wq >> (h0 + k)) & 1u) ? ql[qo + (uint32_t)__popc(miss & below)] : 0.0f;
                         pv[k] = ((ws >> (h0 + k)) & 1u) ? sd[so + (uint32_t)__popc(sm & below)] : 0.0f;
                     });
-                    if constexpr (GAL && h0 == 1)
+                    if constexpr ((GAL || HYB) && h0 == 1)
                         static_for<0, 6>([&](auto kc) { q16[decltype(kc)::value] = qv[decltype(kc)::value]; });
                     static_for<0, nl / 2>([&](auto pc) {
                         constexpr int k = 2 * decltype(pc)::value;
@@ -875,7 +880,7 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
                         f[j] = vj;
                     });
                 });
-            } else if constexpr (GAL) {
+            } else if constexpr (GAL || HYB) {
                 static_for<0, 6>([&](auto kc) { q16[decltype(kc)::value] = 0.0f; });
             }
         }
@@ -889,9 +894,52 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
         p.F[1] = force[1] * d;
         p.F[2] = force[2] * d;
         p.t3 = GAL ? 1.0f : 0.0f;
-        p.sigma = 1.0f;
+        p.sigma = HYB ? a.sigma : 1.0f;
         p.g[0] = p.g[1] = p.g[2] = 0.0f;
-        if constexpr (GAL) {
+        if constexpr (HYB) {
+            // full gradient G[ax][c] = d u_c / d x_ax of the previous velocity (lbm.py:105-126 for every component):
+            // the same one-sided rules as below, per component (a wall link at distance max(q, 0.1) with zero
+            // velocity, an open link the cell's own)
+            float uo[3] = {up[0], up[1], up[2]};
+            if constexpr (!BND) {
+                uo[0] = a.vel_old[slot];
+                uo[1] = a.vel_old[S + slot];
+                uo[2] = a.vel_old[2 * S + slot];
+            }
+            static_for<0, 3>([&](auto ac) {
+                constexpr int ax = decltype(ac)::value;
+                constexpr int sx = ax == 0, sy = ax == 1, sz = ax == 2;
+                constexpr int i1 = 2 * ax + 1, i2 = 2 * ax + 2;
+                int d1, l1, d2, l2;
+                shift_pos<-sx, -sy, -sz>(x, y, z, d1, l1);
+                shift_pos<sx, sy, sz>(x, y, z, d2, l2);
+                const uint32_t s1 = (uint32_t)max(face_brick<-sx, -sy, -sz>(x, y, z, nb), 0) * BC + l1;
+                const uint32_t s2 = (uint32_t)max(face_brick<sx, sy, sz>(x, y, z, nb), 0) * BC + l2;
+                bool fl1 = true, fl2 = true, w1 = false, w2 = false;
+                float h1 = 1.0f, h2 = 1.0f;
+                if constexpr (BND) {
+                    fl1 = (mask >> i1) & 1u;
+                    fl2 = (mask >> i2) & 1u;
+                    w1 = !fl1 && q16[i1 - 1] >= 0.0f;
+                    w2 = !fl2 && q16[i2 - 1] >= 0.0f;
+                    h1 = w1 ? fmaxf(q16[i1 - 1], 0.1f) : 1.0f;
+                    h2 = w2 ? fmaxf(q16[i2 - 1], 0.1f) : 1.0f;
+                }
+                const float r = __builtin_amdgcn_rcpf(h1 * h2 * (h1 + h2));
+                static_for<0, 3>([&](auto cc) {
+                    constexpr int c = decltype(cc)::value;
+                    const float n1 = a.vel_old[c * S + s1], n2 = a.vel_old[c * S + s2];
+                    const float v1 = fl1 ? n1 : (w1 ? 0.0f : uo[c]);
+                    const float v2 = fl2 ? n2 : (w2 ? 0.0f : uo[c]);
+                    p.G[ax][c] = BND ? (h1 * h1 * (v2 - uo[c]) + h2 * h2 * (uo[c] - v1)) * r : (v2 - v1) * 0.5f;
+                });
+            });
+            if constexpr (GAL) {
+                p.g[0] = p.G[0][0];
+                p.g[1] = p.G[1][1];
+                p.g[2] = p.G[2][2];
+            }
+        } else if constexpr (GAL) {
             static_for<0, 3>([&](auto ac) {
                 constexpr int ax = decltype(ac)::value;
                 constexpr int i1 = 2 * ax + 1, i2 = 2 * ax + 2;
@@ -939,7 +987,7 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
         float dr, u[3];
         bool inlet_cell = false;
         if constexpr (BND) inlet_cell = (mask & kInletBit) != 0u;
-        if constexpr (MODEL == 1 && BND) {
+        if constexpr (CM == 1 && BND) {
             // inlet cells: the regularised collision at the prescribed velocity with om = 1, F = 0, g = 0
             float ut[3] = {0.0f, 0.0f, 0.0f};
             if (inlet_cell) {
@@ -951,7 +999,7 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
                 p.g[0] = p.g[1] = p.g[2] = 0.0f;
             }
             if constexpr (FUSED16) collide<1, false, true, true>(f, p, dr, u, inlet_cell, ut[0], ut[1], ut[2], kH16Scale);
-            else collide<MODEL, false>(f, p, dr, u, inlet_cell, ut[0], ut[1], ut[2]);
+            else collide<CM, HYB>(f, p, dr, u, inlet_cell, ut[0], ut[1], ut[2]);
         } else if (inlet_cell) {
             dr = density_deviation(f);
             u[0] = a.uin[((uint32_t)bb * 3 + 0) * BC + L] * d;
@@ -961,7 +1009,7 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
         } else if constexpr (FUSED16) {
             collide<1, false, true, true>(f, p, dr, u, false, 0.0f, 0.0f, 0.0f, kH16Scale);
         } else {
-            collide<MODEL, false>(f, p, dr, u);
+            collide<CM, HYB>(f, p, dr, u);
         }
 
         // ---- outputs -------------------------------------------------------------------------------------------
@@ -1093,7 +1141,10 @@ hipError_t launch_par(const cs_bricks_step_args& a, hipStream_t s, int xcd) {
     const int64_t groups = (a.n_list + W - 1) / W;
     const unsigned grid = (unsigned)(((groups + 7) / 8) * 8);
     const bool h = a.storage == CS_STORAGE_FP16;
-    if (a.parity == 0) {
+    if constexpr (MODEL == 2) {  // the hybrid blend: FP32 storage only (checked by the caller)
+        if (a.parity == 0) k_bricks<W, MODEL, GAL, BND, 0, false><<<grid, 64 * W, 0, s>>>(a, xcd);
+        else k_bricks<W, MODEL, GAL, BND, 1, false><<<grid, 64 * W, 0, s>>>(a, xcd);
+    } else if (a.parity == 0) {
         if (h) k_bricks<W, MODEL, GAL, BND, 0, true><<<grid, 64 * W, 0, s>>>(a, xcd);
         else k_bricks<W, MODEL, GAL, BND, 0, false><<<grid, 64 * W, 0, s>>>(a, xcd);
     } else {
@@ -1112,7 +1163,8 @@ hipError_t launch_model(const cs_bricks_step_args& a, hipStream_t s, int xcd) {
 
 template <int W>
 hipError_t launch_bricks(const cs_bricks_step_args& a, hipStream_t s, int xcd) {
-    return a.model == CS_MODEL_REGULARISED ? launch_model<W, 1>(a, s, xcd) : launch_model<W, 0>(a, s, xcd);
+    if (a.model != CS_MODEL_REGULARISED) return launch_model<W, 0>(a, s, xcd);
+    return a.hybrid ? launch_model<W, 2>(a, s, xcd) : launch_model<W, 1>(a, s, xcd);
 }
 
 // The paired launch: both lists non-empty, same model, Galilean flag, parity and storage (checked by the caller).
@@ -1121,7 +1173,10 @@ hipError_t launch_pair_par(const cs_bricks_step_args& ab, const cs_bricks_step_a
     const int64_t gb = ((ab.n_list + W - 1) / W + 7) / 8 * 8, gi = ((ai.n_list + W - 1) / W + 7) / 8 * 8;
     const unsigned grid = (unsigned)(gb + gi);
     const bool h = ab.storage == CS_STORAGE_FP16;
-    if (ab.parity == 0) {
+    if constexpr (MODEL == 2) {  // the hybrid blend: FP32 storage only (checked by the caller)
+        if (ab.parity == 0) k_bricks_pair<W, MODEL, GAL, 0, false><<<grid, 64 * W, 0, s>>>(ab, ai, (int)gb, xcd);
+        else k_bricks_pair<W, MODEL, GAL, 1, false><<<grid, 64 * W, 0, s>>>(ab, ai, (int)gb, xcd);
+    } else if (ab.parity == 0) {
         if (h) k_bricks_pair<W, MODEL, GAL, 0, true><<<grid, 64 * W, 0, s>>>(ab, ai, (int)gb, xcd);
         else k_bricks_pair<W, MODEL, GAL, 0, false><<<grid, 64 * W, 0, s>>>(ab, ai, (int)gb, xcd);
     } else {
@@ -1133,6 +1188,8 @@ hipError_t launch_pair_par(const cs_bricks_step_args& ab, const cs_bricks_step_a
 
 template <int W>
 hipError_t launch_bricks_pair(const cs_bricks_step_args& ab, const cs_bricks_step_args& ai, hipStream_t s, int xcd) {
+    if (ab.model == CS_MODEL_REGULARISED && ab.hybrid)
+        return ab.galilean ? launch_pair_par<W, 2, true>(ab, ai, s, xcd) : launch_pair_par<W, 2, false>(ab, ai, s, xcd);
     if (ab.model == CS_MODEL_REGULARISED)
         return ab.galilean ? launch_pair_par<W, 1, true>(ab, ai, s, xcd) : launch_pair_par<W, 1, false>(ab, ai, s, xcd);
     return ab.galilean ? launch_pair_par<W, 0, true>(ab, ai, s, xcd) : launch_pair_par<W, 0, false>(ab, ai, s, xcd);

@@ -254,6 +254,7 @@ class _BricksBackend:
         A.force = (ctypes.c_float * 3)(*s.force)
         A.magic = s.magic
         A.bouzidi, A.model, A.galilean = s.bouzidi, s.model, s.gal
+        A.hybrid, A.sigma = int(s.model == 1 and s.sigma < 1.0), s.sigma
         A.parity, A.boundary = a, int(boundary)
         A.variant = s.variant
         return A
@@ -367,8 +368,8 @@ class Solver:
     ``sigma`` < 1 blends the regularised stress with the finite-difference one. ``layout`` selects the device
     storage: "bricks" (4^3 bricks, AA in-place streaming), "indexed" (reference (Q, N) layout, any neighbour table)
     or "auto" (bricks when the lattice and options allow, else indexed). ``storage`` "fp16" keeps the populations
-    in float16 (bricks layout only; float32 arithmetic and reductions). ``brick_layout``: the host layout of
-    ``bricks.build(lat)`` when the caller already has it."""
+    in float16 (bricks layout only, not with the hybrid blend; float32 arithmetic and reductions).
+    ``brick_layout``: the host layout of ``bricks.build(lat)`` when the caller already has it."""
 
     def __init__(self, lat, nu_lu, magic=3.0 / 16.0, bouzidi=True, model="trt", sigma=1.0, galilean=None,
                  device=None, layout="auto", storage="fp32", brick_layout=None):
@@ -419,8 +420,8 @@ class Solver:
                 raise ValueError("storage 'fp16' needs the bricks layout")
             return _IndexedBackend(self, lat)
         reason = None
-        if self.model == 1 and self.sigma < 1.0:
-            reason = "the hybrid blend (sigma < 1) runs on the indexed layout"
+        if self.model == 1 and self.sigma < 1.0 and self.storage != "fp32":
+            reason = "the hybrid blend (sigma < 1) runs in FP32 storage"
         else:
             try:
                 lay = brick_layout if brick_layout is not None else bricks.build(lat)

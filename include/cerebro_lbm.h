@@ -287,6 +287,9 @@ typedef struct cs_bricks_step_args {
     int32_t variant;            /* launch variant, 0 .. cs_bricks_variant_info(-1) - 1 (0 = default) */
     const int32_t* brec;        /* boundary lists: record of each entry (NULL: the entry's position), so that a list
                                    can hold its live entries only while records keep their places */
+    int32_t hybrid;             /* regularised model, FP32 storage: 1 = blend the stress with the finite-difference one
+                                   by sigma (lbm.py:224-229); 0 (a zeroed struct) = pure regularised */
+    float sigma;                /* the blend when hybrid = 1, in [0, 1) */
 } cs_bricks_step_args;
 int cs_bricks_step(const cs_bricks_step_args* a, void* stream);
 /* One step's boundary and interior lists in a single launch (replaces the two k_step launches of a step's bricks,

@@ -44,13 +44,8 @@ def test_golden_vectors(layout, case):
     lat = mf.lattice()
     assert np.array_equal(lat["nbr"], g["nbr"]) and np.array_equal(lat["q"], g["q"])
     model, sigma, bz, steps = mf.CASES[case]
-    if layout == "bricks" and sigma < 1.0:
-        with pytest.raises(ValueError, match="hybrid"):
-            lbm.Solver(lat, mf.NU, model=model, sigma=sigma, bouzidi=bz, layout=layout)
-        s = lbm.Solver(lat, mf.NU, model=model, sigma=sigma, bouzidi=bz, layout="auto")
-        assert s.layout == "indexed"          # the hybrid blend runs on the indexed kernel
-    else:
-        s = lbm.Solver(lat, mf.NU, model=model, sigma=sigma, bouzidi=bz, layout=layout)
+    s = lbm.Solver(lat, mf.NU, model=model, sigma=sigma, bouzidi=bz, layout=layout)
+    assert s.layout == layout
     mf.setup(s)
     s.set_f(g["f0"])
     s.set_velocity(g["u0"])
@@ -62,6 +57,13 @@ def test_golden_vectors(layout, case):
         errs = (rel(s.get_f(), g[f"{case}_f_{n}"]), rel(3 * p, g[f"{case}_rho_{n}"]), rel(u, g[f"{case}_u_{n}"]))
         print(f"{layout} {case} step {n}: rel L2 f/rho/u {errs}", flush=True)
         assert max(errs) <= TOL, errs
+
+
+def test_hybrid_blend_needs_fp32_storage():
+    """The hybrid blend (sigma < 1) runs on the bricks in FP32 storage; FP16 storage refuses it."""
+    mf, _ = golden()
+    with pytest.raises(ValueError, match="hybrid"):
+        lbm.Solver(mf.lattice(), mf.NU, model="regularised", sigma=0.8, layout="bricks", storage="fp16")
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
