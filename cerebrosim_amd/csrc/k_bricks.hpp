@@ -900,12 +900,8 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
             // full gradient G[ax][c] = d u_c / d x_ax of the previous velocity (lbm.py:105-126 for every component):
             // the same one-sided rules as below, per component (a wall link at distance max(q, 0.1) with zero
             // velocity, an open link the cell's own)
-            float uo[3] = {up[0], up[1], up[2]};
-            if constexpr (!BND) {
-                uo[0] = a.vel_old[slot];
-                uo[1] = a.vel_old[S + slot];
-                uo[2] = a.vel_old[2 * S + slot];
-            }
+            // (interior cells: both neighbours are fluid, h = 1, and the difference quotient needs no own velocity;
+            // boundary cells have theirs in up)
             static_for<0, 3>([&](auto ac) {
                 constexpr int ax = decltype(ac)::value;
                 constexpr int sx = ax == 0, sy = ax == 1, sz = ax == 2;
@@ -929,9 +925,9 @@ __device__ __forceinline__ void brick_step(const cs_bricks_step_args& a, int wg,
                 static_for<0, 3>([&](auto cc) {
                     constexpr int c = decltype(cc)::value;
                     const float n1 = a.vel_old[c * S + s1], n2 = a.vel_old[c * S + s2];
-                    const float v1 = fl1 ? n1 : (w1 ? 0.0f : uo[c]);
-                    const float v2 = fl2 ? n2 : (w2 ? 0.0f : uo[c]);
-                    p.G[ax][c] = BND ? (h1 * h1 * (v2 - uo[c]) + h2 * h2 * (uo[c] - v1)) * r : (v2 - v1) * 0.5f;
+                    const float v1 = fl1 ? n1 : (w1 ? 0.0f : up[c]);
+                    const float v2 = fl2 ? n2 : (w2 ? 0.0f : up[c]);
+                    p.G[ax][c] = BND ? (h1 * h1 * (v2 - up[c]) + h2 * h2 * (up[c] - v1)) * r : (v2 - v1) * 0.5f;
                 });
             });
             if constexpr (GAL) {
